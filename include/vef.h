@@ -1,0 +1,126 @@
+/*
+ * vef.h — C ABI of the MI355X residual/Jacobian assembler for the 2D
+ * square-root-conformation viscoelastic flow past a cylinder (9-node Q2
+ * quadrangles, unknowns Ux Uy P Sxx Sxy Syy per node plus one arclength
+ * unknown).
+ *
+ * The assembler replaces, behind one call, the reference's per-Newton-iteration
+ * assembly sequence
+ *     initializeLSObjects        (src/Petsc_mod.F90:747-763: zero b, and A if NRP)
+ *     setEquationsBulk           (src/Equations_mod.F90:21-55, called at
+ *                                 src/Newton_Raphson_mod.F90:332-333)
+ *     setEquationsEdge (Outlet)  (src/Equations_mod.F90:317-342, called at
+ *                                 src/Newton_Raphson_mod.F90:349-350)
+ * i.e. the element residuals, the forward-difference element Jacobians
+ * (eps_jac = 1e-8, src/Equations_mod.F90:264-306) and their ADD_VALUES scatter
+ * (src/Storage_mod.F90:10-115) into the AIJ pattern that
+ * preAllocatePetsc builds (src/Petsc_mod.F90:629-743).  The caller keeps the
+ * rest of assemblePetscObjects (arclength row, Dirichlet rows) and the solve.
+ *
+ * Conventions
+ *   - All indices are 0-based.  Global unknown of node g, equation q is
+ *     row/column 6*g + q (q: 0 Ux, 1 Uy, 2 P, 3 Sxx, 4 Sxy, 5 Syy); the
+ *     arclength unknown is the last one, N-1, with N = 6*n_nodes + 1
+ *     (getRows, src/Tools_mod.F90:735-752; storeJacobian_Ac jcol = Mcols-1).
+ *   - The state is the solution vector x[N] in that ordering: x[6g+q] is
+ *     TL(g,q) and x[N-1] is EU(Nex), the arclength parameter (WiN = WiN0*EU).
+ *   - The matrix is CSR with int64 row pointers and int32 column indices,
+ *     rows sorted; values are FP64.
+ *   - Functions return 0 on success and a nonzero VEF_ERR_* code otherwise;
+ *     vef_last_error() gives the message.  Nothing aborts the process.
+ *   - One handle is used by one host thread at a time.  Work is ordered on the
+ *     HIP stream passed in (NULL = the default stream).
+ */
+#ifndef VEF_H
+#define VEF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vef_handle vef_handle;
+
+enum { VEF_MODEL_LPTT = 0, VEF_MODEL_GIESEKUS = 1 };
+/* Flag_NR: 'MNR' residual only, 'NRP' residual and Jacobian (src/Equations_mod.F90:264) */
+enum { VEF_RES_ONLY = 0, VEF_RES_JAC = 1 };
+enum {
+  VEF_OK = 0,
+  VEF_ERR_ARG = 1,
+  VEF_ERR_MESH = 2,
+  VEF_ERR_HIP = 3,
+  VEF_ERR_UNSUPPORTED = 4,
+  VEF_ERR_NOMEM = 5
+};
+
+/* Dimensionless numbers (src/Parameters_mod.F90:98-111, src/Physical_mod.F90:327-351).
+ * model VEF_MODEL_LPTT: VeN is the PTT epsilon (VeN = 0 is Oldroyd-B);
+ * model VEF_MODEL_GIESEKUS: VeN is the mobility alpha.  DfN > 1e-12 switches
+ * the discontinuity-capturing term off and adds DfN*grad(S):grad(phi). */
+typedef struct {
+  double ReN, BvN, VeN, DfN, WiN0;
+  int32_t model;
+  int32_t reserved; /* must be 0 */
+} vef_params;
+
+/* Mesh: coords[n_nodes][2] (x, y); connectivity[n_elements][9] in the
+ * reference's local order (corners (-1,-1) (1,-1) (1,1) (-1,1), mid-edges
+ * bottom right top left, centre; src/GaussIntegration/Gauss_Quadrangle_mod.F90:413-421);
+ * outlet_edges[n_outlet_edges][2] = (element, edge id 1..4) ascending, the
+ * edge ids of setEdgeQuantities (src/Element_Calculations_mod.F90:887-905).
+ * Builds the CSR pattern and the device-side scatter map; copies the mesh.
+ * Replaces: createPetscMatrix + preAllocatePetsc (src/Petsc_mod.F90:290-339,629-743)
+ * and the set-up of the Gauss tables (src/GaussIntegration/Gauss_Quadrangle_mod.F90:14-34). */
+int vef_create(int32_t n_nodes, int32_t n_elements, const double* coords, const int32_t* connectivity,
+               int32_t n_outlet_edges, const int32_t* outlet_edges, const vef_params* params,
+               vef_handle** out);
+
+/* Replaces changing the module globals ReN, BvN, VeN, DfN and WiN0 between
+ * continuation steps (src/Physical_mod.F90:327-351). */
+int vef_set_params(vef_handle* h, const vef_params* params);
+
+/* N = 6*n_nodes + 1 and the number of stored entries of the pattern. */
+int64_t vef_num_rows(const vef_handle* h);
+int64_t vef_nnz(const vef_handle* h);
+
+/* Copies the CSR pattern to host arrays rowptr[N+1], colind[nnz]
+ * (MatGetRowIJ of the preallocated matrix). */
+int vef_get_pattern(const vef_handle* h, int64_t* rowptr, int32_t* colind);
+
+/* Device copies of the same pattern, owned by the handle. */
+const int64_t* vef_device_rowptr(const vef_handle* h);
+const int32_t* vef_device_colind(const vef_handle* h);
+
+/* One assembly on the GPU, all pointers device pointers.
+ *   x[N]       state (TL node-major, EU last)
+ *   b[N]       residual: zeroed, then element and outlet-edge contributions
+ *              added in the reference's ADD_VALUES order
+ *   values[nnz] matrix values (only written when flag == VEF_RES_JAC):
+ *              zeroed, then the forward-difference element Jacobians added in
+ *              the same order; column N-1 holds dR/dEU.  Row N-1 stays zero
+ *              (the caller's arclength row goes there).
+ * Replaces initializeLSObjects + setEquationsBulk + setEquationsEdge for the
+ * Cylinder_2D outlet (src/Newton_Raphson_mod.F90:101,332-357). */
+int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* hip_stream);
+
+/* The same with host arrays (copies in, assembles, copies out). */
+int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values);
+
+/* Raw forward-difference bank of the last vef_assemble: for element e, row
+ * j = 6*inod + q of the local residual and evaluation k (0: base state,
+ * 1..54: unknown k-1 perturbed, 55: EU perturbed), the residual R_k(j) is at
+ * bank[(e*54 + j)*56 + k].  Copies n_elements*54*56 doubles to host. */
+int vef_copy_residual_bank(const vef_handle* h, double* bank_host);
+
+/* Last error message of the handle (or of the last failed vef_create when h
+ * is NULL). */
+const char* vef_last_error(const vef_handle* h);
+
+void vef_destroy(vef_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VEF_H */

@@ -1,0 +1,164 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle (libvef_oracle.so).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as
+the checker.  See vef_oracle.cpp for what it restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DBL_MAX = float(np.finfo(np.float64).max)
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [("ReN", ctypes.c_double), ("BvN", ctypes.c_double), ("VeN", ctypes.c_double),
+                ("DfN", ctypes.c_double), ("WiN0", ctypes.c_double), ("dt", ctypes.c_double),
+                ("model", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+def params(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1.0e-3, WiN0=1.0, model="L-PTT") -> OracleParams:
+    m = {"L-PTT": 0, "GIESEKUS": 1}[model]
+    return OracleParams(ReN, BvN, VeN, DfN, WiN0, DBL_MAX, m, 0)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_libs = {}
+
+
+def lib(flops: bool = False):
+    name = "libvef_oracle_flops.so" if flops else "libvef_oracle.so"
+    if name not in _libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        if flops:
+            L.vo_count_flops.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, vp]
+        else:
+            L.vo_tables.argtypes = [vp]
+            L.vo_basis2d.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+            L.vo_gauss_fields.argtypes = [vp, vp, ctypes.c_int, vp]
+            L.vo_point_strong_form.argtypes = [ctypes.POINTER(OracleParams), vp, vp, vp, ctypes.c_double, vp, vp, vp,
+                                               ctypes.c_double, vp]
+            L.vo_element.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     vp, vp, vp]
+            L.vo_pattern_nnz.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+            L.vo_pattern_nnz.restype = ctypes.c_int64
+            L.vo_pattern.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp]
+            L.vo_assemble.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, vp,
+                                      vp, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
+        _libs[name] = L
+    return _libs[name]
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def tables():
+    out = np.zeros(6 + 18 + 9 + 6 * 81 + 6 * 108)
+    lib().vo_tables(_p(out))
+    o = 0
+    res = {}
+    for name, size in (("P1", 3), ("W1", 3)):
+        res[name] = out[o:o + size]
+        o += size
+    res["P2"] = out[o:o + 18].reshape(9, 2)
+    o += 18
+    res["W2"] = out[o:o + 9]
+    o += 9
+    for name in ("bfn", "dfdc", "dfde", "d2fdc2", "d2fde2", "d2fdce"):
+        res[name] = out[o:o + 81].reshape(9, 9)  # [gp][node]
+        o += 81
+    for name in ("bfnE", "dfdcE", "dfdeE", "d2fdc2E", "d2fde2E", "d2fdceE"):
+        res[name] = out[o:o + 108].reshape(4, 3, 9)
+        o += 108
+    return res
+
+
+def basis2d(xy, gp, edge=0):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    out = np.zeros(38)
+    lib().vo_basis2d(_p(xy), gp, edge, _p(out))
+    return dict(detJ=out[0], WET=out[1], dFdXi=out[2:20].reshape(2, 9), d2FdXi2=out[20:38].reshape(2, 9))
+
+
+def gauss_fields(xy, TL, gp):
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    TL = np.ascontiguousarray(TL, dtype=np.float64)
+    out = np.zeros(22)
+    lib().vo_gauss_fields(_p(xy), _p(TL), gp, _p(out))
+    return dict(U=out[0:2], GU=out[2:6].reshape(2, 2), G2U=out[6:10].reshape(2, 2), P=out[10], dP=out[11:13],
+                S=out[13:16], dS=out[16:22].reshape(3, 2))
+
+
+def point_strong_form(prm, U, GU, G2U, P, dP, S, dS, EU):
+    """Strong-form quantities at a point from explicit field values.
+
+    GU[j][i] = dU_i/dx_j, G2U[j][i] = d2U_i/dx_j^2, S = (Sxx, Sxy, Syy),
+    dS[s][d] = dS_s/dx_d."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (U, GU, G2U, dP, S, dS)]
+    out = np.zeros(40)
+    lib().vo_point_strong_form(ctypes.byref(prm), _p(a[0]), _p(a[1]), _p(a[2]), float(P), _p(a[3]), _p(a[4]),
+                               _p(a[5]), float(EU), _p(out))
+    return dict(Con_EQ=out[0:4].reshape(2, 2), Momentum=out[4:6], Continuity=out[6], T_ve=out[7:16].reshape(3, 3),
+                P_tot=out[16:25].reshape(3, 3), W12=out[25], UCD=out[26:30].reshape(2, 2), f_T=out[30],
+                C=out[31:35].reshape(2, 2))
+
+
+def element(prm, xy, TL, EU, ied=0, jac=True):
+    """Res[9][6], A[54][54] (A[(inod,ieq)][(jnod,jeq)]), Ac[54] of one element."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    TL = np.ascontiguousarray(TL, dtype=np.float64)
+    Res = np.zeros((9, 6))
+    A = np.zeros((54, 54)) if jac else None
+    Ac = np.zeros(54) if jac else None
+    lib().vo_element(ctypes.byref(prm), _p(xy), _p(TL), float(EU), ied, 1 if jac else 0, _p(Res), _p(A), _p(Ac))
+    return Res, A, Ac
+
+
+def count_flops(prm, xy, TL, EU, ied=0):
+    out = np.zeros(5, dtype=np.uint64)
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    TL = np.ascontiguousarray(TL, dtype=np.float64)
+    lib(flops=True).vo_count_flops(ctypes.byref(prm), _p(xy), _p(TL), float(EU), ied, _p(out))
+    return dict(add=int(out[0]), mul=int(out[1]), div=int(out[2]), sqrt=int(out[3]), total=int(out[4]))
+
+
+def pattern(mesh):
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    nnz = lib().vo_pattern_nnz(mesh.n_nodes, mesh.n_elements, _p(conn))
+    rowptr = np.zeros(mesh.n_unknowns + 1, dtype=np.int64)
+    col = np.zeros(nnz, dtype=np.int32)
+    lib().vo_pattern(mesh.n_nodes, mesh.n_elements, _p(conn), _p(rowptr), _p(col))
+    return rowptr, col
+
+
+def assemble(prm, mesh, x, jac=True, rowptr=None, col=None, el_range=None, nthreads=0):
+    """Reference-order assembly: returns (b[N], values[nnz])."""
+    if rowptr is None:
+        rowptr, col = pattern(mesh)
+    coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    edges = np.ascontiguousarray(mesh.outlet_edges, dtype=np.int32).reshape(-1, 2)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    N = mesh.n_unknowns
+    b = np.zeros(N)
+    vals = np.zeros(int(rowptr[-1])) if jac else None
+    e0, e1 = (0, mesh.n_elements) if el_range is None else el_range
+    rc = lib().vo_assemble(ctypes.byref(prm), mesh.n_nodes, mesh.n_elements, _p(coords), _p(conn), edges.shape[0],
+                           _p(edges), _p(x[:-1]), float(x[-1]), 1 if jac else 0, _p(rowptr), _p(col), _p(b), _p(vals),
+                           e0, e1, nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle assembly failed: pattern mismatch")
+    return b, vals

@@ -1,0 +1,1122 @@
+// =============================================================================
+// TEST INFRASTRUCTURE ONLY — the CPU oracle.
+//
+// A plain C++ restatement of the reference's element-level residual and
+// finite-difference Jacobian assembly for the 2D square-root-conformation
+// viscoelastic cylinder problem (Cylinder_2D, 9-node quadratic quadrangles,
+// Stress_Reform='SQRT', Continuation_Method='Arclength').  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+// as the checker.  The product (the HIP library) never calls into this file.
+//
+// It follows the reference line by line, keeping its data shapes (3x3 tensors
+// whose third row/column is zero in 2D, Fortran's sequential dot_product /
+// matmul / traceTensor / doubleDotProduct summation order, left-to-right
+// evaluation) so that it reproduces the reference's FP64 operation sequence:
+//   src/GaussIntegration/Gauss_Quadrangle_mod.F90:54-101,133-198,243-372,405-478
+//   src/Element_Calculations_mod.F90:106-142,308-361,449-533,537-634,658-773,852-924
+//   src/Constitutive_Models_SQRT_mod.F90:120-180,285-305,379-530,534-584
+//   src/Residuals_mod.F90:161-187,223-234,295-490,599-624
+//   src/Equations_mod.F90:21-144,226-313,317-554
+//   src/Storage_mod.F90:10-115, src/Tools_mod.F90:129-146,206-340,735-855
+//   src/Petsc_mod.F90:629-743 (matrix pattern)
+//
+// Scalar semantics fixed here (the reference is compiled with PetscScalar =
+// complex(8); all imaginary parts are zero on this path):
+//   * complex add/sub/mul/sqrt with zero imaginary parts equal the real ops;
+//   * every division with a complex operand rounds as (a*c)/(c*c) (checked for
+//     flang 22 on 2e5 random operands for complex/complex, complex/real and
+//     real/complex), written cdiv() below; real/real divisions stay a/c;
+//   * no FMA contraction (build with -ffp-contract=off), IEEE div and sqrt.
+// =============================================================================
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifdef VEF_COUNT_FLOPS
+#include "flop_real.hpp"
+typedef FlopReal vreal;
+#else
+typedef double vreal;
+static inline double vsqrt(double a) { return std::sqrt(a); }
+static inline double fabsr(double a) { return std::fabs(a); }
+static inline double val(double a) { return a; }
+#endif
+
+static inline vreal Z() { return vreal(); }  // zero (structural zero when counting)
+
+// Division where either operand is PetscScalar (complex(8)) with zero
+// imaginary parts: flang evaluates re((a+0i)/(c+0i)) as (a*c)/(c*c).
+static inline vreal cdiv(vreal a, vreal c) { return (a * c) / (c * c); }
+
+extern "C" {
+
+// Physical parameters (Parameters_mod.F90:98-111, Physical_mod.F90:327-351).
+typedef struct {
+  double ReN, BvN, VeN, DfN, WiN0, dt;
+  int32_t model;  // 0: L-PTT (VeN = epsilon; VeN=0 is Oldroyd-B), 1: Giesekus (VeN = alpha)
+  int32_t pad;
+} vo_params;
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------
+// a1: Gauss tables, Gauss_Quadrangle_mod.F90
+// ----------------------------------------------------------------------------
+struct Tables {
+  double P1[3], W1[3];
+  double P2[9][2], W2[9];
+  // [gauss point][node]
+  double bfn[9][9], dfdc[9][9], dfde[9][9], d2fdc2[9][9], d2fde2[9][9], d2fdce[9][9];
+  // [edge 0..3][gauss point][node]
+  double bfnE[4][3][9], dfdcE[4][3][9], dfdeE[4][3][9], d2fdc2E[4][3][9], d2fde2E[4][3][9],
+      d2fdceE[4][3][9];
+};
+
+// setQuadraticQuadrangle, Gauss_Quadrangle_mod.F90:405-478
+static void quadratic_quadrangle(double c, double e, double* bfn, double* dfdc, double* dfde,
+                                 double* d2fdc2, double* d2fde2, double* d2fdce) {
+  bfn[0] = +(1.0 - c) * c * (1.0 - e) * e / 4.0;
+  bfn[1] = -((1.0 + c) * c * (1.0 - e) * e / 4.0);
+  bfn[2] = +(1.0 + c) * c * (1.0 + e) * e / 4.0;
+  bfn[3] = -((1.0 - c) * c * (1.0 + e) * e / 4.0);
+  bfn[4] = -((1.0 + c) * (1.0 - c) * (1.0 - e) * e / 2.0);
+  bfn[5] = +(1.0 + c) * c * (1.0 + e) * (1.0 - e) / 2.0;
+  bfn[6] = +(1.0 + c) * (1.0 - c) * (1.0 + e) * e / 2.0;
+  bfn[7] = -((1.0 - c) * c * (1.0 + e) * (1.0 - e) / 2.0);
+  bfn[8] = +(1.0 + c) * (1.0 - c) * (1.0 + e) * (1.0 - e);
+
+  dfdc[0] = +(1.0 - 2.0 * c) * (1.0 - e) * e / 4.0;
+  dfdc[1] = -((1.0 + 2.0 * c) * (1.0 - e) * e / 4.0);
+  dfdc[2] = +(1.0 + 2.0 * c) * (1.0 + e) * e / 4.0;
+  dfdc[3] = -((1.0 - 2.0 * c) * (1.0 + e) * e / 4.0);
+  dfdc[4] = -((-2.0 * c) * (1.0 - e) * e / 2.0);
+  dfdc[5] = +(1.0 + 2.0 * c) * (1.0 + e) * (1.0 - e) / 2.0;
+  dfdc[6] = +(-2.0 * c) * (1.0 + e) * e / 2.0;
+  dfdc[7] = -((1.0 - 2.0 * c) * (1.0 + e) * (1.0 - e) / 2.0);
+  dfdc[8] = +(-2.0 * c) * (1.0 + e) * (1.0 - e);
+
+  d2fdc2[0] = +(-2.0) * (1.0 - e) * e / 4.0;
+  d2fdc2[1] = -((+2.0) * (1.0 - e) * e / 4.0);
+  d2fdc2[2] = +(+2.0) * (1.0 + e) * e / 4.0;
+  d2fdc2[3] = -((-2.0) * (1.0 + e) * e / 4.0);
+  d2fdc2[4] = -((-2.0) * (1.0 - e) * e / 2.0);
+  d2fdc2[5] = +(+2.0) * (1.0 + e) * (1.0 - e) / 2.0;
+  d2fdc2[6] = +(-2.0) * (1.0 + e) * e / 2.0;
+  d2fdc2[7] = -((-2.0) * (1.0 + e) * (1.0 - e) / 2.0);
+  d2fdc2[8] = +(-2.0) * (1.0 + e) * (1.0 - e);
+
+  d2fdce[0] = +(1.0 - 2.0 * c) * (1.0 - 2.0 * e) / 4.0;
+  d2fdce[1] = -((1.0 + 2.0 * c) * (1.0 - 2.0 * e) / 4.0);
+  d2fdce[2] = +(1.0 + 2.0 * c) * (1.0 + 2.0 * e) / 4.0;
+  d2fdce[3] = -((1.0 - 2.0 * c) * (1.0 + 2.0 * e) / 4.0);
+  d2fdce[4] = -((-2.0 * c) * (1.0 - 2.0 * e) / 2.0);
+  d2fdce[5] = +(1.0 + 2.0 * c) * (-2.0 * e) / 2.0;
+  d2fdce[6] = +(-2.0 * c) * (1.0 + 2.0 * e) / 2.0;
+  d2fdce[7] = -((1.0 - 2.0 * c) * (-2.0 * e) / 2.0);
+  d2fdce[8] = +(-2.0 * c) * (-2.0 * e);
+
+  dfde[0] = +(1.0 - c) * c * (1.0 - 2.0 * e) / 4.0;
+  dfde[1] = -((1.0 + c) * c * (1.0 - 2.0 * e) / 4.0);
+  dfde[2] = +(1.0 + c) * c * (1.0 + 2.0 * e) / 4.0;
+  dfde[3] = -((1.0 - c) * c * (1.0 + 2.0 * e) / 4.0);
+  dfde[4] = -((1.0 + c) * (1.0 - c) * (1.0 - 2.0 * e) / 2.0);
+  dfde[5] = +(1.0 + c) * c * (-2.0 * e) / 2.0;
+  dfde[6] = +(1.0 + c) * (1.0 - c) * (1.0 + 2.0 * e) / 2.0;
+  dfde[7] = -((1.0 - c) * c * (-2.0 * e) / 2.0);
+  dfde[8] = +(1.0 + c) * (1.0 - c) * (-2.0 * e);
+
+  d2fde2[0] = +(1.0 - c) * c * (-2.0) / 4.0;
+  d2fde2[1] = -((1.0 + c) * c * (-2.0) / 4.0);
+  d2fde2[2] = +(1.0 + c) * c * (+2.0) / 4.0;
+  d2fde2[3] = -((1.0 - c) * c * (+2.0) / 4.0);
+  d2fde2[4] = -((1.0 + c) * (1.0 - c) * (-2.0) / 2.0);
+  d2fde2[5] = +(1.0 + c) * c * (-2.0) / 2.0;
+  d2fde2[6] = +(1.0 + c) * (1.0 - c) * (+2.0) / 2.0;
+  d2fde2[7] = -((1.0 - c) * c * (-2.0) / 2.0);
+  d2fde2[8] = +(1.0 + c) * (1.0 - c) * (-2.0);
+}
+
+static void build_tables(Tables& T) {
+  // setGaussPointsQuadrangle1D (NGauss_1D = 3), :69-72
+  T.P1[0] = 0.0 + 1.0e-8;
+  T.P1[1] = +std::sqrt(3.0 / 5.0);
+  T.P1[2] = -std::sqrt(3.0 / 5.0);
+  // setGaussWeightsQuadrangle1D, :166-169
+  T.W1[0] = 8.0 / 9.0;
+  T.W1[1] = 5.0 / 9.0;
+  T.W1[2] = 5.0 / 9.0;
+  // 2D points / weights, :92-99 and :189-196 (i: xi outer, j: eta inner)
+  int ng = 0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      T.P2[ng][0] = T.P1[i];
+      T.P2[ng][1] = T.P1[j];
+      T.W2[ng] = T.W1[i] * T.W1[j];
+      ng++;
+    }
+  for (int g = 0; g < 9; g++)
+    quadratic_quadrangle(T.P2[g][0], T.P2[g][1], T.bfn[g], T.dfdc[g], T.dfde[g], T.d2fdc2[g],
+                         T.d2fde2[g], T.d2fdce[g]);
+  // Edge tables, :261-312 (ied 1: eta=-1, 2: xi=+1, 3: eta=+1, 4: xi=-1)
+  for (int g = 0; g < 3; g++)
+    for (int ed = 0; ed < 4; ed++) {
+      double c = 0, e = 0;
+      switch (ed) {
+        case 0: c = T.P1[g]; e = -1.0; break;
+        case 1: c = +1.0; e = T.P1[g]; break;
+        case 2: c = T.P1[g]; e = +1.0; break;
+        default: c = -1.0; e = T.P1[g]; break;
+      }
+      quadratic_quadrangle(c, e, T.bfnE[ed][g], T.dfdcE[ed][g], T.dfdeE[ed][g], T.d2fdc2E[ed][g],
+                           T.d2fde2E[ed][g], T.d2fdceE[ed][g]);
+    }
+}
+
+static Tables make_tables() {
+  Tables T;
+  build_tables(T);
+  return T;
+}
+static const Tables& tables() {
+  static const Tables T = make_tables();  // thread-safe initialisation
+  return T;
+}
+
+// ----------------------------------------------------------------------------
+// Element state (NodeArrays, Global_Variables_mod.F90:86-110) and Gauss-point
+// quantities (GaussPointQuantities, :56-84), real parts only.
+// ----------------------------------------------------------------------------
+struct Elem {
+  vreal TL[9][6];  // TEMP_TL(inod, ieq): Ux Uy P Sxx Sxy Syy
+  vreal EU;        // TEMP_EU(Nex)
+  double X[9][3];  // Xi_loc(inod, 1:3), z = 0
+  vreal dFdXi[3][9], d2FdXi2[3][9];
+  const double *bfn, *dfdc, *dfde, *d2fdc2, *d2fde2, *d2fdce;
+};
+
+struct GsPt {
+  vreal GU[3][3], GUT[3][3], GU_tr[3][3], GUT_tr[3][3], G_dot_tr[3][3], P_tot[3][3], G2U[3][3];
+  vreal U_f[3], dPdXi[3], Momentum[3], Convection[3];
+  vreal Continuity, P;
+  vreal dTdXi[3][3][3], dCdXi[3][3][3], dSdXi[3][3][3];
+  vreal S[3][3], Con_EQ[3][3], T_ve[3][3], C[3][3];
+  vreal f[3], g[2], dgdxi[2][3], f_T;
+  vreal normal[3], dL;
+  vreal dXidci[3][3];
+  vreal detJ, WET;
+  vreal mag_re, mag_im;  // mag_con_EQ (complex sqrt of a real; purely imaginary if negative)
+  vreal tlsic, tlsme, tlsce, tdcs;
+  vreal WiN;             // module global WiN, rewritten per Gauss point
+};
+
+struct Phys {
+  vreal ReN, BvN, VeN, DfN, WiN0, dt;
+  int model;
+  bool dcs_off;  // DfN > 1e-12 switches tdcs off (Element_Calculations_mod.F90:758-760)
+};
+
+static Phys make_phys(const vo_params* p) {
+  Phys q;
+  q.ReN = (p->ReN == 0.0) ? Z() : vreal(p->ReN);
+  q.BvN = vreal(p->BvN);
+  q.VeN = (p->VeN == 0.0) ? Z() : vreal(p->VeN);
+  q.DfN = (p->DfN == 0.0) ? Z() : vreal(p->DfN);
+  q.WiN0 = vreal(p->WiN0);
+  q.dt = vreal(p->dt);
+  q.model = p->model;
+  q.dcs_off = p->DfN > 1.0e-12;
+  return q;
+}
+
+static void zero33(vreal A[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) A[i][j] = Z();
+}
+
+// Fortran matmul of 3x3 arrays: C(i,j) = sum_k A(i,k)*B(k,j), k sequential from 0.
+static void matmul33(const vreal A[3][3], const vreal B[3][3], vreal Cm[3][3]) {
+  vreal R[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      vreal s = Z();
+      for (int k = 0; k < 3; k++) s = s + A[i][k] * B[k][j];
+      R[i][j] = s;
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) Cm[i][j] = R[i][j];
+}
+
+// traceTensor, Tools_mod.F90:206-238
+static vreal trace33(const vreal A[3][3]) {
+  vreal s = Z();
+  for (int i = 0; i < 3; i++) s = s + A[i][i];
+  return s;
+}
+
+// doubleDotProduct, Tools_mod.F90:262-340: sum_i sum_j A(i,j)*B(j,i), i outer.
+static vreal ddot33(const vreal A[3][3], const vreal B[3][3]) {
+  vreal s = Z();
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) s = s + A[i][j] * B[j][i];
+  return s;
+}
+
+// dot_product of two length-3 vectors, sequential from 0.
+static vreal dot3(const vreal* a, const vreal* b) {
+  vreal s = Z();
+  for (int k = 0; k < 3; k++) s = s + a[k] * b[k];
+  return s;
+}
+
+// dot_product(TEMP_TL(:,i), v(:)) over the 9 nodes.
+static vreal dot9_col(const Elem& E, int ieq, const vreal* v) {
+  vreal s = Z();
+  for (int n = 0; n < 9; n++) s = s + E.TL[n][ieq] * v[n];
+  return s;
+}
+static vreal dot9_col_d(const Elem& E, int ieq, const double* v) {
+  vreal s = Z();
+  for (int n = 0; n < 9; n++) s = s + E.TL[n][ieq] * vreal(v[n]);
+  return s;
+}
+
+// ----------------------------------------------------------------------------
+// a3: basis2D, Element_Calculations_mod.F90:308-361 (idir = [1,2]).
+// GsPt%Xi (the physical position) is not used by the residual and is skipped.
+// ----------------------------------------------------------------------------
+static void basis2D(Elem& E, GsPt& G) {
+  zero33(G.dXidci);
+  for (int n = 0; n < 9; n++)
+    for (int j = 0; j < 2; j++) {  // j = 3 (z) multiplies the zero z-coordinate
+      G.dXidci[0][j] = G.dXidci[0][j] + vreal(E.dfdc[n]) * vreal(E.X[n][j]);
+      G.dXidci[1][j] = G.dXidci[1][j] + vreal(E.dfde[n]) * vreal(E.X[n][j]);
+    }
+  vreal dxdc = G.dXidci[0][0], dydc = G.dXidci[0][1];
+  vreal dxde = G.dXidci[1][0], dyde = G.dXidci[1][1];
+  G.detJ = dxdc * dyde - dxde * dydc;
+  vreal dcdx = dyde / G.detJ;
+  vreal dedx = -(dydc / G.detJ);
+  vreal dcdy = -(dxde / G.detJ);
+  vreal dedy = dxdc / G.detJ;
+  for (int r = 0; r < 3; r++)
+    for (int n = 0; n < 9; n++) {
+      E.dFdXi[r][n] = Z();
+      E.d2FdXi2[r][n] = Z();
+    }
+  vreal two = vreal(2.0);
+  for (int n = 0; n < 9; n++) {
+    vreal fc = vreal(E.dfdc[n]), fe = vreal(E.dfde[n]);
+    vreal fcc = vreal(E.d2fdc2[n]), fee = vreal(E.d2fde2[n]), fce = vreal(E.d2fdce[n]);
+    E.dFdXi[0][n] = dcdx * fc + dedx * fe;
+    E.dFdXi[1][n] = dcdy * fc + dedy * fe;
+    E.d2FdXi2[0][n] = (dcdx * dcdx) * fcc + (dedx * dedx) * fee + two * dcdx * dedx * fce;
+    E.d2FdXi2[1][n] = (dcdy * dcdy) * fcc + (dedy * dedy) * fee + two * dcdy * dedy * fce;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// a4: setFlowUnknowns, Element_Calculations_mod.F90:449-533 (Neq_f=2, Neq_s=3)
+// ----------------------------------------------------------------------------
+static void setFlowUnknowns(const Elem& E, GsPt& G) {
+  for (int i = 0; i < 3; i++) G.U_f[i] = Z();
+  zero33(G.GU);
+  zero33(G.G2U);
+  G.P = Z();
+  for (int i = 0; i < 3; i++) G.dPdXi[i] = Z();
+  for (int i = 0; i < 2; i++) {
+    G.U_f[i] = dot9_col_d(E, i, E.bfn);
+    for (int j = 0; j < 3; j++) {
+      G.GU[j][i] = dot9_col(E, i, E.dFdXi[j]);
+      G.G2U[j][i] = dot9_col(E, i, E.d2FdXi2[j]);
+    }
+    G.dPdXi[i] = dot9_col(E, 2, E.dFdXi[i]);
+  }
+  G.P = dot9_col_d(E, 2, E.bfn);
+  zero33(G.S);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      for (int c = 0; c < 3; c++) G.dSdXi[a][b][c] = Z();
+  // getStressComponent order: (1,1), (1,2), (2,2), Tools_mod.F90:769-793
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  for (int s = 0; s < 3; s++) {
+    int j = comp[s][0], k = comp[s][1];
+    G.S[j][k] = dot9_col_d(E, 3 + s, E.bfn);
+    G.S[k][j] = G.S[j][k];
+    for (int d = 0; d < 3; d++) G.dSdXi[j][k][d] = dot9_col(E, 3 + s, E.dFdXi[d]);
+    for (int d = 0; d < 3; d++) G.dSdXi[k][j][d] = G.dSdXi[j][k][d];
+  }
+}
+
+// setFlowKinematics, Element_Calculations_mod.F90:605-634 (N = 2)
+static void setFlowKinematics(GsPt& G, const vreal I[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.GUT[i][j] = G.GU[j][i];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.GU_tr[i][j] = G.GU[i][j];
+  vreal tr = trace33(G.GU);
+  vreal Nc = vreal(2.0);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) G.GU_tr[i][j] = G.GU[i][j] - cdiv(tr * I[i][j], Nc);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.GUT_tr[i][j] = G.GU_tr[j][i];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.G_dot_tr[i][j] = G.GU_tr[i][j] + G.GUT_tr[i][j];
+}
+
+// WTensor2D + omegaNominator2DReal/omegaDenominator2DReal,
+// Constitutive_Models_SQRT_mod.F90:513-584.  S and GUT are the 2x2 leading blocks.
+static vreal omega12(const vreal S[3][3], const vreal GUT[3][3]) {
+  vreal n = -(S[0][0] * GUT[1][0]) + S[0][1] * (GUT[0][0] - GUT[1][1]) + S[1][1] * GUT[0][1];
+  vreal d = S[0][0] + S[1][1];
+  return n / d;
+}
+
+// ----------------------------------------------------------------------------
+// a6: strongFormConEQ_SQRT and helpers, Constitutive_Models_SQRT_mod.F90:120-509
+// ----------------------------------------------------------------------------
+static void strongFormConEQ_SQRT(const Phys& ph, GsPt& G, const vreal I[3][3], vreal UCDout[3][3]) {
+  // setCTensorSQRT :379-393
+  matmul33(G.S, G.S, G.C);
+  for (int k = 0; k < 3; k++) {
+    vreal dS[3][3], A[3][3], B[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) dS[i][j] = G.dSdXi[i][j][k];
+    matmul33(dS, G.S, A);
+    matmul33(G.S, dS, B);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) G.dCdXi[i][j][k] = A[i][j] + B[i][j];
+  }
+  // conEQ_p
+  if (ph.model == 0) {
+    // conEQ_L_PTT_SQRT :153-180 (N = Problem_Main%Ndim = 2)
+    vreal trC = trace33(G.C);
+    G.f_T = vreal(1.0) + ph.VeN * (trC - vreal(2.0));
+    G.f[0] = G.f_T;
+    G.f[1] = G.f[0];
+    G.f[2] = Z();
+  } else {
+    // conEQ_Giesekus_SQRT :285-305
+    vreal aG = ph.VeN;
+    G.f_T = vreal(1.0) - vreal(2.0) * aG;
+    G.f[0] = vreal(1.0) - aG;
+    G.f[1] = vreal(1.0) - vreal(2.0) * aG;
+    G.f[2] = aG;
+  }
+  G.g[0] = vreal(1.0);
+  G.g[1] = vreal(1.0);
+  for (int d = 0; d < 3; d++) {
+    G.dgdxi[0][d] = Z();
+    G.dgdxi[1][d] = Z();
+  }
+  // setStressTensorSQRT :483-509
+  vreal a = (vreal(1.0) - ph.BvN) / G.WiN;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.T_ve[i][j] = G.g[1] * G.C[i][j] - G.g[0] * I[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.T_ve[i][j] = a * G.T_ve[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      G.P_tot[i][j] = G.T_ve[i][j] + ph.BvN * G.G_dot_tr[i][j] - G.P * I[i][j];
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        G.dTdXi[i][j][k] = G.dgdxi[1][k] * G.C[i][j] + G.g[1] * G.dCdXi[i][j][k] - G.dgdxi[0][k] * I[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) G.dTdXi[i][j][k] = a * G.dTdXi[i][j][k];
+
+  // setUpperConvectedDerivativeSQRT :397-416 (dSdt = 0 off the transient path)
+  vreal UCD[3][3];
+  zero33(UCD);
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) UCD[i][j] = UCD[i][j] + G.U_f[k] * G.dSdXi[i][j][k];
+  // setOmegaTensor :420-455 with the traceless transposed gradient
+  vreal W[3][3];
+  zero33(W);
+  W[0][1] = omega12(G.S, G.GUT_tr);
+  W[1][0] = -W[0][1];
+  vreal M1[3][3], M2[3][3];
+  matmul33(G.S, G.GU_tr, M1);
+  matmul33(W, G.S, M2);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) UCD[i][j] = UCD[i][j] - M1[i][j] - M2[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) UCDout[i][j] = UCD[i][j];
+
+  // setSInverse :459-479 -> inverseTensor2D (Tools_mod.F90:129-146)
+  vreal SI[3][3];
+  zero33(SI);
+  {
+    vreal detA = G.S[0][0] * G.S[1][1] - G.S[0][1] * G.S[1][0];
+    SI[0][0] = G.S[1][1] / detA;
+    SI[0][1] = -G.S[0][1] / detA;
+    SI[1][0] = -G.S[1][0] / detA;
+    SI[1][1] = G.S[0][0] / detA;
+  }
+  // S3 = matmul(S, matmul(S,S)); the inner product has the operands of C.
+  // Multiplied by f(3), which is zero for L-PTT.
+  vreal S3[3][3];
+  if (ph.model == 0) zero33(S3);
+  else matmul33(G.S, G.C, S3);
+  vreal term_S[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      term_S[i][j] = G.f[2] * S3[i][j] + G.f[1] * G.S[i][j] - G.f[0] * SI[i][j];
+  vreal two = vreal(2.0);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) G.Con_EQ[i][j] = G.WiN * UCD[i][j] + term_S[i][j] / two;
+}
+
+// ----------------------------------------------------------------------------
+// a5: strongFormMomentum :658-683, strongFormContinuity :687-697
+// ----------------------------------------------------------------------------
+static void strongFormMomentum(const Phys& ph, GsPt& G) {
+  vreal div_Tve[3];
+  for (int i = 0; i < 3; i++) {
+    div_Tve[i] = Z();
+    for (int j = 0; j < 3; j++) div_Tve[i] = div_Tve[i] + G.dTdXi[j][i][j];
+  }
+  vreal div_Ptot[3];
+  for (int i = 0; i < 3; i++)
+    div_Ptot[i] = -G.dPdXi[i] + div_Tve[i] + ph.BvN * (G.G2U[0][i] + G.G2U[1][i] + G.G2U[2][i]);
+  // Convection = dUfdt (zero) + matmul(U_f, GU)
+  for (int j = 0; j < 3; j++) {
+    vreal m = Z();
+    for (int k = 0; k < 3; k++) m = m + G.U_f[k] * G.GU[k][j];
+    G.Convection[j] = Z() + m;
+  }
+  for (int i = 0; i < 3; i++) G.Momentum[i] = ph.ReN * G.Convection[i] - div_Ptot[i];
+}
+
+// ----------------------------------------------------------------------------
+// a7: setStabilizationTerms, Element_Calculations_mod.F90:701-773
+// ----------------------------------------------------------------------------
+static void setStabilizationTerms(const Phys& ph, const Elem& E, GsPt& G) {
+  vreal Helem = Z();
+  vreal tsugn = Z();
+  for (int n = 0; n < 9; n++) {
+    Helem = Helem + E.dFdXi[0][n] * E.dFdXi[0][n] + E.dFdXi[1][n] * E.dFdXi[1][n] +
+            E.dFdXi[2][n] * E.dFdXi[2][n];
+    vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
+    tsugn = tsugn + fabsr(dot3(G.U_f, dF));
+  }
+  Helem = vreal(1.0) / vsqrt(Helem);
+  vreal H2 = Helem * Helem;
+
+  vreal ddT = ddot33(G.T_ve, G.T_ve);
+  vreal ddG = ddot33(G.G_dot_tr, G.G_dot_tr);
+  vreal x = (vreal(1.0) - ph.BvN) / G.WiN;
+  vreal Ha = vsqrt(x * x + vreal(0.5) * ddT);
+  Ha = cdiv(Ha, vsqrt(vreal(1.0) + vreal(0.5) * ddG));
+
+  vreal r1 = ph.ReN / ph.dt;
+  vreal r2 = ph.ReN * tsugn;
+  vreal r3 = ph.BvN / H2;
+  vreal r4 = cdiv(Ha, H2);
+  G.tlsme = r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4;
+  G.tlsme = cdiv(vreal(1.0), vsqrt(G.tlsme));
+  G.tlsic = cdiv(H2, G.tlsme);
+
+  // tdcs = (Helem**2)/sqrt(0.5*S:S), zeroed when DfN > 1e-12 (:746-760).  Work
+  // whose result the reference overwrites or multiplies by a zero coefficient
+  // is skipped; the values that reach the residual are unchanged.
+  if (ph.dcs_off) {
+    G.tdcs = Z();
+  } else {
+    vreal ddS = ddot33(G.S, G.S);
+    G.tdcs = vreal(0.5) * ddS;
+    G.tdcs = cdiv(H2, vsqrt(G.tdcs));
+  }
+  vreal ddC = (ph.model == 0) ? Z() : ddot33(G.C, G.C);  // multiplied by f(3) = 0 for L-PTT
+
+  // mag_con_EQ = sqrt(0.5*Con_EQ:Con_EQ); complex sqrt of a real argument.  Only
+  // used multiplied by tdcs.
+  G.mag_re = Z();
+  G.mag_im = Z();
+  if (!ph.dcs_off) {
+    vreal mc = vreal(0.5) * ddot33(G.Con_EQ, G.Con_EQ);
+    if (val(mc) >= 0.0) G.mag_re = vsqrt(mc);
+    else G.mag_im = vsqrt(-mc);
+  }
+
+  vreal t1 = G.WiN / ph.dt;
+  vreal t2 = G.WiN * tsugn;
+  vreal t3 = G.WiN * vsqrt(vreal(0.5) * ddG);
+  vreal t4 = G.f[1];
+  vreal t5 = G.f[2] * vsqrt(vreal(0.5) * ddC);
+  // ((tdcs*mag)/H2)**2 with mag complex: exactly one of mag_re, mag_im is nonzero,
+  // so the square is real: +re^2 or -im^2.
+  vreal z6r = cdiv(G.tdcs * G.mag_re, H2);
+  vreal z6i = cdiv(G.tdcs * G.mag_im, H2);
+  vreal t6 = z6r * z6r - z6i * z6i;
+  vreal t7 = ph.DfN / H2;
+  G.tlsce = t1 * t1 + t2 * t2 + t3 * t3 + t4 * t4 + t5 * t5 + t6 + t7 * t7;
+  G.tlsce = cdiv(vreal(1.0), vsqrt(G.tlsce));
+}
+
+// setFlowQuantities :247-262 (+ setContinuationParameter :577-601)
+static void setFlowQuantities(const Phys& ph, const Elem& E, GsPt& G, const vreal I[3][3]) {
+  setFlowUnknowns(E, G);
+  G.WiN = ph.WiN0 * E.EU;  // WiN = WiN0*EU(Nex)
+  setFlowKinematics(G, I);
+  vreal UCD[3][3];
+  strongFormConEQ_SQRT(ph, G, I, UCD);
+  strongFormMomentum(ph, G);
+  G.Continuity = trace33(G.GU);
+  setStabilizationTerms(ph, E, G);
+}
+
+static void identity(vreal I[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) I[i][j] = (i == j) ? vreal(1.0) : Z();
+}
+
+// ----------------------------------------------------------------------------
+// a8: test-node residual terms, Residuals_mod.F90 + equationsResidualsBulkMain
+// (Equations_mod.F90:102-144)
+// ----------------------------------------------------------------------------
+static void nodeResidual(const Phys& ph, const Elem& E, const GsPt& G, int n, vreal TERM[6]) {
+  vreal bifn = vreal(E.bfn[n]);
+  vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
+  vreal one = vreal(1.0);
+  // Values the reference recomputes with identical operands for each i or each
+  // stress component are computed once here (bitwise the same).
+  vreal omb = one - ph.BvN;
+  vreal Udf = dot3(G.U_f, dF);
+  vreal lsme_m = G.tlsme * ph.ReN * Udf;  // LSMEMomentum :161-173
+  // momentumMainSQRT :295-337
+  for (int i = 0; i < 2; i++) {
+    vreal lsme = lsme_m * G.Momentum[i];
+    vreal lsic = G.tlsic * dF[i];                      // LSICMomentum :177-187
+    lsic = lsic * G.Continuity;
+    // LSCEMomentumSQRT :388-433
+    vreal GUT_l[3][3], GU_l[3][3];
+    zero33(GUT_l);
+    for (int k = 0; k < 3; k++) GUT_l[i][k] = dF[k];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) GU_l[a][b] = GUT_l[b][a];
+    vreal W_l[3][3];
+    zero33(W_l);
+    W_l[0][1] = omega12(G.S, GUT_l);
+    W_l[1][0] = -W_l[0][1];
+    vreal M1[3][3], M2[3][3], lsce[3][3];
+    matmul33(G.S, GU_l, M1);
+    matmul33(W_l, G.S, M2);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        vreal U = -M1[a][b] - M2[a][b];
+        lsce[a][b] = cdiv(G.tlsce * U * omb, G.WiN);
+      }
+    vreal lsce_term = ddot33(lsce, G.Con_EQ);
+    vreal Prow[3] = {G.P_tot[i][0], G.P_tot[i][1], G.P_tot[i][2]};
+    TERM[i] = ph.ReN * G.Convection[i] * bifn + dot3(Prow, dF) + lsme + lsic + lsce_term;
+  }
+  // continuityBulkMain :90-110 with LSMEContinuity :223-234
+  {
+    vreal lsme = G.tlsme * dot3(dF, G.Momentum);
+    TERM[2] = G.Continuity * bifn + lsme;
+  }
+  // stressesBulkMainSQRT :341-384
+  // LSCEStressSQRT :459-475 (independent of the stress component)
+  vreal lsce_s = G.WiN * Udf;
+  lsce_s = lsce_s + cdiv(G.f_T * bifn, vreal(2.0));
+  lsce_s = G.tlsce * lsce_s;
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  for (int s = 0; s < 3; s++) {
+    int j = comp[s][0], k = comp[s][1];
+    // LSMEStressSQRT :437-455
+    vreal lsme = -(dF[k] * G.Momentum[j]);
+    if (j != k) lsme = lsme - dF[j] * G.Momentum[k];
+    lsme = G.tlsme * lsme * G.WiN;
+    vreal lsce = lsce_s * G.Con_EQ[j][k];
+    // DCSStressSQRT :479-490 times mag_con_EQ (real part of a complex product)
+    vreal dS[3] = {G.dSdXi[j][k][0], G.dSdXi[j][k][1], G.dSdXi[j][k][2]};
+    vreal dcs = G.tdcs * dot3(dS, dF);
+    dcs = dcs * G.mag_re;
+    TERM[3 + s] = G.Con_EQ[j][k] * bifn + ph.DfN * dot3(dS, dF) + lsme + lsce + dcs;
+  }
+}
+
+// residualBulk, Equations_mod.F90:59-98
+static void residualBulk(const Phys& ph, Elem& E, vreal Res[9][6]) {
+  const Tables& T = tables();
+  vreal I[3][3];
+  identity(I);
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) Res[n][q] = Z();
+  for (int g = 0; g < 9; g++) {
+    E.bfn = T.bfn[g];
+    E.dfdc = T.dfdc[g];
+    E.dfde = T.dfde[g];
+    E.d2fdc2 = T.d2fdc2[g];
+    E.d2fde2 = T.d2fde2[g];
+    E.d2fdce = T.d2fdce[g];
+    GsPt G;
+    basis2D(E, G);
+    setFlowQuantities(ph, E, G, I);
+    G.WET = vreal(T.W2[g]) * fabsr(G.detJ);
+    for (int n = 0; n < 9; n++) {
+      vreal TERM[6];
+      nodeResidual(ph, E, G, n, TERM);
+      for (int q = 0; q < 6; q++) Res[n][q] = Res[n][q] + TERM[q] * G.WET;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// a12: edge path, residualEdge (Equations_mod.F90:387-424), setEdgeQuantities
+// (Element_Calculations_mod.F90:852-924), openBC (Residuals_mod.F90:599-624)
+// ----------------------------------------------------------------------------
+static void residualEdge(const Phys& ph, Elem& E, int ied /*1..4*/, vreal Res[9][6]) {
+  const Tables& T = tables();
+  vreal I[3][3];
+  identity(I);
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) Res[n][q] = Z();
+  int ed = ied - 1;
+  for (int g = 0; g < 3; g++) {
+    E.bfn = T.bfnE[ed][g];
+    E.dfdc = T.dfdcE[ed][g];
+    E.dfde = T.dfdeE[ed][g];
+    E.d2fdc2 = T.d2fdc2E[ed][g];
+    E.d2fde2 = T.d2fde2E[ed][g];
+    E.d2fdce = T.d2fdceE[ed][g];
+    GsPt G;
+    basis2D(E, G);
+    setFlowQuantities(ph, E, G, I);
+    vreal dxdc = G.dXidci[0][0], dydc = G.dXidci[0][1];
+    vreal dxde = G.dXidci[1][0], dyde = G.dXidci[1][1];
+    G.normal[2] = Z();
+    switch (ied) {
+      case 1: G.normal[0] = dydc; G.normal[1] = -dxdc; break;
+      case 2: G.normal[0] = dyde; G.normal[1] = -dxde; break;
+      case 3: G.normal[0] = -dydc; G.normal[1] = dxdc; break;
+      default: G.normal[0] = -dyde; G.normal[1] = dxde; break;
+    }
+    G.dL = vsqrt(G.normal[0] * G.normal[0] + G.normal[1] * G.normal[1]);
+    G.normal[0] = G.normal[0] / G.dL;
+    G.normal[1] = G.normal[1] / G.dL;
+    G.WET = vreal(T.W1[g]) * G.dL;
+    for (int n = 0; n < 9; n++) {
+      vreal TERM[6];
+      for (int q = 0; q < 6; q++) TERM[q] = Z();
+      vreal bifn = vreal(E.bfn[n]);
+      for (int i = 0; i < 2; i++) {
+        vreal v[3];
+        for (int k = 0; k < 3; k++) v[k] = G.P_tot[k][i] + G.P * I[k][i];
+        TERM[i] = -(dot3(G.normal, v) * bifn);
+      }
+      for (int q = 0; q < 6; q++) Res[n][q] = Res[n][q] + TERM[q] * G.WET;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// a10: storeBulk / storeEdge FD Jacobian, Equations_mod.F90:226-313, 465-554;
+// perturbVariable (Tools_mod.F90:831-840): eps_jac = 1e-8.
+// A[(inod,ieq)][(jnod,jeq)] = TEMP_A_f(inod,ieq,jnod,jeq) = dR(jnod,jeq)/du(inod,ieq);
+// Ac[(jnod,jeq)] = TEMP_Ac_f(jnod,jeq,1).  The perturbation is applied and
+// removed in place, so entries perturbed earlier keep fl(fl(u+h)-h).
+// ----------------------------------------------------------------------------
+static void elementFD(const Phys& ph, Elem& E, int ied, bool jac, vreal Res[9][6], vreal* A, vreal* Ac) {
+  auto resid = [&](vreal R[9][6]) {
+    if (ied == 0) residualBulk(ph, E, R);
+    else residualEdge(ph, E, ied, R);
+  };
+  resid(Res);
+  if (!jac) return;
+  const vreal h = vreal(1.0e-8);
+  vreal dRes[9][6];
+  for (int inod = 0; inod < 9; inod++)
+    for (int ieq = 0; ieq < 6; ieq++) {
+      E.TL[inod][ieq] = E.TL[inod][ieq] + h;
+      resid(dRes);
+      E.TL[inod][ieq] = E.TL[inod][ieq] - h;
+      int col = inod * 6 + ieq;
+      for (int jnod = 0; jnod < 9; jnod++)
+        for (int jeq = 0; jeq < 6; jeq++)
+          A[col * 54 + jnod * 6 + jeq] = cdiv(dRes[jnod][jeq] - Res[jnod][jeq], h);
+    }
+  E.EU = E.EU + h;
+  resid(dRes);
+  E.EU = E.EU - h;
+  for (int jnod = 0; jnod < 9; jnod++)
+    for (int jeq = 0; jeq < 6; jeq++) Ac[jnod * 6 + jeq] = cdiv(dRes[jnod][jeq] - Res[jnod][jeq], h);
+}
+
+// copyToLocal, Element_Calculations_mod.F90:106-142 (global node numbering)
+static void copyToLocal(Elem& E, const double* coords, const int32_t* conn, int e, const double* TL,
+                        double EU) {
+  for (int n = 0; n < 9; n++) {
+    int gn = conn[e * 9 + n];
+    for (int q = 0; q < 6; q++) E.TL[n][q] = vreal(TL[(size_t)gn * 6 + q]);
+    E.X[n][0] = coords[(size_t)gn * 2 + 0];
+    E.X[n][1] = coords[(size_t)gn * 2 + 1];
+    E.X[n][2] = 0.0;
+  }
+  E.EU = vreal(EU);
+}
+
+// ----------------------------------------------------------------------------
+// a11/a13: matrix pattern (preAllocatePetsc, Petsc_mod.F90:629-743) and the
+// ADD_VALUES scatter (Storage_mod.F90:10-115), rows/cols 0-based
+// (gnod*6 + ieq), the arclength unknown last (N-1).
+// ----------------------------------------------------------------------------
+struct CSR {
+  int64_t n;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> col;
+};
+
+static void build_pattern(int n_nodes, int n_el, const int32_t* conn, CSR& P) {
+  int64_t N = (int64_t)n_nodes * 6 + 1;
+  P.n = N;
+  std::vector<std::vector<int32_t>> nb(n_nodes);
+  for (int e = 0; e < n_el; e++)
+    for (int a = 0; a < 9; a++)
+      for (int b = 0; b < 9; b++) nb[conn[e * 9 + a]].push_back(conn[e * 9 + b]);
+  for (auto& v : nb) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  P.rowptr.assign(N + 1, 0);
+  for (int r = 0; r < n_nodes; r++)
+    for (int q = 0; q < 6; q++) P.rowptr[(int64_t)r * 6 + q + 1] = (int64_t)nb[r].size() * 6 + 1;
+  P.rowptr[N] = N;  // dense arclength row
+  for (int64_t i = 0; i < N; i++) P.rowptr[i + 1] += P.rowptr[i];
+  P.col.resize(P.rowptr[N]);
+  for (int r = 0; r < n_nodes; r++)
+    for (int q = 0; q < 6; q++) {
+      int64_t p = P.rowptr[(int64_t)r * 6 + q];
+      for (int32_t c : nb[r])
+        for (int iq = 0; iq < 6; iq++) P.col[p++] = c * 6 + iq;
+      P.col[p++] = (int32_t)(N - 1);
+    }
+  int64_t p = P.rowptr[N - 1];
+  for (int64_t c = 0; c < N; c++) P.col[p++] = (int32_t)c;
+}
+
+static inline int64_t find_slot(const int64_t* rowptr, const int32_t* col, int64_t row, int32_t c) {
+  const int32_t* b = col + rowptr[row];
+  const int32_t* en = col + rowptr[row + 1];
+  const int32_t* it = std::lower_bound(b, en, c);
+  if (it == en || *it != c) return -1;
+  return it - col;
+}
+
+// Adds one element's local residual / Jacobian in PETSc's ADD_VALUES order.
+static int scatter_element(const int32_t* nodes, const double* Res54, const double* A, const double* Ac,
+                           int64_t N, const int64_t* rowptr, const int32_t* col, double* b, double* vals) {
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) b[(int64_t)nodes[n] * 6 + q] += Res54[n * 6 + q];
+  if (!A) return 0;
+  for (int jn = 0; jn < 9; jn++)
+    for (int jq = 0; jq < 6; jq++) {
+      int64_t row = (int64_t)nodes[jn] * 6 + jq;
+      for (int in = 0; in < 9; in++)
+        for (int iq = 0; iq < 6; iq++) {
+          int64_t s = find_slot(rowptr, col, row, nodes[in] * 6 + iq);
+          if (s < 0) return -1;
+          vals[s] += A[(in * 6 + iq) * 54 + jn * 6 + jq];
+        }
+    }
+  for (int jn = 0; jn < 9; jn++)
+    for (int jq = 0; jq < 6; jq++) {
+      int64_t row = (int64_t)nodes[jn] * 6 + jq;
+      int64_t s = find_slot(rowptr, col, row, (int32_t)(N - 1));
+      if (s < 0) return -1;
+      vals[s] += Ac[jn * 6 + jq];
+    }
+  return 0;
+}
+
+#ifndef VEF_COUNT_FLOPS
+static void to_double(const vreal* a, double* b, int n) {
+  for (int i = 0; i < n; i++) b[i] = val(a[i]);
+}
+#endif
+
+extern "C" {
+
+#ifndef VEF_COUNT_FLOPS
+
+// Gauss tables: out[0:3] P1, [3:6] W1, [6:24] P2, [24:33] W2, then 6 bulk
+// tables [9 gp][9 node] each, then 6 edge tables [4][3][9] each.
+int vo_tables(double* out) {
+  const Tables& T = tables();
+  double* o = out;
+  for (int i = 0; i < 3; i++) *o++ = T.P1[i];
+  for (int i = 0; i < 3; i++) *o++ = T.W1[i];
+  for (int g = 0; g < 9; g++) { *o++ = T.P2[g][0]; *o++ = T.P2[g][1]; }
+  for (int g = 0; g < 9; g++) *o++ = T.W2[g];
+  const double(*bt[6])[9] = {T.bfn, T.dfdc, T.dfde, T.d2fdc2, T.d2fde2, T.d2fdce};
+  for (int t = 0; t < 6; t++)
+    for (int g = 0; g < 9; g++)
+      for (int n = 0; n < 9; n++) *o++ = bt[t][g][n];
+  const double(*et[6])[3][9] = {T.bfnE, T.dfdcE, T.dfdeE, T.d2fdc2E, T.d2fde2E, T.d2fdceE};
+  for (int t = 0; t < 6; t++)
+    for (int ed = 0; ed < 4; ed++)
+      for (int g = 0; g < 3; g++)
+        for (int n = 0; n < 9; n++) *o++ = et[t][ed][g][n];
+  return (int)(o - out);
+}
+
+// Geometry at one bulk Gauss point: out = [detJ, WET, dFdXi(2x9), d2FdXi2(2x9)].
+int vo_basis2d(const double* xy /*[9][2]*/, int gp, int edge /*0 bulk, 1..4*/, double* out) {
+  const Tables& T = tables();
+  Elem E;
+  for (int n = 0; n < 9; n++) {
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0;
+  }
+  if (edge == 0) {
+    E.bfn = T.bfn[gp]; E.dfdc = T.dfdc[gp]; E.dfde = T.dfde[gp];
+    E.d2fdc2 = T.d2fdc2[gp]; E.d2fde2 = T.d2fde2[gp]; E.d2fdce = T.d2fdce[gp];
+  } else {
+    int ed = edge - 1;
+    E.bfn = T.bfnE[ed][gp]; E.dfdc = T.dfdcE[ed][gp]; E.dfde = T.dfdeE[ed][gp];
+    E.d2fdc2 = T.d2fdc2E[ed][gp]; E.d2fde2 = T.d2fde2E[ed][gp]; E.d2fdce = T.d2fdceE[ed][gp];
+  }
+  GsPt G;
+  basis2D(E, G);
+  out[0] = G.detJ;
+  out[1] = (edge == 0) ? T.W2[gp] * std::fabs(G.detJ) : 0.0;
+  for (int r = 0; r < 2; r++)
+    for (int n = 0; n < 9; n++) {
+      out[2 + r * 9 + n] = E.dFdXi[r][n];
+      out[20 + r * 9 + n] = E.d2FdXi2[r][n];
+    }
+  return 0;
+}
+
+// Interpolated fields at bulk Gauss point gp (setFlowUnknowns):
+// out = U[2], GU[2][2] (GU(j,i) = dU_i/dx_j), G2U[2][2], P, dPdXi[2],
+//       S[3] (xx,xy,yy), dSdXi[3][2]  -> 22 values.
+int vo_gauss_fields(const double* xy, const double* TL, int gp, double* out) {
+  const Tables& T = tables();
+  Elem E;
+  for (int n = 0; n < 9; n++) {
+    for (int q = 0; q < 6; q++) E.TL[n][q] = TL[n * 6 + q];
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0;
+  }
+  E.bfn = T.bfn[gp]; E.dfdc = T.dfdc[gp]; E.dfde = T.dfde[gp];
+  E.d2fdc2 = T.d2fdc2[gp]; E.d2fde2 = T.d2fde2[gp]; E.d2fdce = T.d2fdce[gp];
+  GsPt G;
+  basis2D(E, G);
+  setFlowUnknowns(E, G);
+  double* o = out;
+  *o++ = G.U_f[0];
+  *o++ = G.U_f[1];
+  for (int j = 0; j < 2; j++)
+    for (int i = 0; i < 2; i++) *o++ = G.GU[j][i];
+  for (int j = 0; j < 2; j++)
+    for (int i = 0; i < 2; i++) *o++ = G.G2U[j][i];
+  *o++ = G.P;
+  *o++ = G.dPdXi[0];
+  *o++ = G.dPdXi[1];
+  *o++ = G.S[0][0];
+  *o++ = G.S[0][1];
+  *o++ = G.S[1][1];
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  for (int s2 = 0; s2 < 3; s2++)
+    for (int d = 0; d < 2; d++) *o++ = G.dSdXi[comp[s2][0]][comp[s2][1]][d];
+  return (int)(o - out);
+}
+
+// Pointwise strong form at one point from explicit field values, for the
+// closed-form checks.  in: U[2], GU[2][2] (GU(j,i) = dU_i/dx_j), G2U[2][2]
+// (G2U(j,i) = d2U_i/dx_j2), P, dPdXi[2], S[3] (xx,xy,yy), dS[3][2], EU.
+// out: Con_EQ[2][2], Momentum[2], Continuity, T_ve[3][3], P_tot[3][3], W12,
+//      UCD[2][2], f_T, C[2][2] -> 4+2+1+9+9+1+4+1+4 = 35 values.
+int vo_point_strong_form(const vo_params* p, const double* U, const double* GU, const double* G2U, double P,
+                         const double* dP, const double* S, const double* dS, double EU, double* out) {
+  Phys ph = make_phys(p);
+  vreal I[3][3];
+  identity(I);
+  GsPt G;
+  for (int i = 0; i < 3; i++) G.U_f[i] = 0.0, G.dPdXi[i] = 0.0;
+  zero33(G.GU);
+  zero33(G.G2U);
+  for (int i = 0; i < 2; i++) {
+    G.U_f[i] = U[i];
+    G.dPdXi[i] = dP[i];
+    for (int j = 0; j < 2; j++) {
+      G.GU[j][i] = GU[j * 2 + i];
+      G.G2U[j][i] = G2U[j * 2 + i];
+    }
+  }
+  G.P = P;
+  zero33(G.S);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      for (int c = 0; c < 3; c++) G.dSdXi[a][b][c] = 0.0;
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  for (int s = 0; s < 3; s++) {
+    int j = comp[s][0], k = comp[s][1];
+    G.S[j][k] = S[s];
+    G.S[k][j] = S[s];
+    for (int d = 0; d < 2; d++) {
+      G.dSdXi[j][k][d] = dS[s * 2 + d];
+      G.dSdXi[k][j][d] = dS[s * 2 + d];
+    }
+  }
+  G.WiN = ph.WiN0 * EU;
+  setFlowKinematics(G, I);
+  vreal UCD[3][3];
+  strongFormConEQ_SQRT(ph, G, I, UCD);
+  strongFormMomentum(ph, G);
+  G.Continuity = trace33(G.GU);
+  double* o = out;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) *o++ = G.Con_EQ[i][j];
+  *o++ = G.Momentum[0];
+  *o++ = G.Momentum[1];
+  *o++ = G.Continuity;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) *o++ = G.T_ve[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) *o++ = G.P_tot[i][j];
+  *o++ = omega12(G.S, G.GUT_tr);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) *o++ = UCD[i][j];
+  *o++ = G.f_T;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) *o++ = G.C[i][j];
+  return (int)(o - out);
+}
+
+// One element: residual (and FD Jacobian).  ied = 0 bulk, 1..4 outlet edge.
+// Res[9][6]; A[54*54] (column-block layout of TEMP_A_f); Ac[54].
+int vo_element(const vo_params* p, const double* xy, const double* TL /*[9][6]*/, double EU, int ied, int jac,
+               double* Res, double* A, double* Ac) {
+  Phys ph = make_phys(p);
+  Elem E;
+  for (int n = 0; n < 9; n++) {
+    for (int q = 0; q < 6; q++) E.TL[n][q] = TL[n * 6 + q];
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0.0;
+  }
+  E.EU = EU;
+  vreal R[9][6];
+  std::vector<vreal> Av(jac ? 54 * 54 : 0), Acv(jac ? 54 : 0);
+  elementFD(ph, E, ied, jac != 0, R, jac ? Av.data() : nullptr, jac ? Acv.data() : nullptr);
+  to_double(&R[0][0], Res, 54);
+  if (jac) {
+    to_double(Av.data(), A, 54 * 54);
+    to_double(Acv.data(), Ac, 54);
+  }
+  return 0;
+}
+
+// Pattern size and arrays (preAllocatePetsc).
+int64_t vo_pattern_nnz(int n_nodes, int n_el, const int32_t* conn) {
+  CSR P;
+  build_pattern(n_nodes, n_el, conn, P);
+  return P.rowptr[P.n];
+}
+int vo_pattern(int n_nodes, int n_el, const int32_t* conn, int64_t* rowptr, int32_t* col) {
+  CSR P;
+  build_pattern(n_nodes, n_el, conn, P);
+  std::memcpy(rowptr, P.rowptr.data(), sizeof(int64_t) * (P.n + 1));
+  std::memcpy(col, P.col.data(), sizeof(int32_t) * P.rowptr[P.n]);
+  return 0;
+}
+
+// Full assembly in the reference's order: initializeLSObjects (zero b, and the
+// matrix values if jac) then setEquationsBulk over all elements, then
+// setEquationsEdge over the outlet edges (Newton_Raphson_mod.F90:332-357).
+// Element-local work of a block of elements runs on `nthreads` OpenMP threads;
+// the ADD_VALUES scatter is applied serially in element order.
+int vo_assemble(const vo_params* p, int n_nodes, int n_el, const double* coords, const int32_t* conn,
+                int n_edges, const int32_t* edges /*[n][2] (elem, ied)*/, const double* TL, double EU, int jac,
+                const int64_t* rowptr, const int32_t* col, double* b, double* vals, int el_begin, int el_end,
+                int nthreads) {
+  Phys ph = make_phys(p);
+  int64_t N = (int64_t)n_nodes * 6 + 1;
+  int64_t nnz = rowptr[N];
+  std::fill(b, b + N, 0.0);
+  if (jac) std::fill(vals, vals + nnz, 0.0);
+  if (el_end < 0 || el_end > n_el) el_end = n_el;
+  const int BLK = 256;
+  std::vector<double> Rbuf(BLK * 54), Abuf(jac ? (size_t)BLK * 2916 : 0), Acbuf(jac ? BLK * 54 : 0);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  for (int e0 = el_begin; e0 < el_end; e0 += BLK) {
+    int e1 = std::min(el_end, e0 + BLK);
+#pragma omp parallel for schedule(static)
+    for (int e = e0; e < e1; e++) {
+      Elem E;
+      copyToLocal(E, coords, conn, e, TL, EU);
+      vreal R[9][6];
+      int l = e - e0;
+      elementFD(ph, E, 0, jac != 0, R, jac ? &Abuf[(size_t)l * 2916] : nullptr, jac ? &Acbuf[(size_t)l * 54] : nullptr);
+      to_double(&R[0][0], &Rbuf[(size_t)l * 54], 54);
+    }
+    for (int e = e0; e < e1; e++) {
+      int l = e - e0;
+      if (scatter_element(conn + (size_t)e * 9, &Rbuf[(size_t)l * 54], jac ? &Abuf[(size_t)l * 2916] : nullptr,
+                          jac ? &Acbuf[(size_t)l * 54] : nullptr, N, rowptr, col, b, vals))
+        return -1;
+    }
+  }
+  if (el_begin == 0 && el_end == n_el) {
+    for (int k = 0; k < n_edges; k++) {
+      int e = edges[2 * k], ied = edges[2 * k + 1];
+      Elem E;
+      copyToLocal(E, coords, conn, e, TL, EU);
+      vreal R[9][6];
+      double Rd[54];
+      std::vector<double> A(jac ? 2916 : 0), Ac(jac ? 54 : 0);
+      elementFD(ph, E, ied, jac != 0, R, jac ? A.data() : nullptr, jac ? Ac.data() : nullptr);
+      to_double(&R[0][0], Rd, 54);
+      if (scatter_element(conn + (size_t)e * 9, Rd, jac ? A.data() : nullptr, jac ? Ac.data() : nullptr, N, rowptr,
+                          col, b, vals))
+        return -1;
+    }
+  }
+  return 0;
+}
+
+#else  // VEF_COUNT_FLOPS
+
+// Operation counts of one element-residual evaluation (bulk or edge) at a given
+// state: out = [add, mul, div, sqrt, total].
+int vo_count_flops(const vo_params* p, const double* xy, const double* TL, double EU, int ied, uint64_t* out) {
+  Phys ph = make_phys(p);
+  Elem E;
+  for (int n = 0; n < 9; n++) {
+    for (int q = 0; q < 6; q++) E.TL[n][q] = vreal(TL[n * 6 + q]);
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0.0;
+  }
+  E.EU = vreal(EU);
+  vreal R[9][6];
+  FlopCounter::reset();
+  if (ied == 0) residualBulk(ph, E, R);
+  else residualEdge(ph, E, ied, R);
+  out[0] = FlopCounter::add;
+  out[1] = FlopCounter::mul;
+  out[2] = FlopCounter::div;
+  out[3] = FlopCounter::sqrt;
+  out[4] = FlopCounter::total();
+  return 0;
+}
+
+#endif
+
+}  // extern "C"

@@ -1,0 +1,1124 @@
+// vef_assembly.hip — MI355X (gfx950) residual and forward-difference Jacobian
+// assembly for the 2D square-root-conformation viscoelastic cylinder problem,
+// behind the C ABI of include/vef.h.
+//
+// What runs where
+//   vef_bulk_kernel<1,56>  one workgroup per element, 504 lanes = 9 Gauss
+//                          points x 56 evaluations (base state, the 54 nodal
+//                          perturbations, the EU perturbation).  Each lane
+//                          evaluates the Gauss-point strong form and the 9
+//                          test-node residual rows at its state; an ordered
+//                          LDS reduction over the Gauss points gives the 54
+//                          element residual rows of every evaluation, written
+//                          as the per-element bank R[e][row][k].
+//   vef_bulk_kernel<7,1>   residual only: one wavefront per 7 elements.
+//   vef_edge_kernel        outlet-edge traction (openBC) bank, 3 edge Gauss
+//                          points x 56 evaluations per outlet edge.
+//   vef_gather_kernel      CSR values: each matrix slot sums the
+//                          forward-difference quotients of its contributing
+//                          elements in ascending element order, then outlet
+//                          edges in ascending edge order — the order of
+//                          PETSc's ADD_VALUES calls in the reference — so the
+//                          result is deterministic and bitwise reproducible.
+//   vef_rhs_kernel         residual vector b in the same order.
+//
+// The arithmetic of each lane restates src/Element_Calculations_mod.F90,
+// src/Constitutive_Models_SQRT_mod.F90 and src/Residuals_mod.F90 in 2D form:
+// the reference's 3x3 tensors have a zero third row and column here, and the
+// terms they contribute are exact zeros whose addition leaves every value
+// unchanged, so they are dropped.  The order of every remaining operation is
+// the reference's (left-to-right sums, sequential dot products, complex
+// divisions with zero imaginary part rounded as (a*c)/(c*c)), and the file is
+// compiled with -ffp-contract=off, so the FP64 results equal the sequential
+// reference evaluation bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vef.h"
+
+namespace {
+
+constexpr double kEpsJac = 1.0e-8;  // perturbVariable, src/Tools_mod.F90:831-840
+constexpr int kNeval = 56;          // 1 base + 54 nodal + 1 EU evaluations
+
+// ----------------------------------------------------------------------------
+// Gauss tables (src/GaussIntegration/Gauss_Quadrangle_mod.F90), host-built.
+// ----------------------------------------------------------------------------
+struct GaussTables {
+  double W2[9];
+  double W1[3];
+  double bf[9][9], dc[9][9], de[9][9], dcc[9][9], dee[9][9], dce[9][9];  // [gp][node]
+  double bfE[4][3][9], dcE[4][3][9], deE[4][3][9];                       // [edge][gp][node]
+};
+__constant__ GaussTables c_tab;
+
+// setQuadraticQuadrangle, Gauss_Quadrangle_mod.F90:405-478: Lagrange products
+// of the 1D quadratics, evaluated in the source's operation order.
+void q2_basis(double c, double e, double* bfn, double* dfdc, double* dfde, double* d2fdc2, double* d2fde2,
+              double* d2fdce) {
+  const double a = 1.0 - c, b = 1.0 + c, p = 1.0 - e, q = 1.0 + e;
+  bfn[0] = a * c * p * e / 4.0;
+  bfn[1] = -(b * c * p * e / 4.0);
+  bfn[2] = b * c * q * e / 4.0;
+  bfn[3] = -(a * c * q * e / 4.0);
+  bfn[4] = -(b * a * p * e / 2.0);
+  bfn[5] = b * c * q * p / 2.0;
+  bfn[6] = b * a * q * e / 2.0;
+  bfn[7] = -(a * c * q * p / 2.0);
+  bfn[8] = b * a * q * p;
+  const double c2m = 1.0 - 2.0 * c, c2p = 1.0 + 2.0 * c, cm2 = -2.0 * c;
+  const double e2m = 1.0 - 2.0 * e, e2p = 1.0 + 2.0 * e, em2 = -2.0 * e;
+  dfdc[0] = c2m * p * e / 4.0;
+  dfdc[1] = -(c2p * p * e / 4.0);
+  dfdc[2] = c2p * q * e / 4.0;
+  dfdc[3] = -(c2m * q * e / 4.0);
+  dfdc[4] = -(cm2 * p * e / 2.0);
+  dfdc[5] = c2p * q * p / 2.0;
+  dfdc[6] = cm2 * q * e / 2.0;
+  dfdc[7] = -(c2m * q * p / 2.0);
+  dfdc[8] = cm2 * q * p;
+  d2fdc2[0] = -2.0 * p * e / 4.0;
+  d2fdc2[1] = -(2.0 * p * e / 4.0);
+  d2fdc2[2] = 2.0 * q * e / 4.0;
+  d2fdc2[3] = -(-2.0 * q * e / 4.0);
+  d2fdc2[4] = -(-2.0 * p * e / 2.0);
+  d2fdc2[5] = 2.0 * q * p / 2.0;
+  d2fdc2[6] = -2.0 * q * e / 2.0;
+  d2fdc2[7] = -(-2.0 * q * p / 2.0);
+  d2fdc2[8] = -2.0 * q * p;
+  d2fdce[0] = c2m * e2m / 4.0;
+  d2fdce[1] = -(c2p * e2m / 4.0);
+  d2fdce[2] = c2p * e2p / 4.0;
+  d2fdce[3] = -(c2m * e2p / 4.0);
+  d2fdce[4] = -(cm2 * e2m / 2.0);
+  d2fdce[5] = c2p * em2 / 2.0;
+  d2fdce[6] = cm2 * e2p / 2.0;
+  d2fdce[7] = -(c2m * em2 / 2.0);
+  d2fdce[8] = cm2 * em2;
+  dfde[0] = a * c * e2m / 4.0;
+  dfde[1] = -(b * c * e2m / 4.0);
+  dfde[2] = b * c * e2p / 4.0;
+  dfde[3] = -(a * c * e2p / 4.0);
+  dfde[4] = -(b * a * e2m / 2.0);
+  dfde[5] = b * c * em2 / 2.0;
+  dfde[6] = b * a * e2p / 2.0;
+  dfde[7] = -(a * c * em2 / 2.0);
+  dfde[8] = b * a * em2;
+  d2fde2[0] = a * c * -2.0 / 4.0;
+  d2fde2[1] = -(b * c * -2.0 / 4.0);
+  d2fde2[2] = b * c * 2.0 / 4.0;
+  d2fde2[3] = -(a * c * 2.0 / 4.0);
+  d2fde2[4] = -(b * a * -2.0 / 2.0);
+  d2fde2[5] = b * c * -2.0 / 2.0;
+  d2fde2[6] = b * a * 2.0 / 2.0;
+  d2fde2[7] = -(a * c * -2.0 / 2.0);
+  d2fde2[8] = b * a * -2.0;
+}
+
+void build_gauss_tables(GaussTables& T) {
+  // 3-point rule with the centre abscissa shifted by 1e-8 (Gauss_Quadrangle_mod.F90:69-72,166-169)
+  const double P1[3] = {0.0 + 1.0e-8, +std::sqrt(3.0 / 5.0), -std::sqrt(3.0 / 5.0)};
+  const double W1[3] = {8.0 / 9.0, 5.0 / 9.0, 5.0 / 9.0};
+  for (int i = 0; i < 3; i++) T.W1[i] = W1[i];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      int g = i * 3 + j;  // xi outer, eta inner (:93-99)
+      T.W2[g] = W1[i] * W1[j];
+      q2_basis(P1[i], P1[j], T.bf[g], T.dc[g], T.de[g], T.dcc[g], T.dee[g], T.dce[g]);
+    }
+  for (int ed = 0; ed < 4; ed++)
+    for (int g = 0; g < 3; g++) {
+      double c = (ed == 1) ? 1.0 : (ed == 3) ? -1.0 : P1[g];
+      double e = (ed == 0) ? -1.0 : (ed == 2) ? 1.0 : P1[g];
+      double t1[9], t2[9], t3[9];  // second derivatives: not used on edges
+      q2_basis(c, e, T.bfE[ed][g], T.dcE[ed][g], T.deE[ed][g], t1, t2, t3);
+    }
+}
+
+struct DevParams {
+  double ReN, BvN, VeN, DfN, WiN0;
+  double dt;    // +huge(1d0) off the transient path (src/Physical_mod.F90:345)
+  double omb;   // 1 - BvN
+  int model;    // 0 L-PTT, 1 Giesekus
+  int dcs_on;   // DfN <= 1e-12
+};
+
+// Geometry of one element at one Gauss point (basis2D, Element_Calculations_mod.F90:308-361;
+// Helem from setStabilizationTerms :718-727).
+struct Geo {
+  double dFx[9], dFy[9], d2x[9], d2y[9];
+  double WET, H2;
+};
+
+// Complex division with zero imaginary parts: re((a+0i)/(c+0i)) = (a*c)/(c*c).
+__device__ __forceinline__ double cdiv(double a, double c) { return (a * c) / (c * c); }
+
+// basis2D for one Gauss point from the nine node coordinates; fills Geo.
+__device__ void geometry(const double* X /*[9][2]*/, const double* dc, const double* de, const double* dcc,
+                         const double* dee, const double* dce, double wgt, Geo& G) {
+  double dxdc = dc[0] * X[0], dydc = dc[0] * X[1], dxde = de[0] * X[0], dyde = de[0] * X[1];
+#pragma unroll
+  for (int n = 1; n < 9; n++) {
+    dxdc = dxdc + dc[n] * X[2 * n];
+    dydc = dydc + dc[n] * X[2 * n + 1];
+    dxde = dxde + de[n] * X[2 * n];
+    dyde = dyde + de[n] * X[2 * n + 1];
+  }
+  const double detJ = dxdc * dyde - dxde * dydc;
+  const double dcdx = dyde / detJ;
+  const double dedx = -(dydc / detJ);
+  const double dcdy = -(dxde / detJ);
+  const double dedy = dxdc / detJ;
+  double H = 0.0;
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    const double fx = dcdx * dc[n] + dedx * de[n];
+    const double fy = dcdy * dc[n] + dedy * de[n];
+    G.dFx[n] = fx;
+    G.dFy[n] = fy;
+    G.d2x[n] = (dcdx * dcdx) * dcc[n] + (dedx * dedx) * dee[n] + 2.0 * dcdx * dedx * dce[n];
+    G.d2y[n] = (dcdy * dcdy) * dcc[n] + (dedy * dedy) * dee[n] + 2.0 * dcdy * dedy * dce[n];
+    H = (n == 0) ? fx * fx + fy * fy : H + fx * fx + fy * fy;
+  }
+  const double Helem = 1.0 / sqrt(H);
+  G.H2 = Helem * Helem;
+  G.WET = wgt * fabs(detJ);
+}
+
+// Gauss-point quantities that the test-node terms read (GaussPointQuantities,
+// src/Global_Variables_mod.F90:56-84, 2D part).
+struct GP {
+  double Ux, Uy;
+  double Mom0, Mom1, Cont;
+  double RC0, RC1;  // ReN*Convection(i)
+  double P00, P01, P10, P11;  // P_tot
+  double CE00, CE01, CE10, CE11;  // Con_EQ
+  double S00, S01, S11, dW;  // S and S11+S22 (omega denominator)
+  double dS00x, dS00y, dS01x, dS01y, dS11x, dS11y;
+  double WiN, f_T;
+  double tlsme, tlsic, tlsce, tdcs, mag_re;
+};
+
+// setFlowUnknowns .. setStabilizationTerms for one state.  v(n, q) reads the
+// lane's nodal value of unknown q at node n; bf are the basis values at the
+// Gauss point; H2 = Helem**2.
+template <class V>
+__device__ __forceinline__ void gauss_point(const V& v, const double* bf, const Geo& G, double EU,
+                                            const DevParams& prm, GP& o) {
+  // Interpolation, each sum sequential over the nodes (dot_product).
+  double Ux, Uy, GU00, GU10, GU01, GU11, G2x0, G2y0, G2x1, G2y1, P, dPx, dPy;
+  double S00, S01, S11, dS00x, dS00y, dS01x, dS01y, dS11x, dS11y;
+  {
+    const double b = bf[0], fx = G.dFx[0], fy = G.dFy[0], gx = G.d2x[0], gy = G.d2y[0];
+    const double u0 = v(0, 0), u1 = v(0, 1), u2 = v(0, 2), u3 = v(0, 3), u4 = v(0, 4), u5 = v(0, 5);
+    Ux = u0 * b; GU00 = u0 * fx; GU10 = u0 * fy; G2x0 = u0 * gx; G2y0 = u0 * gy;
+    Uy = u1 * b; GU01 = u1 * fx; GU11 = u1 * fy; G2x1 = u1 * gx; G2y1 = u1 * gy;
+    P = u2 * b; dPx = u2 * fx; dPy = u2 * fy;
+    S00 = u3 * b; dS00x = u3 * fx; dS00y = u3 * fy;
+    S01 = u4 * b; dS01x = u4 * fx; dS01y = u4 * fy;
+    S11 = u5 * b; dS11x = u5 * fx; dS11y = u5 * fy;
+  }
+#pragma unroll
+  for (int n = 1; n < 9; n++) {
+    const double b = bf[n], fx = G.dFx[n], fy = G.dFy[n], gx = G.d2x[n], gy = G.d2y[n];
+    const double u0 = v(n, 0), u1 = v(n, 1), u2 = v(n, 2), u3 = v(n, 3), u4 = v(n, 4), u5 = v(n, 5);
+    Ux = Ux + u0 * b; GU00 = GU00 + u0 * fx; GU10 = GU10 + u0 * fy; G2x0 = G2x0 + u0 * gx; G2y0 = G2y0 + u0 * gy;
+    Uy = Uy + u1 * b; GU01 = GU01 + u1 * fx; GU11 = GU11 + u1 * fy; G2x1 = G2x1 + u1 * gx; G2y1 = G2y1 + u1 * gy;
+    P = P + u2 * b; dPx = dPx + u2 * fx; dPy = dPy + u2 * fy;
+    S00 = S00 + u3 * b; dS00x = dS00x + u3 * fx; dS00y = dS00y + u3 * fy;
+    S01 = S01 + u4 * b; dS01x = dS01x + u4 * fx; dS01y = dS01y + u4 * fy;
+    S11 = S11 + u5 * b; dS11x = dS11x + u5 * fx; dS11y = dS11y + u5 * fy;
+  }
+  const double S10 = S01;
+
+  // setContinuationParameter: WiN = WiN0*EU(Nex)
+  const double WiN = prm.WiN0 * EU;
+
+  // setFlowKinematics (N = 2): traceless gradient, G_dot_tr
+  const double tr = GU00 + GU11;
+  const double htr = (tr * 2.0) / (2.0 * 2.0);  // (traceTensor*I)/N as a complex division
+  const double Gt00 = GU00 - htr, Gt01 = GU01, Gt10 = GU10, Gt11 = GU11 - htr;
+  const double Gd00 = Gt00 + Gt00, Gd01 = Gt01 + Gt10, Gd10 = Gt10 + Gt01, Gd11 = Gt11 + Gt11;
+
+  // setCTensorSQRT: C = S.S, dC/dx_k = dS_k.S + S.dS_k
+  const double C00 = S00 * S00 + S01 * S10, C01 = S00 * S01 + S01 * S11;
+  const double C10 = S10 * S00 + S11 * S10, C11 = S10 * S01 + S11 * S11;
+  const double dC00x = (dS00x * S00 + dS01x * S10) + (S00 * dS00x + S01 * dS01x);
+  const double dC01x = (dS00x * S01 + dS01x * S11) + (S00 * dS01x + S01 * dS11x);
+  const double dC10x = (dS01x * S00 + dS11x * S10) + (S10 * dS00x + S11 * dS01x);
+  const double dC11x = (dS01x * S01 + dS11x * S11) + (S10 * dS01x + S11 * dS11x);
+  const double dC00y = (dS00y * S00 + dS01y * S10) + (S00 * dS00y + S01 * dS01y);
+  const double dC01y = (dS00y * S01 + dS01y * S11) + (S00 * dS01y + S01 * dS11y);
+  const double dC10y = (dS01y * S00 + dS11y * S10) + (S10 * dS00y + S11 * dS01y);
+  const double dC11y = (dS01y * S01 + dS11y * S11) + (S10 * dS01y + S11 * dS11y);
+
+  // conEQ_L_PTT_SQRT / conEQ_Giesekus_SQRT: f(1..3), f_T (g = 1, dg/dxi = 0)
+  double f0, f1, f2, f_T;
+  if (prm.model == 0) {
+    const double trC = C00 + C11;
+    f_T = 1.0 + prm.VeN * (trC - 2.0);
+    f0 = f_T;
+    f1 = f_T;
+    f2 = 0.0;
+  } else {
+    f_T = 1.0 - 2.0 * prm.VeN;
+    f0 = 1.0 - prm.VeN;
+    f1 = 1.0 - 2.0 * prm.VeN;
+    f2 = prm.VeN;
+  }
+
+  // setStressTensorSQRT: T = ((1-BvN)/WiN)(C - I), P_tot = T + BvN G_dot_tr - P I
+  const double a = prm.omb / WiN;
+  const double T00 = a * (C00 - 1.0), T01 = a * (C01 - 0.0), T10 = a * (C10 - 0.0), T11 = a * (C11 - 1.0);
+  const double T22 = a * (0.0 - 1.0);
+  o.P00 = T00 + prm.BvN * Gd00 - P;
+  o.P01 = T01 + prm.BvN * Gd01;
+  o.P10 = T10 + prm.BvN * Gd10;
+  o.P11 = T11 + prm.BvN * Gd11 - P;
+
+  // setUpperConvectedDerivativeSQRT: UCD = U.grad S - S.GU_tr - W.S
+  const double W12 = (-(S00 * Gt01) + S01 * (Gt00 - Gt11) + S11 * Gt10) / (S00 + S11);
+  const double W21 = -W12;
+  double U00 = Ux * dS00x + Uy * dS00y, U01 = Ux * dS01x + Uy * dS01y;
+  double U10 = U01, U11 = Ux * dS11x + Uy * dS11y;
+  U00 = U00 - (S00 * Gt00 + S01 * Gt10) - W12 * S10;
+  U01 = U01 - (S00 * Gt01 + S01 * Gt11) - W12 * S11;
+  U10 = U10 - (S10 * Gt00 + S11 * Gt10) - W21 * S00;
+  U11 = U11 - (S10 * Gt01 + S11 * Gt11) - W21 * S01;
+
+  // setSInverse (inverseTensor2D)
+  const double detS = S00 * S11 - S01 * S10;
+  const double SI00 = S11 / detS, SI01 = (-S01) / detS, SI10 = (-S10) / detS, SI11 = S00 / detS;
+
+  // term_S = f3 S^3 + f2 S - f1 S^-1 ; Con_EQ = WiN*UCD + term_S/2
+  double tS00, tS01, tS10, tS11;
+  if (prm.model == 0) {
+    tS00 = f1 * S00 - f0 * SI00;
+    tS01 = f1 * S01 - f0 * SI01;
+    tS10 = f1 * S10 - f0 * SI10;
+    tS11 = f1 * S11 - f0 * SI11;
+  } else {
+    const double S3_00 = S00 * C00 + S01 * C10, S3_01 = S00 * C01 + S01 * C11;
+    const double S3_10 = S10 * C00 + S11 * C10, S3_11 = S10 * C01 + S11 * C11;
+    tS00 = f2 * S3_00 + f1 * S00 - f0 * SI00;
+    tS01 = f2 * S3_01 + f1 * S01 - f0 * SI01;
+    tS10 = f2 * S3_10 + f1 * S10 - f0 * SI10;
+    tS11 = f2 * S3_11 + f1 * S11 - f0 * SI11;
+  }
+  const double CE00 = WiN * U00 + tS00 / 2.0, CE01 = WiN * U01 + tS01 / 2.0;
+  const double CE10 = WiN * U10 + tS10 / 2.0, CE11 = WiN * U11 + tS11 / 2.0;
+
+  // strongFormMomentum: Momentum = ReN*(U.grad U) - (-grad P + div T + BvN lap U)
+  const double divT0 = a * dC00x + a * dC10y;
+  const double divT1 = a * dC01x + a * dC11y;
+  const double dP0 = -dPx + divT0 + prm.BvN * (G2x0 + G2y0);
+  const double dP1 = -dPy + divT1 + prm.BvN * (G2x1 + G2y1);
+  const double Cv0 = Ux * GU00 + Uy * GU10;
+  const double Cv1 = Ux * GU01 + Uy * GU11;
+  o.RC0 = prm.ReN * Cv0;
+  o.RC1 = prm.ReN * Cv1;
+  o.Mom0 = o.RC0 - dP0;
+  o.Mom1 = o.RC1 - dP1;
+  o.Cont = GU00 + GU11;  // strongFormContinuity
+
+  // setStabilizationTerms
+  double tsugn = fabs(Ux * G.dFx[0] + Uy * G.dFy[0]);
+#pragma unroll
+  for (int n = 1; n < 9; n++) tsugn = tsugn + fabs(Ux * G.dFx[n] + Uy * G.dFy[n]);
+  const double H2 = G.H2;
+  const double ddT = T00 * T00 + T01 * T10 + T10 * T01 + T11 * T11 + T22 * T22;
+  const double ddG = Gd00 * Gd00 + Gd01 * Gd10 + Gd10 * Gd01 + Gd11 * Gd11;
+  double Ha = sqrt(a * a + 0.5 * ddT);
+  Ha = cdiv(Ha, sqrt(1.0 + 0.5 * ddG));
+  const double r1 = prm.ReN / prm.dt, r2 = prm.ReN * tsugn, r3 = prm.BvN / H2, r4 = cdiv(Ha, H2);
+  const double tlsme = cdiv(1.0, sqrt(r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4));
+  o.tlsme = tlsme;
+  o.tlsic = cdiv(H2, tlsme);
+  double tdcs = 0.0, mag_re = 0.0, mag_im = 0.0;
+  if (prm.dcs_on) {
+    const double ddS = S00 * S00 + S01 * S10 + S10 * S01 + S11 * S11;
+    tdcs = cdiv(H2, sqrt(0.5 * ddS));
+    const double mc = 0.5 * (CE00 * CE00 + CE01 * CE10 + CE10 * CE01 + CE11 * CE11);
+    if (mc >= 0.0) mag_re = sqrt(mc);
+    else mag_im = sqrt(-mc);
+  }
+  o.tdcs = tdcs;
+  o.mag_re = mag_re;
+  const double t1 = WiN / prm.dt, t2 = WiN * tsugn, t3 = WiN * sqrt(0.5 * ddG), t4 = f1;
+  double sum = t1 * t1 + t2 * t2 + t3 * t3 + t4 * t4;
+  if (prm.model != 0) {
+    const double ddC = C00 * C00 + C01 * C10 + C10 * C01 + C11 * C11;
+    const double t5 = f2 * sqrt(0.5 * ddC);
+    sum = sum + t5 * t5;
+  }
+  if (prm.dcs_on) {
+    const double zr = cdiv(tdcs * mag_re, H2), zi = cdiv(tdcs * mag_im, H2);
+    sum = sum + (zr * zr - zi * zi);
+  }
+  const double t7 = prm.DfN / H2;
+  sum = sum + t7 * t7;
+  o.tlsce = cdiv(1.0, sqrt(sum));
+
+  o.Ux = Ux; o.Uy = Uy;
+  o.CE00 = CE00; o.CE01 = CE01; o.CE10 = CE10; o.CE11 = CE11;
+  o.S00 = S00; o.S01 = S01; o.S11 = S11; o.dW = S00 + S11;
+  o.dS00x = dS00x; o.dS00y = dS00y; o.dS01x = dS01x; o.dS01y = dS01y; o.dS11x = dS11x; o.dS11y = dS11y;
+  o.WiN = WiN; o.f_T = f_T;
+}
+
+// Test-node rows of one Gauss point (momentumMainSQRT, continuityBulkMain,
+// stressesBulkMainSQRT and their LSME/LSIC/LSCE/DCS terms, Residuals_mod.F90:
+// 90-110,161-234,295-490), already multiplied by WET.
+__device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, double fy, const DevParams& prm,
+                                           double WET, double* out6) {
+  const double S10 = g.S01;
+  const double Udf = g.Ux * fx + g.Uy * fy;
+  const double lsme_m = g.tlsme * prm.ReN * Udf;
+  const double wscale = g.WiN * g.WiN;
+  double T0, T1;
+  {
+    // i = 1: GUT_lsce row 1 = grad(phi)
+    const double lsme = lsme_m * g.Mom0;
+    const double lsic = (g.tlsic * fx) * g.Cont;
+    const double w = (g.S01 * fx + g.S11 * fy) / g.dW;  // omegaNominator2D / omegaDenominator2D
+    const double m00 = g.S00 * fx + g.S01 * fy, m10 = S10 * fx + g.S11 * fy;
+    const double u00 = -m00 - w * S10, u01 = -(w * g.S11);
+    const double u10 = -m10 - (-w) * g.S00, u11 = -((-w) * g.S01);
+    const double l00 = ((g.tlsce * u00) * prm.omb * g.WiN) / wscale;
+    const double l01 = ((g.tlsce * u01) * prm.omb * g.WiN) / wscale;
+    const double l10 = ((g.tlsce * u10) * prm.omb * g.WiN) / wscale;
+    const double l11 = ((g.tlsce * u11) * prm.omb * g.WiN) / wscale;
+    const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
+    T0 = g.RC0 * bifn + (g.P00 * fx + g.P01 * fy) + lsme + lsic + lsce;
+  }
+  {
+    // i = 2: GUT_lsce row 2 = grad(phi)
+    const double lsme = lsme_m * g.Mom1;
+    const double lsic = (g.tlsic * fy) * g.Cont;
+    const double w = (-(g.S00 * fx) + g.S01 * (-fy)) / g.dW;
+    const double m01 = g.S00 * fx + g.S01 * fy, m11 = S10 * fx + g.S11 * fy;
+    const double u00 = -(w * S10), u01 = -m01 - w * g.S11;
+    const double u10 = -((-w) * g.S00), u11 = -m11 - (-w) * g.S01;
+    const double l00 = ((g.tlsce * u00) * prm.omb * g.WiN) / wscale;
+    const double l01 = ((g.tlsce * u01) * prm.omb * g.WiN) / wscale;
+    const double l10 = ((g.tlsce * u10) * prm.omb * g.WiN) / wscale;
+    const double l11 = ((g.tlsce * u11) * prm.omb * g.WiN) / wscale;
+    const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
+    T1 = g.RC1 * bifn + (g.P10 * fx + g.P11 * fy) + lsme + lsic + lsce;
+  }
+  // continuity row
+  const double T2 = g.Cont * bifn + g.tlsme * (fx * g.Mom0 + fy * g.Mom1);
+  // stress rows (xx, xy, yy)
+  double lsce_s = g.WiN * Udf;
+  lsce_s = lsce_s + ((g.f_T * bifn) * 2.0) / (2.0 * 2.0);
+  lsce_s = g.tlsce * lsce_s;
+  const double d00 = g.dS00x * fx + g.dS00y * fy;
+  const double d01 = g.dS01x * fx + g.dS01y * fy;
+  const double d11 = g.dS11x * fx + g.dS11y * fy;
+  double T3 = g.CE00 * bifn + prm.DfN * d00 + (g.tlsme * (-(fx * g.Mom0)) * g.WiN) + lsce_s * g.CE00;
+  double T4 = g.CE01 * bifn + prm.DfN * d01 + (g.tlsme * (-(fy * g.Mom0) - fx * g.Mom1) * g.WiN) + lsce_s * g.CE01;
+  double T5 = g.CE11 * bifn + prm.DfN * d11 + (g.tlsme * (-(fy * g.Mom1)) * g.WiN) + lsce_s * g.CE11;
+  if (prm.dcs_on) {
+    T3 = T3 + (g.tdcs * d00) * g.mag_re;
+    T4 = T4 + (g.tdcs * d01) * g.mag_re;
+    T5 = T5 + (g.tdcs * d11) * g.mag_re;
+  }
+  out6[0] = T0 * WET;
+  out6[1] = T1 * WET;
+  out6[2] = T2 * WET;
+  out6[3] = T3 * WET;
+  out6[4] = T4 * WET;
+  out6[5] = T5 * WET;
+}
+
+// Lane state: unknown j of the element is u[j] for j > p, fl(u+h) for j == p
+// and fl(fl(u+h)-h) for j < p — the in-place perturb/restore sequence of
+// storeBulk (Equations_mod.F90:267-284) seen by perturbation p.
+struct LaneState {
+  const double* u;  // [3][54] in LDS: original, restored (drifted), perturbed
+  int p;
+  __device__ __forceinline__ double operator()(int n, int q) const {
+    const int j = n * 6 + q;
+    const int sel = (j < p) ? 54 : ((j == p) ? 108 : 0);
+    return u[sel + j];
+  }
+};
+struct PlainState {
+  const double* u;  // [54]
+  __device__ __forceinline__ double operator()(int n, int q) const { return u[n * 6 + q]; }
+};
+
+// ----------------------------------------------------------------------------
+// Bulk kernel.  EPB elements per workgroup, NEVAL evaluations per element.
+// Lane t < 9*NSLOT: Gauss point g = t / NSLOT, slot s = t % NSLOT,
+// element el = s / NEVAL, evaluation k = s % NEVAL.
+// ----------------------------------------------------------------------------
+template <int EPB, int NEVAL>
+__global__ void __launch_bounds__(((9 * EPB * NEVAL + 63) / 64) * 64)
+vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
+                int64_t eu_index, int32_t n_el, DevParams prm, double* __restrict__ bank) {
+  constexpr int NSLOT = EPB * NEVAL;
+  constexpr int NLANE = 9 * NSLOT;
+  constexpr int NSTATE = (NEVAL > 1) ? 3 : 1;
+  __shared__ double sU[EPB][NSTATE * 54];
+  __shared__ double sX[EPB][18];
+  __shared__ Geo sG[EPB][9];
+  __shared__ double sRed[2][9][NSLOT][6];
+
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const double EU = x[eu_index];
+
+  // Gather element unknowns and coordinates (copyToLocal).
+  for (int t = tid; t < EPB * 54; t += blockDim.x) {
+    const int el = t / 54, j = t % 54;
+    const int64_t e = e0 + el;
+    double u = 0.0;
+    if (e < n_el) {
+      const int32_t gn = conn[e * 9 + j / 6];
+      u = x[(int64_t)gn * 6 + (j % 6)];
+    }
+    sU[el][j] = u;
+    if (NSTATE == 3) {
+      const double up = u + kEpsJac;
+      sU[el][108 + j] = up;
+      sU[el][54 + j] = up - kEpsJac;
+    }
+  }
+  for (int t = tid; t < EPB * 18; t += blockDim.x) {
+    const int el = t / 18, c = t % 18;
+    const int64_t e = e0 + el;
+    double v = (c & 1) ? 1.0 : 0.0;
+    if (e < n_el) v = coords[(int64_t)conn[e * 9 + c / 2] * 2 + (c & 1)];
+    sX[el][c] = v;
+  }
+  __syncthreads();
+  for (int t = tid; t < EPB * 9; t += blockDim.x) {
+    const int el = t / 9, g = t % 9;
+    geometry(sX[el], c_tab.dc[g], c_tab.de[g], c_tab.dcc[g], c_tab.dee[g], c_tab.dce[g], c_tab.W2[g], sG[el][g]);
+  }
+  __syncthreads();
+
+  const bool active = tid < NLANE;
+  const int g = active ? tid / NSLOT : 0;
+  const int s = tid % NSLOT;
+  const int el = s / NEVAL;
+  const int k = s % NEVAL;
+  const bool valid = active && (e0 + el < n_el);
+
+  GP q;
+  if (valid) {
+    const double EUk = (NEVAL > 1 && k == NEVAL - 1) ? EU + kEpsJac : EU;
+    if (NEVAL > 1) {
+      LaneState st{sU[el], k - 1};
+      gauss_point(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
+    } else {
+      PlainState st{sU[el]};
+      gauss_point(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
+    }
+  }
+  const double WET = sG[el][g].WET;
+
+  for (int n = 0; n < 9; n++) {
+    const int buf = n & 1;
+    if (valid) {
+      double r[6];
+      node_terms(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET, r);
+#pragma unroll
+      for (int c = 0; c < 6; c++) sRed[buf][g][s][c] = r[c];
+    }
+    __syncthreads();
+    // Ordered sum over the Gauss points: Res = Res + TERM_RES*WET, ig = 1..9.
+    for (int t = tid; t < 6 * NSLOT; t += blockDim.x) {
+      const int c = t / NSLOT, ss = t % NSLOT;
+      const int eel = ss / NEVAL, kk = ss % NEVAL;
+      if (e0 + eel < n_el) {
+        double acc = sRed[buf][0][ss][c];
+#pragma unroll
+        for (int gg = 1; gg < 9; gg++) acc = acc + sRed[buf][gg][ss][c];
+        bank[((e0 + eel) * 54 + n * 6 + c) * NEVAL + kk] = acc;
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Outlet edges: openBC traction (Residuals_mod.F90:599-624) at the 3 edge
+// Gauss points, with setEdgeQuantities normals (Element_Calculations_mod.F90:887-916).
+// Bank layout: ebank[(edge*18 + n*2 + i)*56 + k], rows (node n, momentum i).
+// ----------------------------------------------------------------------------
+struct EdgeGeo {
+  double dFx[9], dFy[9];
+  double n0, n1, WET;
+};
+
+__global__ void __launch_bounds__(192)
+vef_edge_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const int32_t* __restrict__ edges,
+                const double* __restrict__ x, int64_t eu_index, int32_t n_edges, DevParams prm,
+                double* __restrict__ ebank) {
+  __shared__ double sU[3 * 54];
+  __shared__ double sX[18];
+  __shared__ EdgeGeo sG[3];
+  __shared__ double sRed[2][3][kNeval][2];
+  const int tid = threadIdx.x;
+  const int ed_idx = blockIdx.x;
+  const int32_t e = edges[2 * ed_idx];
+  const int ied = edges[2 * ed_idx + 1];
+  const int ed = ied - 1;
+  const double EU = x[eu_index];
+  for (int j = tid; j < 54; j += blockDim.x) {
+    const double u = x[(int64_t)conn[(int64_t)e * 9 + j / 6] * 6 + (j % 6)];
+    const double up = u + kEpsJac;
+    sU[j] = u;
+    sU[108 + j] = up;
+    sU[54 + j] = up - kEpsJac;
+  }
+  for (int c = tid; c < 18; c += blockDim.x) sX[c] = coords[(int64_t)conn[(int64_t)e * 9 + c / 2] * 2 + (c & 1)];
+  __syncthreads();
+  if (tid < 3) {
+    const int g = tid;
+    const double* dc = c_tab.dcE[ed][g];
+    const double* de = c_tab.deE[ed][g];
+    double dxdc = dc[0] * sX[0], dydc = dc[0] * sX[1], dxde = de[0] * sX[0], dyde = de[0] * sX[1];
+    for (int n = 1; n < 9; n++) {
+      dxdc = dxdc + dc[n] * sX[2 * n];
+      dydc = dydc + dc[n] * sX[2 * n + 1];
+      dxde = dxde + de[n] * sX[2 * n];
+      dyde = dyde + de[n] * sX[2 * n + 1];
+    }
+    const double detJ = dxdc * dyde - dxde * dydc;
+    const double dcdx = dyde / detJ, dedx = -(dydc / detJ), dcdy = -(dxde / detJ), dedy = dxdc / detJ;
+    for (int n = 0; n < 9; n++) {
+      sG[g].dFx[n] = dcdx * dc[n] + dedx * de[n];
+      sG[g].dFy[n] = dcdy * dc[n] + dedy * de[n];
+    }
+    double n0, n1;
+    switch (ied) {
+      case 1: n0 = dydc; n1 = -dxdc; break;
+      case 2: n0 = dyde; n1 = -dxde; break;
+      case 3: n0 = -dydc; n1 = dxdc; break;
+      default: n0 = -dyde; n1 = dxde; break;
+    }
+    const double dL = sqrt(n0 * n0 + n1 * n1);
+    sG[g].n0 = n0 / dL;
+    sG[g].n1 = n1 / dL;
+    sG[g].WET = c_tab.W1[g] * dL;
+  }
+  __syncthreads();
+  const bool active = tid < 3 * kNeval;
+  const int g = active ? tid / kNeval : 0;
+  const int k = tid % kNeval;
+  double P00 = 0, P01 = 0, P10 = 0, P11 = 0, P = 0;
+  if (active) {
+    const int p = k - 1;
+    const double EUk = (k == kNeval - 1) ? EU + kEpsJac : EU;
+    const double* bf = c_tab.bfE[ed][g];
+    auto v = [&](int n, int qq) {
+      const int j = n * 6 + qq;
+      const int sel = (j < p) ? 54 : ((j == p) ? 108 : 0);
+      return sU[sel + j];
+    };
+    double GU00, GU10, GU01, GU11, S00, S01, S11;
+    {
+      const double fx = sG[g].dFx[0], fy = sG[g].dFy[0], b = bf[0];
+      GU00 = v(0, 0) * fx; GU10 = v(0, 0) * fy; GU01 = v(0, 1) * fx; GU11 = v(0, 1) * fy;
+      P = v(0, 2) * b; S00 = v(0, 3) * b; S01 = v(0, 4) * b; S11 = v(0, 5) * b;
+    }
+    for (int n = 1; n < 9; n++) {
+      const double fx = sG[g].dFx[n], fy = sG[g].dFy[n], b = bf[n];
+      GU00 = GU00 + v(n, 0) * fx; GU10 = GU10 + v(n, 0) * fy;
+      GU01 = GU01 + v(n, 1) * fx; GU11 = GU11 + v(n, 1) * fy;
+      P = P + v(n, 2) * b; S00 = S00 + v(n, 3) * b; S01 = S01 + v(n, 4) * b; S11 = S11 + v(n, 5) * b;
+    }
+    const double S10 = S01;
+    const double WiN = prm.WiN0 * EUk;
+    const double tr = GU00 + GU11;
+    const double htr = (tr * 2.0) / (2.0 * 2.0);
+    const double Gt00 = GU00 - htr, Gt01 = GU01, Gt10 = GU10, Gt11 = GU11 - htr;
+    const double Gd00 = Gt00 + Gt00, Gd01 = Gt01 + Gt10, Gd10 = Gt10 + Gt01, Gd11 = Gt11 + Gt11;
+    const double C00 = S00 * S00 + S01 * S10, C01 = S00 * S01 + S01 * S11;
+    const double C10 = S10 * S00 + S11 * S10, C11 = S10 * S01 + S11 * S11;
+    const double a = prm.omb / WiN;
+    P00 = a * (C00 - 1.0) + prm.BvN * Gd00 - P;
+    P01 = a * (C01 - 0.0) + prm.BvN * Gd01;
+    P10 = a * (C10 - 0.0) + prm.BvN * Gd10;
+    P11 = a * (C11 - 1.0) + prm.BvN * Gd11 - P;
+  }
+  for (int n = 0; n < 9; n++) {
+    const int buf = n & 1;
+    if (active) {
+      const double bifn = c_tab.bfE[ed][g][n];
+      const double n0 = sG[g].n0, n1 = sG[g].n1, WET = sG[g].WET;
+      // TERM_RES(i) = -dot_product(normal, P_tot(:,i) + P*I(:,i))*bifn
+      const double T0 = -((n0 * (P00 + P) + n1 * P10) * bifn);
+      const double T1 = -((n0 * P01 + n1 * (P11 + P)) * bifn);
+      sRed[buf][g][k][0] = T0 * WET;
+      sRed[buf][g][k][1] = T1 * WET;
+    }
+    __syncthreads();
+    if (tid < 2 * kNeval) {
+      const int i = tid / kNeval, kk = tid % kNeval;
+      const double acc = (sRed[buf][0][kk][i] + sRed[buf][1][kk][i]) + sRed[buf][2][kk][i];
+      ebank[((int64_t)ed_idx * 18 + n * 2 + i) * kNeval + kk] = acc;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Scatter (storeResidual / storeJacobian / storeJacobian_Ac with ADD_VALUES,
+// Storage_mod.F90:10-115) as a deterministic gather.  Node pair (r, c) of the
+// pattern has the list of bulk contributions (element e, local index jn of r,
+// local index in of c), ascending in e, packed as e*81 + jn*9 + in, then the
+// outlet-edge contributions packed as ed*81 + jn*9 + in, ascending in ed.
+// One wavefront per matrix row; lane l handles row positions l, l+64, ...
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double fd_quotient(double rk, double r0) {
+  // (dRes - Res)/eps_jac with a complex numerator: ((dR-R)*h)/(h*h)
+  return ((rk - r0) * kEpsJac) / (kEpsJac * kEpsJac);
+}
+
+__global__ void __launch_bounds__(256)
+vef_gather_kernel(int64_t n_node_rows, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ pair_start,
+                  const int32_t* __restrict__ cstart, const int32_t* __restrict__ contrib,
+                  const int32_t* __restrict__ estart, const int32_t* __restrict__ econtrib,
+                  const double* __restrict__ bank, const double* __restrict__ ebank, double* __restrict__ values) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n_node_rows) return;
+  const int64_t r = row / 6;
+  const int jq = (int)(row % 6);
+  const int64_t p0 = pair_start[r];
+  const int64_t npos = (pair_start[r + 1] - p0) * 6;  // node-block columns; column N-1 follows
+  const int64_t base = rowptr[row];
+  for (int64_t pos = lane; pos < npos; pos += 64) {
+    const int64_t pp = p0 + pos / 6;
+    const int iq = (int)(pos % 6);
+    double acc = 0.0;
+    bool first = true;
+    for (int32_t c = cstart[pp]; c < cstart[pp + 1]; c++) {
+      const int32_t w = contrib[c];
+      const int64_t e = w / 81;
+      const int jn = (w / 9) % 9, in = w % 9;
+      const double* R = bank + (e * 54 + jn * 6 + jq) * kNeval;
+      const double v = fd_quotient(R[1 + in * 6 + iq], R[0]);
+      acc = first ? v : acc + v;
+      first = false;
+    }
+    if (jq < 2) {
+      for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
+        const int32_t w = econtrib[c];
+        const int64_t ed = w / 81;
+        const int jn = (w / 9) % 9, in = w % 9;
+        const double* R = ebank + (ed * 18 + jn * 2 + jq) * kNeval;
+        const double v = fd_quotient(R[1 + in * 6 + iq], R[0]);
+        acc = first ? v : acc + v;
+        first = false;
+      }
+    }
+    values[base + pos] = acc;
+  }
+}
+
+// Residual b and the dR/dEU column: one thread per node row, summing over the
+// elements (then outlet edges) that contain the node, in ascending order.
+__global__ void __launch_bounds__(256)
+vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ diag_pair,
+               const int32_t* __restrict__ cstart, const int32_t* __restrict__ contrib,
+               const int32_t* __restrict__ estart, const int32_t* __restrict__ econtrib,
+               const double* __restrict__ bank, int bank_stride, const double* __restrict__ ebank,
+               double* __restrict__ values, double* __restrict__ b) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_node_rows) return;
+  const int64_t r = row / 6;
+  const int jq = (int)(row % 6);
+  const int64_t pp = diag_pair[r];
+  double rb = 0.0, ra = 0.0;
+  bool first = true;
+  for (int32_t c = cstart[pp]; c < cstart[pp + 1]; c++) {
+    const int32_t w = contrib[c];
+    const int64_t e = w / 81;
+    const int jn = (w / 9) % 9;
+    const double* R = bank + (e * 54 + jn * 6 + jq) * bank_stride;
+    const double vb = R[0];
+    rb = first ? vb : rb + vb;
+    if (jac) {
+      const double va = fd_quotient(R[kNeval - 1], R[0]);
+      ra = first ? va : ra + va;
+    }
+    first = false;
+  }
+  if (jq < 2) {
+    for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
+      const int32_t w = econtrib[c];
+      const int64_t ed = w / 81;
+      const int jn = (w / 9) % 9;
+      const double* R = ebank + (ed * 18 + jn * 2 + jq) * kNeval;
+      rb = rb + R[0];
+      if (jac) ra = ra + fd_quotient(R[kNeval - 1], R[0]);
+    }
+  }
+  b[row] = rb;
+  if (jac) values[rowptr[row + 1] - 1] = ra;
+}
+
+__global__ void vef_fill_kernel(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+}  // namespace
+
+// =============================================================================
+// Host side
+// =============================================================================
+struct vef_handle {
+  int32_t n_nodes = 0, n_el = 0, n_edges = 0;
+  int64_t N = 0, nnz = 0;
+  vef_params params{};
+  DevParams dprm{};
+  std::vector<int64_t> h_rowptr;
+  std::vector<int32_t> h_col;
+  // device
+  double* d_coords = nullptr;
+  int32_t* d_conn = nullptr;
+  int32_t* d_edges = nullptr;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_col = nullptr;
+  int64_t* d_pair_start = nullptr;
+  int64_t* d_diag_pair = nullptr;
+  int32_t* d_cstart = nullptr;
+  int32_t* d_contrib = nullptr;
+  int32_t* d_estart = nullptr;
+  int32_t* d_econtrib = nullptr;
+  double* d_bank = nullptr;   // n_el*54*56
+  double* d_ebank = nullptr;  // n_edges*18*56
+  double* d_tmp_x = nullptr;  // host-assemble staging
+  double* d_tmp_b = nullptr;
+  double* d_tmp_v = nullptr;
+  int last_flag = -1;
+  std::string err;
+};
+
+namespace {
+std::string g_create_err;
+
+int set_err(vef_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  else g_create_err = msg;
+  return code;
+}
+
+#define VEF_HIP(h, call)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err(h, VEF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int check_params(vef_handle* h, const vef_params* p) {
+  if (!p) return set_err(h, VEF_ERR_ARG, "params is NULL");
+  if (p->model != VEF_MODEL_LPTT && p->model != VEF_MODEL_GIESEKUS)
+    return set_err(h, VEF_ERR_UNSUPPORTED, "model must be VEF_MODEL_LPTT or VEF_MODEL_GIESEKUS");
+  if (p->reserved != 0) return set_err(h, VEF_ERR_ARG, "params.reserved must be 0");
+  if (!(p->WiN0 != 0.0) || !std::isfinite(p->WiN0)) return set_err(h, VEF_ERR_ARG, "WiN0 must be finite and nonzero");
+  return VEF_OK;
+}
+
+DevParams make_dev_params(const vef_params& p) {
+  DevParams d;
+  d.ReN = p.ReN;
+  d.BvN = p.BvN;
+  d.VeN = p.VeN;
+  d.DfN = p.DfN;
+  d.WiN0 = p.WiN0;
+  d.dt = DBL_MAX;
+  d.omb = 1.0 - p.BvN;
+  d.model = p.model;
+  d.dcs_on = (p.DfN > 1.0e-12) ? 0 : 1;
+  return d;
+}
+
+template <class T>
+int dev_upload(vef_handle* h, T** dst, const T* src, size_t n) {
+  if (n == 0) n = 1;
+  VEF_HIP(h, hipMalloc((void**)dst, n * sizeof(T)));
+  if (src) VEF_HIP(h, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return VEF_OK;
+}
+
+int upload_tables(vef_handle* h) {
+  // Gauss tables live in each device's constant memory; upload on the current device.
+  GaussTables T;
+  std::memset(&T, 0, sizeof(T));
+  build_gauss_tables(T);
+  VEF_HIP(h, hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &T, sizeof(T)));
+  return VEF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* vef_last_error(const vef_handle* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int64_t vef_num_rows(const vef_handle* h) { return h ? h->N : -1; }
+int64_t vef_nnz(const vef_handle* h) { return h ? h->nnz : -1; }
+const int64_t* vef_device_rowptr(const vef_handle* h) { return h ? h->d_rowptr : nullptr; }
+const int32_t* vef_device_colind(const vef_handle* h) { return h ? h->d_col : nullptr; }
+
+int vef_get_pattern(const vef_handle* h, int64_t* rowptr, int32_t* colind) {
+  if (!h || !rowptr || !colind) return VEF_ERR_ARG;
+  std::memcpy(rowptr, h->h_rowptr.data(), sizeof(int64_t) * (h->N + 1));
+  std::memcpy(colind, h->h_col.data(), sizeof(int32_t) * h->nnz);
+  return VEF_OK;
+}
+
+void vef_destroy(vef_handle* h) {
+  if (!h) return;
+  void* ptrs[] = {h->d_coords, h->d_conn, h->d_edges, h->d_rowptr, h->d_col, h->d_pair_start, h->d_diag_pair,
+                  h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, h->d_tmp_x,
+                  h->d_tmp_b, h->d_tmp_v};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+int vef_set_params(vef_handle* h, const vef_params* p) {
+  if (!h) return VEF_ERR_ARG;
+  int rc = check_params(h, p);
+  if (rc) return rc;
+  h->params = *p;
+  h->dprm = make_dev_params(*p);
+  return VEF_OK;
+}
+
+int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_t* conn, int32_t n_edges,
+               const int32_t* edges, const vef_params* params, vef_handle** out) {
+  if (!out) return set_err(nullptr, VEF_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n_nodes <= 0 || n_el <= 0 || !coords || !conn) return set_err(nullptr, VEF_ERR_ARG, "empty mesh");
+  if (n_edges < 0 || (n_edges > 0 && !edges)) return set_err(nullptr, VEF_ERR_ARG, "bad outlet edge list");
+  if ((int64_t)n_el * 81 >= INT32_MAX || (int64_t)n_edges * 81 >= INT32_MAX)
+    return set_err(nullptr, VEF_ERR_UNSUPPORTED, "too many elements for the packed scatter map");
+  if ((int64_t)n_nodes * 6 + 1 >= INT32_MAX) return set_err(nullptr, VEF_ERR_UNSUPPORTED, "too many nodes");
+  int rc = check_params(nullptr, params);
+  if (rc) return rc;
+  for (int64_t i = 0; i < (int64_t)n_el * 9; i++)
+    if (conn[i] < 0 || conn[i] >= n_nodes) return set_err(nullptr, VEF_ERR_MESH, "connectivity out of range");
+  for (int64_t e = 0; e < n_el; e++)
+    for (int a = 0; a < 9; a++)
+      for (int b2 = a + 1; b2 < 9; b2++)
+        if (conn[e * 9 + a] == conn[e * 9 + b2])
+          return set_err(nullptr, VEF_ERR_MESH, "element with a repeated node");
+  for (int i = 0; i < n_edges; i++) {
+    if (edges[2 * i] < 0 || edges[2 * i] >= n_el || edges[2 * i + 1] < 1 || edges[2 * i + 1] > 4)
+      return set_err(nullptr, VEF_ERR_MESH, "outlet edge out of range");
+  }
+
+  vef_handle* h = new vef_handle();
+  h->n_nodes = n_nodes;
+  h->n_el = n_el;
+  h->n_edges = n_edges;
+  h->N = (int64_t)n_nodes * 6 + 1;
+  h->params = *params;
+  h->dprm = make_dev_params(*params);
+
+  // Node -> elements (ascending) incidence.
+  std::vector<int64_t> ne_start(n_nodes + 1, 0);
+  for (int64_t i = 0; i < (int64_t)n_el * 9; i++) ne_start[conn[i] + 1]++;
+  for (int i = 0; i < n_nodes; i++) ne_start[i + 1] += ne_start[i];
+  std::vector<int32_t> ne_list(ne_start[n_nodes]);
+  {
+    std::vector<int64_t> fill(ne_start.begin(), ne_start.end() - 1);
+    for (int32_t e = 0; e < n_el; e++)
+      for (int a = 0; a < 9; a++) ne_list[fill[conn[(int64_t)e * 9 + a]]++] = e;  // ascending e by construction
+  }
+  // Node pairs (r, c): neighbours of r sorted; contributions per pair.
+  std::vector<int64_t> pair_start(n_nodes + 1, 0);
+  std::vector<int32_t> pair_col;
+  std::vector<int32_t> cstart, contrib;
+  std::vector<int64_t> diag_pair(n_nodes);
+  pair_col.reserve((size_t)n_el * 64);
+  cstart.reserve((size_t)n_el * 64 + 1);
+  contrib.reserve((size_t)n_el * 81);
+  cstart.push_back(0);
+  std::vector<std::pair<int32_t, int32_t>> tmp;  // (c, packed contribution)
+  for (int32_t r = 0; r < n_nodes; r++) {
+    tmp.clear();
+    for (int64_t t = ne_start[r]; t < ne_start[r + 1]; t++) {
+      const int32_t e = ne_list[t];
+      int jn = 0;
+      while (conn[(int64_t)e * 9 + jn] != r) jn++;
+      for (int in = 0; in < 9; in++) tmp.push_back({conn[(int64_t)e * 9 + in], e * 81 + jn * 9 + in});
+    }
+    std::stable_sort(tmp.begin(), tmp.end(),
+                     [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+                       return a.first < b.first;
+                     });
+    size_t i = 0;
+    while (i < tmp.size()) {
+      const int32_t c = tmp[i].first;
+      if (c == r) diag_pair[r] = (int64_t)pair_col.size();
+      pair_col.push_back(c);
+      while (i < tmp.size() && tmp[i].first == c) contrib.push_back(tmp[i++].second);  // ascending e
+      cstart.push_back((int32_t)contrib.size());
+    }
+    pair_start[r + 1] = (int64_t)pair_col.size();
+  }
+  const int64_t npairs = (int64_t)pair_col.size();
+  // Outlet-edge contributions: rows of the element's nodes, columns of its nodes.
+  std::vector<int32_t> ecount(npairs + 1, 0);
+  auto find_pair = [&](int32_t r, int32_t c) -> int64_t {
+    const int32_t* b0 = pair_col.data() + pair_start[r];
+    const int32_t* b1 = pair_col.data() + pair_start[r + 1];
+    const int32_t* it = std::lower_bound(b0, b1, c);
+    return (it != b1 && *it == c) ? (it - pair_col.data()) : -1;
+  };
+  for (int32_t k = 0; k < n_edges; k++) {
+    const int32_t e = edges[2 * k];
+    for (int jn = 0; jn < 9; jn++)
+      for (int in = 0; in < 9; in++) ecount[find_pair(conn[(int64_t)e * 9 + jn], conn[(int64_t)e * 9 + in]) + 1]++;
+  }
+  std::vector<int32_t> estart(npairs + 1, 0);
+  for (int64_t i = 0; i < npairs; i++) estart[i + 1] = estart[i] + ecount[i + 1];
+  std::vector<int32_t> econtrib(std::max<int32_t>(estart[npairs], 1));
+  {
+    std::vector<int32_t> fill(estart.begin(), estart.end() - 1);
+    for (int32_t k = 0; k < n_edges; k++) {
+      const int32_t e = edges[2 * k];
+      for (int jn = 0; jn < 9; jn++)
+        for (int in = 0; in < 9; in++) {
+          const int64_t pp = find_pair(conn[(int64_t)e * 9 + jn], conn[(int64_t)e * 9 + in]);
+          econtrib[fill[pp]++] = k * 81 + jn * 9 + in;  // ascending edge index
+        }
+    }
+  }
+  // CSR pattern: row (r, q) = the 6 columns of each neighbour c, then N-1; row N-1 dense.
+  h->h_rowptr.assign(h->N + 1, 0);
+  for (int32_t r = 0; r < n_nodes; r++)
+    for (int q = 0; q < 6; q++) h->h_rowptr[(int64_t)r * 6 + q + 1] = (pair_start[r + 1] - pair_start[r]) * 6 + 1;
+  h->h_rowptr[h->N] = h->N;
+  for (int64_t i = 0; i < h->N; i++) h->h_rowptr[i + 1] += h->h_rowptr[i];
+  h->nnz = h->h_rowptr[h->N];
+  h->h_col.resize(h->nnz);
+  for (int32_t r = 0; r < n_nodes; r++)
+    for (int q = 0; q < 6; q++) {
+      int64_t p = h->h_rowptr[(int64_t)r * 6 + q];
+      for (int64_t pp = pair_start[r]; pp < pair_start[r + 1]; pp++)
+        for (int iq = 0; iq < 6; iq++) h->h_col[p++] = pair_col[pp] * 6 + iq;
+      h->h_col[p++] = (int32_t)(h->N - 1);
+    }
+  {
+    int64_t p = h->h_rowptr[h->N - 1];
+    for (int64_t c = 0; c < h->N; c++) h->h_col[p++] = (int32_t)c;
+  }
+
+  // Device copies.
+#define UP(dst, src, n)                         \
+  do {                                          \
+    int rc_ = dev_upload(h, &(dst), (src), (n)); \
+    if (rc_) {                                  \
+      g_create_err = h->err;                    \
+      vef_destroy(h);                           \
+      return rc_;                               \
+    }                                           \
+  } while (0)
+  UP(h->d_coords, coords, (size_t)n_nodes * 2);
+  UP(h->d_conn, conn, (size_t)n_el * 9);
+  UP(h->d_edges, edges, (size_t)n_edges * 2);
+  UP(h->d_rowptr, h->h_rowptr.data(), (size_t)h->N + 1);
+  UP(h->d_col, h->h_col.data(), (size_t)h->nnz);
+  UP(h->d_pair_start, pair_start.data(), (size_t)n_nodes + 1);
+  UP(h->d_diag_pair, diag_pair.data(), (size_t)n_nodes);
+  UP(h->d_cstart, cstart.data(), cstart.size());
+  UP(h->d_contrib, contrib.data(), contrib.size());
+  UP(h->d_estart, estart.data(), estart.size());
+  UP(h->d_econtrib, econtrib.data(), econtrib.size());
+  UP(h->d_bank, (const double*)nullptr, (size_t)n_el * 54 * kNeval);
+  UP(h->d_ebank, (const double*)nullptr, (size_t)std::max(n_edges, 1) * 18 * kNeval);
+#undef UP
+  if (upload_tables(h)) {
+    g_create_err = h->err;
+    vef_destroy(h);
+    return VEF_ERR_HIP;
+  }
+  *out = h;
+  return VEF_OK;
+}
+
+int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "flag must be VEF_RES_ONLY or VEF_RES_JAC");
+  if (flag == VEF_RES_JAC && !values) return set_err(h, VEF_ERR_ARG, "values is required for VEF_RES_JAC");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t eu = h->N - 1;
+  const int jac = flag == VEF_RES_JAC;
+  if (jac) {
+    hipLaunchKernelGGL((vef_bulk_kernel<1, kNeval>), dim3(h->n_el), dim3(512), 0, st, h->d_coords, h->d_conn, x, eu,
+                       h->n_el, h->dprm, h->d_bank);
+  } else {
+    hipLaunchKernelGGL((vef_bulk_kernel<7, 1>), dim3((h->n_el + 6) / 7), dim3(64), 0, st, h->d_coords, h->d_conn, x,
+                       eu, h->n_el, h->dprm, h->d_bank);
+  }
+  VEF_HIP(h, hipGetLastError());
+  if (h->n_edges > 0) {
+    hipLaunchKernelGGL(vef_edge_kernel, dim3(h->n_edges), dim3(192), 0, st, h->d_coords, h->d_conn, h->d_edges, x,
+                       eu, h->n_edges, h->dprm, h->d_ebank);
+    VEF_HIP(h, hipGetLastError());
+  }
+  const int64_t node_rows = (int64_t)h->n_nodes * 6;
+  if (jac) {
+    hipLaunchKernelGGL(vef_gather_kernel, dim3((unsigned)((node_rows + 3) / 4)), dim3(256), 0, st, node_rows,
+                       h->d_rowptr, h->d_pair_start, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
+                       h->d_ebank, values);
+    VEF_HIP(h, hipGetLastError());
+    // Row N-1 (arclength row) is not touched by the bulk assembly: zero.
+    const int64_t last0 = h->h_rowptr[h->N - 1];
+    hipLaunchKernelGGL(vef_fill_kernel, dim3(1024), dim3(256), 0, st, values + last0, h->nnz - last0);
+    VEF_HIP(h, hipGetLastError());
+  }
+  hipLaunchKernelGGL(vef_rhs_kernel, dim3((unsigned)((node_rows + 255) / 256)), dim3(256), 0, st, node_rows, jac,
+                     h->d_rowptr, h->d_diag_pair, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
+                     jac ? kNeval : 1, h->d_ebank, values, b);
+  VEF_HIP(h, hipGetLastError());
+  hipLaunchKernelGGL(vef_fill_kernel, dim3(1), dim3(64), 0, st, b + (h->N - 1), (int64_t)1);
+  VEF_HIP(h, hipGetLastError());
+  h->last_flag = flag;
+  return VEF_OK;
+}
+
+int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !b || (flag == VEF_RES_JAC && !values)) return set_err(h, VEF_ERR_ARG, "missing array");
+  if (!h->d_tmp_x) {
+    VEF_HIP(h, hipMalloc((void**)&h->d_tmp_x, sizeof(double) * h->N));
+    VEF_HIP(h, hipMalloc((void**)&h->d_tmp_b, sizeof(double) * h->N));
+  }
+  if (flag == VEF_RES_JAC && !h->d_tmp_v) VEF_HIP(h, hipMalloc((void**)&h->d_tmp_v, sizeof(double) * h->nnz));
+  VEF_HIP(h, hipMemcpy(h->d_tmp_x, x, sizeof(double) * h->N, hipMemcpyHostToDevice));
+  int rc = vef_assemble(h, h->d_tmp_x, flag, h->d_tmp_b, flag == VEF_RES_JAC ? h->d_tmp_v : nullptr, nullptr);
+  if (rc) return rc;
+  VEF_HIP(h, hipDeviceSynchronize());
+  VEF_HIP(h, hipMemcpy(b, h->d_tmp_b, sizeof(double) * h->N, hipMemcpyDeviceToHost));
+  if (flag == VEF_RES_JAC) VEF_HIP(h, hipMemcpy(values, h->d_tmp_v, sizeof(double) * h->nnz, hipMemcpyDeviceToHost));
+  return VEF_OK;
+}
+
+int vef_copy_residual_bank(const vef_handle* hc, double* out) {
+  vef_handle* h = const_cast<vef_handle*>(hc);
+  if (!h || !out) return VEF_ERR_ARG;
+  if (h->last_flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "no Jacobian assembly has run");
+  VEF_HIP(h, hipDeviceSynchronize());
+  VEF_HIP(h, hipMemcpy(out, h->d_bank, sizeof(double) * (size_t)h->n_el * 54 * kNeval, hipMemcpyDeviceToHost));
+  return VEF_OK;
+}
+
+}  // extern "C"
