@@ -1,0 +1,177 @@
+#!/usr/bin/env python
+"""Assembly benchmark: residual + forward-difference Jacobian assembly of the
+100,000-element cylinder mesh on one MI355X.
+
+A step is one full assembly (vef_assemble, Flag_NR = 'NRP'): element
+residuals and their 56-evaluation forward-difference Jacobians, outlet-edge
+traction, and the deterministic scatter into the fixed CSR pattern.  Inputs
+(mesh, pattern, state) are resident in HBM before the timed region.
+
+Workload: channel x in [-15, 15], y in [-2, 2], cylinder R = 1, 100,000 9-node
+elements (400,848 nodes, N = 2,411,089 unknowns); Oldroyd-B (L-PTT with
+VeN = 0), Wi = WiN0*EU = 0.5, beta = 0.59, Re = 0, DfN = 1e-3; state = smooth
+divergence-free flow + 8-mode Fourier perturbation (amplitude 1e-2, seed
+20261203) on every field.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build runs
+on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic operations of one element-residual evaluation (adds, muls,
+# divisions, square roots; FMA-free) for this workload's configuration, from
+# the operation-counting build of the CPU restatement; pinned by
+# tests/test_flop_count.py.
+FLOPS_PER_RESIDUAL_EVAL = 24426
+EVALS_PER_ELEMENT = 56            # base + 54 nodal + 1 arclength perturbation
+QUOTIENT_FLOPS = 54 * 55 * 3      # ((dR - R)*h)/(h*h): sub, mul, div per entry
+SCATTER_FLOPS = 2916 + 2 * 54     # one add per ADD_VALUES contribution
+PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector = matrix), FMA counted as 2
+PEAK_HBM_GBS = 8000.0
+N_ELEMENTS = 100000
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--elements", type=int, default=N_ELEMENTS)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(mesh, x, target_s):
+    """The CPU restatement (oracle) on the host cores, reference order: element
+    work on OpenMP threads, the ADD_VALUES scatter serial."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    prm = oracle.params(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1e-3, WiN0=1.0)
+    rowptr, col = oracle.pattern(mesh)
+    probe = min(2000, mesh.n_elements)
+    t0 = time.perf_counter()
+    oracle.assemble(prm, mesh, x, rowptr=rowptr, col=col, el_range=(0, probe), nthreads=threads)
+    rate = probe / (time.perf_counter() - t0)
+    n = int(min(mesh.n_elements, max(probe, rate * target_s)))
+    t0 = time.perf_counter()
+    oracle.assemble(prm, mesh, x, rowptr=rowptr, col=col, el_range=(0, n), nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "elements/s", "cores": threads, "kind": "port",
+            "sample": f"elements 0..{n} of the same mesh and state ({n} of {mesh.n_elements}), residual + "
+                      f"FD Jacobian + serial ADD_VALUES scatter, {dt:.1f} s"}
+
+
+def traffic_from_profiles(n_elements):
+    path = os.path.join(ROOT, "profiles", "vef_bulk_kernel_pmc_100k.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("elements") != n_elements:
+        return None
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py: this assembler runs on one GPU; multi-GPU runs are not part of this build "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np  # noqa: F401
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd import Assembler, Params, smooth_state, workload_mesh
+
+    mesh = workload_mesh(args.elements)
+    x = smooth_state(mesh, EU=0.5)
+    params = Params(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1.0e-3, WiN0=1.0, model="L-PTT")
+    asm = Assembler(mesh, params)
+    xd = torch.from_numpy(x).to("cuda:0")
+    b = torch.empty(asm.N, dtype=torch.float64, device="cuda:0")
+    v = torch.empty(asm.nnz, dtype=torch.float64, device="cuda:0")
+    for _ in range(args.warmup):
+        asm.assemble(xd, "NRP", b=b, values=v)
+    torch.cuda.synchronize()
+
+    asm.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        asm.assemble(xd, "NRP", b=b, values=v)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    bulk_ms, scatter_ms, launches = asm.profile_read()
+    asm.profile(False)
+
+    # residual-only assemblies (Flag_NR = 'MNR'), reported beside
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        asm.assemble(xd, "MNR", b=b)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    ne = mesh.n_elements
+    bulk_ms_launch = bulk_ms / max(launches, 1)
+    bulk_flops = ne * EVALS_PER_ELEMENT * FLOPS_PER_RESIDUAL_EVAL
+    achieved = bulk_flops / (bulk_ms_launch * 1e-3) / 1e12
+    total_flops = ne * (EVALS_PER_ELEMENT * FLOPS_PER_RESIDUAL_EVAL + QUOTIENT_FLOPS + SCATTER_FLOPS)
+    out = {
+        "metric": "elements/s residual+Jacobian assembly",
+        "value": ne / (ms_step * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cylinder-in-channel Q2 mesh, {ne} elements, residual + FD Jacobian (Flag_NR=NRP), "
+                        "Oldroyd-B Wi=0.5 beta=0.59 Re=0 DfN=1e-3, fixed CSR pattern",
+            "elements": ne, "unknowns": asm.N, "nnz": asm.nnz,
+            "residual_only_ms": (t3 - t2) * 1e3 / args.steps,
+            "element_kernel_ms": bulk_ms_launch,
+            "scatter_ms": scatter_ms / max(launches, 1),
+            "total_tflops": total_flops / (ms_step * 1e-3) / 1e12,
+        },
+        "roofline": {
+            "kernel": "vef_bulk_kernel<1,56> (element residuals of all 56 FD evaluations)",
+            "bound": "mfma",
+            "ceiling": "FP64 compute (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; "
+                       "the kernel issues FP64 VALU ops without FMA contraction, so 0.5 of peak is its ceiling)",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic_from_profiles(ne),
+            "flops_per_launch": bulk_flops,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(mesh, x, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

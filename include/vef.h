@@ -113,6 +113,15 @@ int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, d
  * bank[(e*54 + j)*56 + k].  Copies n_elements*54*56 doubles to host. */
 int vef_copy_residual_bank(const vef_handle* h, double* bank_host);
 
+/* Kernel timing, the counterpart of the reference's "Bulk EQs time" and
+ * "Assembly time" prints (src/Newton_Raphson_mod.F90:337,398).  While enabled,
+ * each vef_assemble records HIP events on its stream around the element
+ * kernel (bulk) and around the scatter kernels.  vef_profile_read waits for
+ * the recorded events, returns the summed milliseconds and the number of
+ * assemblies since the last read, and clears them.  (Re-)enabling clears too. */
+int vef_profile(vef_handle* h, int32_t enable);
+int vef_profile_read(vef_handle* h, double* bulk_ms, double* scatter_ms, int64_t* launches);
+
 /* Last error message of the handle (or of the last failed vef_create when h
  * is NULL). */
 const char* vef_last_error(const vef_handle* h);

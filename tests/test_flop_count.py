@@ -1,0 +1,42 @@
+"""The algorithmic operation count bench.py uses for the roofline: counted by
+the operation-counting build of the oracle on one element residual of the
+workload configuration (Oldroyd-B, Re = 0, DfN = 1e-3), independent of the
+state values (no data-dependent branches in this configuration)."""
+import importlib.util
+import os
+
+import numpy as np
+
+from test_oracle_element import curved_xy, random_state
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def bench_constant():
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.FLOPS_PER_RESIDUAL_EVAL
+
+
+def test_bench_flop_constant_matches_oracle_count(oracle):
+    rng = np.random.default_rng(0)
+    prm = oracle.params(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1e-3, WiN0=1.0, model="L-PTT")
+    counts = set()
+    for _ in range(3):
+        xy = curved_xy(rng)
+        counts.add(oracle.count_flops(prm, xy, random_state(rng, xy), 0.5)["total"])
+    assert counts == {bench_constant()}
+
+
+def test_counter_sees_model_terms(oracle):
+    """The counter is not blind: Giesekus adds the S^3 and C:C work, inertia the
+    convective terms."""
+    rng = np.random.default_rng(1)
+    xy = curved_xy(rng)
+    TL = random_state(rng, xy)
+    base = oracle.count_flops(oracle.params(), xy, TL, 0.5)["total"]
+    gies = oracle.count_flops(oracle.params(VeN=0.1, model="GIESEKUS"), xy, TL, 0.5)["total"]
+    inert = oracle.count_flops(oracle.params(ReN=1.0), xy, TL, 0.5)["total"]
+    dcs = oracle.count_flops(oracle.params(DfN=0.0), xy, TL, 0.5)["total"]
+    assert gies > base and inert > base and dcs > base

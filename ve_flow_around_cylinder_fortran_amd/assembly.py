@@ -104,6 +104,10 @@ def load_library():
     lib.vef_assemble_host.restype = ctypes.c_int
     lib.vef_copy_residual_bank.argtypes = [c_vp, c_vp]
     lib.vef_copy_residual_bank.restype = ctypes.c_int
+    lib.vef_profile.argtypes = [c_vp, c_i32]
+    lib.vef_profile.restype = ctypes.c_int
+    lib.vef_profile_read.argtypes = [c_vp, c_vp, c_vp, c_vp]
+    lib.vef_profile_read.restype = ctypes.c_int
     lib.vef_last_error.argtypes = [c_vp]
     lib.vef_last_error.restype = ctypes.c_char_p
     lib.vef_destroy.argtypes = [c_vp]
@@ -210,6 +214,16 @@ class Assembler:
         rc = self.lib.vef_assemble_host(self._h, _ptr(x), flag, _ptr(b), _ptr(values) if values is not None else None)
         self._check(rc, "vef_assemble_host")
         return b, values
+
+    def profile(self, enable: bool = True):
+        self._check(self.lib.vef_profile(self._h, 1 if enable else 0), "vef_profile")
+
+    def profile_read(self):
+        """(bulk kernel ms, scatter ms, assemblies) summed since the last read."""
+        tb, ts, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self._check(self.lib.vef_profile_read(self._h, ctypes.byref(tb), ctypes.byref(ts), ctypes.byref(n)),
+                    "vef_profile_read")
+        return tb.value, ts.value, n.value
 
     def residual_bank(self) -> np.ndarray:
         """R_k(e, row) of the last Jacobian assembly, shape (n_el, 54, 56)."""

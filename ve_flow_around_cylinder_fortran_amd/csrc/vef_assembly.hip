@@ -3,15 +3,15 @@
 // behind the C ABI of include/vef.h.
 //
 // What runs where
-//   vef_bulk_kernel<1,56>  one workgroup per element, 504 lanes = 9 Gauss
-//                          points x 56 evaluations (base state, the 54 nodal
-//                          perturbations, the EU perturbation).  Each lane
-//                          evaluates the Gauss-point strong form and the 9
-//                          test-node residual rows at its state; an ordered
-//                          LDS reduction over the Gauss points gives the 54
-//                          element residual rows of every evaluation, written
-//                          as the per-element bank R[e][row][k].
-//   vef_bulk_kernel<7,1>   residual only: one wavefront per 7 elements.
+//   vef_bulk_kernel<1,7,56> eight single-wavefront workgroups per element, each
+//                          63 lanes = 9 Gauss points x 7 of the 56 evaluations
+//                          (base state, the 54 nodal perturbations, the EU
+//                          perturbation).  Each lane evaluates the Gauss-point
+//                          strong form and the 9 test-node residual rows at its
+//                          state; an ordered LDS reduction over the Gauss
+//                          points gives the 54 element residual rows of every
+//                          evaluation, written as the bank R[e][row][k].
+//   vef_bulk_kernel<7,1,1> residual only: one wavefront per 7 elements.
 //   vef_edge_kernel        outlet-edge traction (openBC) bank, 3 edge Gauss
 //                          points x 56 evaluations per outlet edge.
 //   vef_gather_kernel      CSR values: each matrix slot sums the
@@ -156,10 +156,35 @@ struct DevParams {
 struct Geo {
   double dFx[9], dFy[9], d2x[9], d2y[9];
   double WET, H2;
+  double H4, yH4;     // H2*H2 and RN(1/(H2*H2)) (denominator of the complex divisions by H2)
+  double r3, t7;      // BvN/H2 and DfN/H2
 };
 
 // Complex division with zero imaginary parts: re((a+0i)/(c+0i)) = (a*c)/(c*c).
 __device__ __forceinline__ double cdiv(double a, double c) { return (a * c) / (c * c); }
+
+// Division by a denominator shared by many numerators.  With y = RN(1/d) and
+// q0 = RN(a*y), the residual r = a - d*q0 is exact in one FMA and
+// q1 = RN(q0 + r*y) is the correctly rounded a/d (Markstein's theorem for
+// round-to-nearest), so q1 equals the IEEE division bit for bit as long as no
+// intermediate leaves the normal range; outside it the IEEE division is used.
+struct Recip {
+  double d, y;
+  bool fast;
+};
+__device__ __forceinline__ Recip make_recip(double d) {
+  const double ad = fabs(d);
+  return Recip{d, 1.0 / d, ad >= 0x1p-500 && ad <= 0x1p500};
+}
+__device__ __forceinline__ double qdiv(double a, const Recip& r) {
+  const double q0 = a * r.y;
+  const double aq = fabs(q0);
+  if (__builtin_expect(r.fast && aq >= 0x1p-900 && aq <= 0x1p900, 1)) {
+    const double e = fma(-r.d, q0, a);
+    return fma(e, r.y, q0);
+  }
+  return a / r.d;
+}
 
 // basis2D for one Gauss point from the nine node coordinates; fills Geo.
 __device__ void geometry(const double* X /*[9][2]*/, const double* dc, const double* de, const double* dcc,
@@ -191,6 +216,8 @@ __device__ void geometry(const double* X /*[9][2]*/, const double* dc, const dou
   const double Helem = 1.0 / sqrt(H);
   G.H2 = Helem * Helem;
   G.WET = wgt * fabs(detJ);
+  G.H4 = G.H2 * G.H2;
+  G.yH4 = 1.0 / G.H4;
 }
 
 // Gauss-point quantities that the test-node terms read (GaussPointQuantities,
@@ -201,7 +228,9 @@ struct GP {
   double RC0, RC1;  // ReN*Convection(i)
   double P00, P01, P10, P11;  // P_tot
   double CE00, CE01, CE10, CE11;  // Con_EQ
-  double S00, S01, S11, dW;  // S and S11+S22 (omega denominator)
+  double S00, S01, S11;
+  Recip rW;    // S11+S22, the omega denominator
+  Recip rW2;   // WiN*WiN
   double dS00x, dS00y, dS01x, dS01y, dS11x, dS11y;
   double WiN, f_T;
   double tlsme, tlsic, tlsce, tdcs, mag_re;
@@ -210,7 +239,12 @@ struct GP {
 // setFlowUnknowns .. setStabilizationTerms for one state.  v(n, q) reads the
 // lane's nodal value of unknown q at node n; bf are the basis values at the
 // Gauss point; H2 = Helem**2.
-template <class V>
+// MODEL: 0 L-PTT, 1 Giesekus.  DCS: discontinuity capturing on (DfN <= 1e-12).
+// INERTIA: ReN /= 0.  With ReN = 0 the convective and (ReN*tsugn)**2 terms are
+// signed zeros whose additions leave every value unchanged, so they are not
+// computed; likewise S^3 and C:C (multiplied by f(3) = 0 for L-PTT) and the DCS
+// terms (tdcs = 0 when DfN > 1e-12).
+template <int MODEL, bool DCS, bool INERTIA, class V>
 __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const Geo& G, double EU,
                                             const DevParams& prm, GP& o) {
   // Interpolation, each sum sequential over the nodes (dot_product).
@@ -226,7 +260,7 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
     S01 = u4 * b; dS01x = u4 * fx; dS01y = u4 * fy;
     S11 = u5 * b; dS11x = u5 * fx; dS11y = u5 * fy;
   }
-#pragma unroll
+#pragma unroll 2
   for (int n = 1; n < 9; n++) {
     const double b = bf[n], fx = G.dFx[n], fy = G.dFy[n], gx = G.d2x[n], gy = G.d2y[n];
     const double u0 = v(n, 0), u1 = v(n, 1), u2 = v(n, 2), u3 = v(n, 3), u4 = v(n, 4), u5 = v(n, 5);
@@ -262,7 +296,7 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
 
   // conEQ_L_PTT_SQRT / conEQ_Giesekus_SQRT: f(1..3), f_T (g = 1, dg/dxi = 0)
   double f0, f1, f2, f_T;
-  if (prm.model == 0) {
+  if constexpr (MODEL == 0) {
     const double trC = C00 + C11;
     f_T = 1.0 + prm.VeN * (trC - 2.0);
     f0 = f_T;
@@ -285,7 +319,8 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   o.P11 = T11 + prm.BvN * Gd11 - P;
 
   // setUpperConvectedDerivativeSQRT: UCD = U.grad S - S.GU_tr - W.S
-  const double W12 = (-(S00 * Gt01) + S01 * (Gt00 - Gt11) + S11 * Gt10) / (S00 + S11);
+  const Recip rW = make_recip(S00 + S11);
+  const double W12 = qdiv(-(S00 * Gt01) + S01 * (Gt00 - Gt11) + S11 * Gt10, rW);
   const double W21 = -W12;
   double U00 = Ux * dS00x + Uy * dS00y, U01 = Ux * dS01x + Uy * dS01y;
   double U10 = U01, U11 = Ux * dS11x + Uy * dS11y;
@@ -295,12 +330,12 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   U11 = U11 - (S10 * Gt01 + S11 * Gt11) - W21 * S01;
 
   // setSInverse (inverseTensor2D)
-  const double detS = S00 * S11 - S01 * S10;
-  const double SI00 = S11 / detS, SI01 = (-S01) / detS, SI10 = (-S10) / detS, SI11 = S00 / detS;
+  const Recip rS = make_recip(S00 * S11 - S01 * S10);
+  const double SI00 = qdiv(S11, rS), SI01 = qdiv(-S01, rS), SI10 = qdiv(-S10, rS), SI11 = qdiv(S00, rS);
 
   // term_S = f3 S^3 + f2 S - f1 S^-1 ; Con_EQ = WiN*UCD + term_S/2
   double tS00, tS01, tS10, tS11;
-  if (prm.model == 0) {
+  if constexpr (MODEL == 0) {
     tS00 = f1 * S00 - f0 * SI00;
     tS01 = f1 * S01 - f0 * SI01;
     tS10 = f1 * S10 - f0 * SI10;
@@ -321,12 +356,19 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   const double divT1 = a * dC01x + a * dC11y;
   const double dP0 = -dPx + divT0 + prm.BvN * (G2x0 + G2y0);
   const double dP1 = -dPy + divT1 + prm.BvN * (G2x1 + G2y1);
-  const double Cv0 = Ux * GU00 + Uy * GU10;
-  const double Cv1 = Ux * GU01 + Uy * GU11;
-  o.RC0 = prm.ReN * Cv0;
-  o.RC1 = prm.ReN * Cv1;
-  o.Mom0 = o.RC0 - dP0;
-  o.Mom1 = o.RC1 - dP1;
+  if constexpr (INERTIA) {
+    const double Cv0 = Ux * GU00 + Uy * GU10;
+    const double Cv1 = Ux * GU01 + Uy * GU11;
+    o.RC0 = prm.ReN * Cv0;
+    o.RC1 = prm.ReN * Cv1;
+    o.Mom0 = o.RC0 - dP0;
+    o.Mom1 = o.RC1 - dP1;
+  } else {
+    o.RC0 = 0.0;
+    o.RC1 = 0.0;
+    o.Mom0 = -dP0;
+    o.Mom1 = -dP1;
+  }
   o.Cont = GU00 + GU11;  // strongFormContinuity
 
   // setStabilizationTerms
@@ -338,12 +380,20 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   const double ddG = Gd00 * Gd00 + Gd01 * Gd10 + Gd10 * Gd01 + Gd11 * Gd11;
   double Ha = sqrt(a * a + 0.5 * ddT);
   Ha = cdiv(Ha, sqrt(1.0 + 0.5 * ddG));
-  const double r1 = prm.ReN / prm.dt, r2 = prm.ReN * tsugn, r3 = prm.BvN / H2, r4 = cdiv(Ha, H2);
-  const double tlsme = cdiv(1.0, sqrt(r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4));
+  // (ReN/dt)**2 with dt = +huge is exactly zero and adds nothing.
+  const double r3 = G.r3;
+  const double r4 = qdiv(Ha * H2, Recip{G.H4, G.yH4, true});
+  double tlsme;
+  if constexpr (INERTIA) {
+    const double r2 = prm.ReN * tsugn;
+    tlsme = cdiv(1.0, sqrt(r2 * r2 + r3 * r3 + r4 * r4));
+  } else {
+    tlsme = cdiv(1.0, sqrt(r3 * r3 + r4 * r4));
+  }
   o.tlsme = tlsme;
   o.tlsic = cdiv(H2, tlsme);
   double tdcs = 0.0, mag_re = 0.0, mag_im = 0.0;
-  if (prm.dcs_on) {
+  if constexpr (DCS) {
     const double ddS = S00 * S00 + S01 * S10 + S10 * S01 + S11 * S11;
     tdcs = cdiv(H2, sqrt(0.5 * ddS));
     const double mc = 0.5 * (CE00 * CE00 + CE01 * CE10 + CE10 * CE01 + CE11 * CE11);
@@ -352,24 +402,27 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   }
   o.tdcs = tdcs;
   o.mag_re = mag_re;
-  const double t1 = WiN / prm.dt, t2 = WiN * tsugn, t3 = WiN * sqrt(0.5 * ddG), t4 = f1;
-  double sum = t1 * t1 + t2 * t2 + t3 * t3 + t4 * t4;
-  if (prm.model != 0) {
+  // (WiN/dt)**2 with dt = +huge is exactly zero and adds nothing.
+  const double t2 = WiN * tsugn, t3 = WiN * sqrt(0.5 * ddG), t4 = f1;
+  double sum = t2 * t2 + t3 * t3 + t4 * t4;
+  if constexpr (MODEL != 0) {
     const double ddC = C00 * C00 + C01 * C10 + C10 * C01 + C11 * C11;
     const double t5 = f2 * sqrt(0.5 * ddC);
     sum = sum + t5 * t5;
   }
-  if (prm.dcs_on) {
+  if constexpr (DCS) {
     const double zr = cdiv(tdcs * mag_re, H2), zi = cdiv(tdcs * mag_im, H2);
     sum = sum + (zr * zr - zi * zi);
   }
-  const double t7 = prm.DfN / H2;
+  const double t7 = G.t7;
   sum = sum + t7 * t7;
   o.tlsce = cdiv(1.0, sqrt(sum));
 
   o.Ux = Ux; o.Uy = Uy;
   o.CE00 = CE00; o.CE01 = CE01; o.CE10 = CE10; o.CE11 = CE11;
-  o.S00 = S00; o.S01 = S01; o.S11 = S11; o.dW = S00 + S11;
+  o.S00 = S00; o.S01 = S01; o.S11 = S11;
+  o.rW = rW;
+  o.rW2 = make_recip(WiN * WiN);
   o.dS00x = dS00x; o.dS00y = dS00y; o.dS01x = dS01x; o.dS01y = dS01y; o.dS11x = dS11x; o.dS11y = dS11y;
   o.WiN = WiN; o.f_T = f_T;
 }
@@ -377,42 +430,42 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
 // Test-node rows of one Gauss point (momentumMainSQRT, continuityBulkMain,
 // stressesBulkMainSQRT and their LSME/LSIC/LSCE/DCS terms, Residuals_mod.F90:
 // 90-110,161-234,295-490), already multiplied by WET.
+template <int MODEL, bool DCS, bool INERTIA>
 __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, double fy, const DevParams& prm,
                                            double WET, double* out6) {
   const double S10 = g.S01;
   const double Udf = g.Ux * fx + g.Uy * fy;
-  const double lsme_m = g.tlsme * prm.ReN * Udf;
-  const double wscale = g.WiN * g.WiN;
+  const double lsme_m = INERTIA ? g.tlsme * prm.ReN * Udf : 0.0;
   double T0, T1;
   {
     // i = 1: GUT_lsce row 1 = grad(phi)
-    const double lsme = lsme_m * g.Mom0;
     const double lsic = (g.tlsic * fx) * g.Cont;
-    const double w = (g.S01 * fx + g.S11 * fy) / g.dW;  // omegaNominator2D / omegaDenominator2D
+    const double w = qdiv(g.S01 * fx + g.S11 * fy, g.rW);  // omegaNominator2D / omegaDenominator2D
     const double m00 = g.S00 * fx + g.S01 * fy, m10 = S10 * fx + g.S11 * fy;
     const double u00 = -m00 - w * S10, u01 = -(w * g.S11);
     const double u10 = -m10 - (-w) * g.S00, u11 = -((-w) * g.S01);
-    const double l00 = ((g.tlsce * u00) * prm.omb * g.WiN) / wscale;
-    const double l01 = ((g.tlsce * u01) * prm.omb * g.WiN) / wscale;
-    const double l10 = ((g.tlsce * u10) * prm.omb * g.WiN) / wscale;
-    const double l11 = ((g.tlsce * u11) * prm.omb * g.WiN) / wscale;
+    const double l00 = qdiv((g.tlsce * u00) * prm.omb * g.WiN, g.rW2);
+    const double l01 = qdiv((g.tlsce * u01) * prm.omb * g.WiN, g.rW2);
+    const double l10 = qdiv((g.tlsce * u10) * prm.omb * g.WiN, g.rW2);
+    const double l11 = qdiv((g.tlsce * u11) * prm.omb * g.WiN, g.rW2);
     const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
-    T0 = g.RC0 * bifn + (g.P00 * fx + g.P01 * fy) + lsme + lsic + lsce;
+    if constexpr (INERTIA) T0 = g.RC0 * bifn + (g.P00 * fx + g.P01 * fy) + lsme_m * g.Mom0 + lsic + lsce;
+    else T0 = (g.P00 * fx + g.P01 * fy) + lsic + lsce;
   }
   {
     // i = 2: GUT_lsce row 2 = grad(phi)
-    const double lsme = lsme_m * g.Mom1;
     const double lsic = (g.tlsic * fy) * g.Cont;
-    const double w = (-(g.S00 * fx) + g.S01 * (-fy)) / g.dW;
+    const double w = qdiv(-(g.S00 * fx) + g.S01 * (-fy), g.rW);
     const double m01 = g.S00 * fx + g.S01 * fy, m11 = S10 * fx + g.S11 * fy;
     const double u00 = -(w * S10), u01 = -m01 - w * g.S11;
     const double u10 = -((-w) * g.S00), u11 = -m11 - (-w) * g.S01;
-    const double l00 = ((g.tlsce * u00) * prm.omb * g.WiN) / wscale;
-    const double l01 = ((g.tlsce * u01) * prm.omb * g.WiN) / wscale;
-    const double l10 = ((g.tlsce * u10) * prm.omb * g.WiN) / wscale;
-    const double l11 = ((g.tlsce * u11) * prm.omb * g.WiN) / wscale;
+    const double l00 = qdiv((g.tlsce * u00) * prm.omb * g.WiN, g.rW2);
+    const double l01 = qdiv((g.tlsce * u01) * prm.omb * g.WiN, g.rW2);
+    const double l10 = qdiv((g.tlsce * u10) * prm.omb * g.WiN, g.rW2);
+    const double l11 = qdiv((g.tlsce * u11) * prm.omb * g.WiN, g.rW2);
     const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
-    T1 = g.RC1 * bifn + (g.P10 * fx + g.P11 * fy) + lsme + lsic + lsce;
+    if constexpr (INERTIA) T1 = g.RC1 * bifn + (g.P10 * fx + g.P11 * fy) + lsme_m * g.Mom1 + lsic + lsce;
+    else T1 = (g.P10 * fx + g.P11 * fy) + lsic + lsce;
   }
   // continuity row
   const double T2 = g.Cont * bifn + g.tlsme * (fx * g.Mom0 + fy * g.Mom1);
@@ -426,7 +479,7 @@ __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, 
   double T3 = g.CE00 * bifn + prm.DfN * d00 + (g.tlsme * (-(fx * g.Mom0)) * g.WiN) + lsce_s * g.CE00;
   double T4 = g.CE01 * bifn + prm.DfN * d01 + (g.tlsme * (-(fy * g.Mom0) - fx * g.Mom1) * g.WiN) + lsce_s * g.CE01;
   double T5 = g.CE11 * bifn + prm.DfN * d11 + (g.tlsme * (-(fy * g.Mom1)) * g.WiN) + lsce_s * g.CE11;
-  if (prm.dcs_on) {
+  if constexpr (DCS) {
     T3 = T3 + (g.tdcs * d00) * g.mag_re;
     T4 = T4 + (g.tdcs * d01) * g.mag_re;
     T5 = T5 + (g.tdcs * d11) * g.mag_re;
@@ -457,16 +510,20 @@ struct PlainState {
 };
 
 // ----------------------------------------------------------------------------
-// Bulk kernel.  EPB elements per workgroup, NEVAL evaluations per element.
-// Lane t < 9*NSLOT: Gauss point g = t / NSLOT, slot s = t % NSLOT,
-// element el = s / NEVAL, evaluation k = s % NEVAL.
+// Bulk kernel: one wavefront per workgroup.  A workgroup holds EPB elements x
+// KPB evaluations = 7 slots; lane t < 63 is Gauss point g = t / 7 of slot
+// s = t % 7 (element el = s / KPB, evaluation k = kg*KPB + s % KPB, where kg
+// is the workgroup's evaluation group of its element).  NEVAL = 56 assembles
+// residual and FD Jacobian (8 workgroups per element), NEVAL = 1 the residual
+// only (7 elements per workgroup).
 // ----------------------------------------------------------------------------
-template <int EPB, int NEVAL>
-__global__ void __launch_bounds__(((9 * EPB * NEVAL + 63) / 64) * 64)
+template <int EPB, int KPB, int NEVAL, int MODEL, bool DCS, bool INERTIA>
+__global__ void __launch_bounds__(64)
 vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
                 int64_t eu_index, int32_t n_el, DevParams prm, double* __restrict__ bank) {
-  constexpr int NSLOT = EPB * NEVAL;
-  constexpr int NLANE = 9 * NSLOT;
+  constexpr int NSLOT = EPB * KPB;
+  static_assert(NSLOT == 7, "seven slots per wavefront");
+  constexpr int BPE = NEVAL / KPB;  // workgroups per element group
   constexpr int NSTATE = (NEVAL > 1) ? 3 : 1;
   __shared__ double sU[EPB][NSTATE * 54];
   __shared__ double sX[EPB][18];
@@ -474,11 +531,12 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   __shared__ double sRed[2][9][NSLOT][6];
 
   const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int64_t e0 = (int64_t)(blockIdx.x / BPE) * EPB;
+  const int kg = blockIdx.x % BPE;
   const double EU = x[eu_index];
 
   // Gather element unknowns and coordinates (copyToLocal).
-  for (int t = tid; t < EPB * 54; t += blockDim.x) {
+  for (int t = tid; t < EPB * 54; t += 64) {
     const int el = t / 54, j = t % 54;
     const int64_t e = e0 + el;
     double u = 0.0;
@@ -493,7 +551,7 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
       sU[el][54 + j] = up - kEpsJac;
     }
   }
-  for (int t = tid; t < EPB * 18; t += blockDim.x) {
+  for (int t = tid; t < EPB * 18; t += 64) {
     const int el = t / 18, c = t % 18;
     const int64_t e = e0 + el;
     double v = (c & 1) ? 1.0 : 0.0;
@@ -501,17 +559,20 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     sX[el][c] = v;
   }
   __syncthreads();
-  for (int t = tid; t < EPB * 9; t += blockDim.x) {
+  for (int t = tid; t < EPB * 9; t += 64) {
     const int el = t / 9, g = t % 9;
-    geometry(sX[el], c_tab.dc[g], c_tab.de[g], c_tab.dcc[g], c_tab.dee[g], c_tab.dce[g], c_tab.W2[g], sG[el][g]);
+    Geo& G = sG[el][g];
+    geometry(sX[el], c_tab.dc[g], c_tab.de[g], c_tab.dcc[g], c_tab.dee[g], c_tab.dce[g], c_tab.W2[g], G);
+    G.r3 = prm.BvN / G.H2;
+    G.t7 = prm.DfN / G.H2;
   }
   __syncthreads();
 
-  const bool active = tid < NLANE;
+  const bool active = tid < 9 * NSLOT;
   const int g = active ? tid / NSLOT : 0;
   const int s = tid % NSLOT;
-  const int el = s / NEVAL;
-  const int k = s % NEVAL;
+  const int el = s / KPB;
+  const int k = kg * KPB + s % KPB;
   const bool valid = active && (e0 + el < n_el);
 
   GP q;
@@ -519,10 +580,10 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     const double EUk = (NEVAL > 1 && k == NEVAL - 1) ? EU + kEpsJac : EU;
     if (NEVAL > 1) {
       LaneState st{sU[el], k - 1};
-      gauss_point(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
+      gauss_point<MODEL, DCS, INERTIA>(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
     } else {
       PlainState st{sU[el]};
-      gauss_point(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
+      gauss_point<MODEL, DCS, INERTIA>(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
     }
   }
   const double WET = sG[el][g].WET;
@@ -531,15 +592,15 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     const int buf = n & 1;
     if (valid) {
       double r[6];
-      node_terms(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET, r);
+      node_terms<MODEL, DCS, INERTIA>(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET, r);
 #pragma unroll
       for (int c = 0; c < 6; c++) sRed[buf][g][s][c] = r[c];
     }
     __syncthreads();
     // Ordered sum over the Gauss points: Res = Res + TERM_RES*WET, ig = 1..9.
-    for (int t = tid; t < 6 * NSLOT; t += blockDim.x) {
-      const int c = t / NSLOT, ss = t % NSLOT;
-      const int eel = ss / NEVAL, kk = ss % NEVAL;
+    if (tid < 6 * NSLOT) {
+      const int c = tid / NSLOT, ss = tid % NSLOT;
+      const int eel = ss / KPB, kk = kg * KPB + ss % KPB;
       if (e0 + eel < n_el) {
         double acc = sRed[buf][0][ss][c];
 #pragma unroll
@@ -682,7 +743,8 @@ vef_edge_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ double fd_quotient(double rk, double r0) {
   // (dRes - Res)/eps_jac with a complex numerator: ((dR-R)*h)/(h*h)
-  return ((rk - r0) * kEpsJac) / (kEpsJac * kEpsJac);
+  constexpr double hh = kEpsJac * kEpsJac;
+  return qdiv((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh, true});
 }
 
 __global__ void __launch_bounds__(256)
@@ -805,6 +867,10 @@ struct vef_handle {
   double* d_tmp_v = nullptr;
   int last_flag = -1;
   std::string err;
+  // Optional kernel timing: event pairs around the element kernel and around
+  // the scatter kernels of each assembly.
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_bulk, ev_scatter;  // start/stop pairs
 };
 
 namespace {
@@ -864,6 +930,35 @@ int upload_tables(vef_handle* h) {
 }
 }  // namespace
 
+namespace {
+template <int MODEL, bool DCS, bool INERTIA>
+void launch_bulk_t(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
+  if (jac)
+    hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>), dim3((unsigned)h->n_el * (kNeval / 7)),
+                       dim3(64), 0, st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_bank);
+  else
+    hipLaunchKernelGGL((vef_bulk_kernel<7, 1, 1, MODEL, DCS, INERTIA>), dim3((unsigned)(h->n_el + 6) / 7), dim3(64), 0,
+                       st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_bank);
+}
+
+template <int MODEL, bool DCS>
+void launch_bulk_i(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
+  if (h->dprm.ReN != 0.0) launch_bulk_t<MODEL, DCS, true>(h, jac, x, eu, st);
+  else launch_bulk_t<MODEL, DCS, false>(h, jac, x, eu, st);
+}
+
+template <int MODEL>
+void launch_bulk_d(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
+  if (h->dprm.dcs_on) launch_bulk_i<MODEL, true>(h, jac, x, eu, st);
+  else launch_bulk_i<MODEL, false>(h, jac, x, eu, st);
+}
+
+void launch_bulk(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
+  if (h->dprm.model == 0) launch_bulk_d<0>(h, jac, x, eu, st);
+  else launch_bulk_d<1>(h, jac, x, eu, st);
+}
+}  // namespace
+
 extern "C" {
 
 const char* vef_last_error(const vef_handle* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
@@ -882,6 +977,8 @@ int vef_get_pattern(const vef_handle* h, int64_t* rowptr, int32_t* colind) {
 
 void vef_destroy(vef_handle* h) {
   if (!h) return;
+  for (hipEvent_t e : h->ev_bulk) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->ev_scatter) (void)hipEventDestroy(e);
   void* ptrs[] = {h->d_coords, h->d_conn, h->d_edges, h->d_rowptr, h->d_col, h->d_pair_start, h->d_diag_pair,
                   h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, h->d_tmp_x,
                   h->d_tmp_b, h->d_tmp_v};
@@ -1061,20 +1158,21 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
   hipStream_t st = (hipStream_t)stream;
   const int64_t eu = h->N - 1;
   const int jac = flag == VEF_RES_JAC;
-  if (jac) {
-    hipLaunchKernelGGL((vef_bulk_kernel<1, kNeval>), dim3(h->n_el), dim3(512), 0, st, h->d_coords, h->d_conn, x, eu,
-                       h->n_el, h->dprm, h->d_bank);
-  } else {
-    hipLaunchKernelGGL((vef_bulk_kernel<7, 1>), dim3((h->n_el + 6) / 7), dim3(64), 0, st, h->d_coords, h->d_conn, x,
-                       eu, h->n_el, h->dprm, h->d_bank);
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (h->profiling) {
+    for (int i = 0; i < 4; i++) VEF_HIP(h, hipEventCreate(&ev[i]));
+    VEF_HIP(h, hipEventRecord(ev[0], st));
   }
+  launch_bulk(h, jac, x, eu, st);
   VEF_HIP(h, hipGetLastError());
+  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[1], st));
   if (h->n_edges > 0) {
     hipLaunchKernelGGL(vef_edge_kernel, dim3(h->n_edges), dim3(192), 0, st, h->d_coords, h->d_conn, h->d_edges, x,
                        eu, h->n_edges, h->dprm, h->d_ebank);
     VEF_HIP(h, hipGetLastError());
   }
   const int64_t node_rows = (int64_t)h->n_nodes * 6;
+  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[2], st));
   if (jac) {
     hipLaunchKernelGGL(vef_gather_kernel, dim3((unsigned)((node_rows + 3) / 4)), dim3(256), 0, st, node_rows,
                        h->d_rowptr, h->d_pair_start, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
@@ -1091,7 +1189,52 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
   VEF_HIP(h, hipGetLastError());
   hipLaunchKernelGGL(vef_fill_kernel, dim3(1), dim3(64), 0, st, b + (h->N - 1), (int64_t)1);
   VEF_HIP(h, hipGetLastError());
+  if (h->profiling) {
+    VEF_HIP(h, hipEventRecord(ev[3], st));
+    h->ev_bulk.push_back(ev[0]);
+    h->ev_bulk.push_back(ev[1]);
+    h->ev_scatter.push_back(ev[2]);
+    h->ev_scatter.push_back(ev[3]);
+  }
   h->last_flag = flag;
+  return VEF_OK;
+}
+
+namespace {
+void release_events(vef_handle* h) {
+  for (hipEvent_t e : h->ev_bulk) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->ev_scatter) (void)hipEventDestroy(e);
+  h->ev_bulk.clear();
+  h->ev_scatter.clear();
+}
+}  // namespace
+
+int vef_profile(vef_handle* h, int32_t enable) {
+  if (!h) return VEF_ERR_ARG;
+  release_events(h);
+  h->profiling = enable != 0;
+  return VEF_OK;
+}
+
+int vef_profile_read(vef_handle* h, double* bulk_ms, double* scatter_ms, int64_t* launches) {
+  if (!h || !bulk_ms || !scatter_ms || !launches) return VEF_ERR_ARG;
+  double tb = 0.0, ts = 0.0;
+  for (size_t i = 0; i + 1 < h->ev_bulk.size(); i += 2) {
+    float ms = 0.0f;
+    VEF_HIP(h, hipEventSynchronize(h->ev_bulk[i + 1]));
+    VEF_HIP(h, hipEventElapsedTime(&ms, h->ev_bulk[i], h->ev_bulk[i + 1]));
+    tb += ms;
+  }
+  for (size_t i = 0; i + 1 < h->ev_scatter.size(); i += 2) {
+    float ms = 0.0f;
+    VEF_HIP(h, hipEventSynchronize(h->ev_scatter[i + 1]));
+    VEF_HIP(h, hipEventElapsedTime(&ms, h->ev_scatter[i], h->ev_scatter[i + 1]));
+    ts += ms;
+  }
+  *bulk_ms = tb;
+  *scatter_ms = ts;
+  *launches = (int64_t)(h->ev_bulk.size() / 2);
+  release_events(h);
   return VEF_OK;
 }
 
