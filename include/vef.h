@@ -104,6 +104,21 @@ const int32_t* vef_device_colind(const vef_handle* h);
  * Cylinder_2D outlet (src/Newton_Raphson_mod.F90:101,332-357). */
 int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* hip_stream);
 
+/* The arclength row N-1, applied after vef_assemble (arclength and
+ * pseudoArclength, src/Extra_Equations_mod.F90:698-828).
+ *   VEF_ARC_PIN (continuation steps 1-2): b[N-1] = x[N-1] - (LN0 + (increment-1)*dArN);
+ *     with VEF_RES_JAC, row and column N-1 are zeroed and the diagonal set to 1
+ *     (MatZeroRowsColumns).  x_old and tangent may be NULL.
+ *   VEF_ARC_PSEUDO (steps >= 3): b[N-1] = tangent . (x - x_old) - dS, summed per
+ *     equation over the nodes then the arclength entry, as the reference does;
+ *     with VEF_RES_JAC, row N-1 = tangent.  The sum uses a fixed blocked order
+ *     (deterministic; within 1e-13 of the sequential sum relative to the sum of
+ *     |terms|).  tangent[N] holds dTLodS node-major and dEUodS last. */
+enum { VEF_ARC_PIN = 0, VEF_ARC_PSEUDO = 1 };
+int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
+                  int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
+                  void* hip_stream);
+
 /* The same with host arrays (copies in, assembles, copies out). */
 int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values);
 

@@ -51,6 +51,8 @@ def lib(flops: bool = False):
                                                ctypes.c_double, vp]
             L.vo_element.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                      vp, vp, vp]
+            L.vo_arclength.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp]
             L.vo_pattern_nnz.argtypes = [ctypes.c_int, ctypes.c_int, vp]
             L.vo_pattern_nnz.restype = ctypes.c_int64
             L.vo_pattern.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp]
@@ -162,3 +164,10 @@ def assemble(prm, mesh, x, jac=True, rowptr=None, col=None, el_range=None, nthre
     if rc != 0:
         raise RuntimeError("oracle assembly failed: pattern mismatch")
     return b, vals
+
+
+def arclength(mode, n_nodes, x, xo, t, increment, LN0, dArN, dSo, rowptr, col, vals, b, jac=True):
+    """Applies the arclength row to (vals, b) in place (reference order)."""
+    x, xo, t = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, xo, t))
+    lib().vo_arclength(mode, n_nodes, _p(x), _p(xo), _p(t), increment, LN0, dArN, dSo, 1 if jac else 0, _p(rowptr),
+                       _p(col), _p(vals), _p(b))

@@ -1091,6 +1091,39 @@ int vo_assemble(const vo_params* p, int n_nodes, int n_el, const double* coords,
   return 0;
 }
 
+// a14: the arclength row after the bulk assembly (arclength / pseudoArclength,
+// src/Extra_Equations_mod.F90:698-828).  mode 0 (Increment <= 2): b(N-1) =
+// EU - (LN0 + (Increment-1)*dArN) and, with the Jacobian, MatZeroRowsColumns
+// of row/column N-1 with diagonal 1.  mode 1: b(N-1) = sum_ieq
+// dot(dTLodS(:,ieq), TL(:,ieq)-TLo(:,ieq)) + dot(dEUodS, EU-EUo) - dSo and row
+// N-1 = [dTLodS (node-major), dEUodS].  t is the tangent in the global
+// ordering (t[6g+q] = dTLodS(g,q), t[N-1] = dEUodS).
+int vo_arclength(int mode, int n_nodes, const double* x, const double* xo, const double* t, int increment,
+                 double LN0, double dArN, double dSo, int jac, const int64_t* rowptr, const int32_t* col,
+                 double* vals, double* b) {
+  const int64_t N = (int64_t)n_nodes * 6 + 1;
+  if (mode == 0) {
+    b[N - 1] = x[N - 1] - (LN0 + (double)(increment - 1) * dArN);
+    if (jac) {
+      for (int64_t r = 0; r < N; r++)
+        for (int64_t p = rowptr[r]; p < rowptr[r + 1]; p++)
+          if (r == N - 1 || col[p] == N - 1) vals[p] = (r == N - 1 && col[p] == N - 1) ? 1.0 : 0.0;
+    }
+    return 0;
+  }
+  double value = 0.0;
+  for (int q = 0; q < 6; q++) {
+    double d = 0.0;
+    for (int64_t g = 0; g < n_nodes; g++) d = d + t[g * 6 + q] * (x[g * 6 + q] - xo[g * 6 + q]);
+    value = value + d;
+  }
+  value = value + t[N - 1] * (x[N - 1] - xo[N - 1]) - dSo;
+  b[N - 1] = value;
+  if (jac)
+    for (int64_t p = rowptr[N - 1]; p < rowptr[N]; p++) vals[p] = t[col[p]];
+  return 0;
+}
+
 #else  // VEF_COUNT_FLOPS
 
 // Operation counts of one element-residual evaluation (bulk or edge) at a given

@@ -140,9 +140,53 @@ def test_workload_residual_flags_and_determinism(big):
     assert torch.isfinite(v).all()
 
 
+def test_workload_sampled_rows_bitwise(oracle, big):
+    """Full-size scatter: the stored rows of 40 sampled nodes equal the sum, in
+    element order then outlet-edge order, of the oracle's element Jacobians."""
+    mesh, x, asm, xd, b, v = big
+    print("[gpu parity] 100k sampled rows", flush=True)
+    rowptr, col = asm.pattern()
+    bh, vh = b.cpu().numpy(), v.cpu().numpy()
+    prm = oracle.params(**PARAM_SETS["oldroyd_b"])
+    TLg = x[:-1].reshape(-1, 6)
+    N = asm.N
+    rng = np.random.default_rng(1)
+    outlet = {int(e): int(ied) for e, ied in mesh.outlet_edges}
+    nodes = list(rng.choice(mesh.n_nodes, 36, replace=False))
+    nodes += [int(mesh.conn[int(mesh.outlet_edges[0, 0]), 1]), int(mesh.conn[int(mesh.outlet_edges[-1, 0]), 2])]
+    for r in nodes:
+        elems = sorted(set(np.nonzero((mesh.conn == r).any(axis=1))[0].tolist()))
+        contribs = [(e, 0) for e in elems] + [(e, outlet[e]) for e in elems if e in outlet]
+        for q in range(6):
+            row = 6 * r + q
+            cols = col[rowptr[row]:rowptr[row + 1]]
+            pos = {int(c): i for i, c in enumerate(cols)}
+            exp = np.zeros(len(cols))
+            seen = np.zeros(len(cols), bool)
+            bexp = 0.0
+            for e, ied in contribs:
+                en = mesh.conn[e]
+                Res, A, Ac = oracle.element(prm, mesh.coords[en], TLg[en], x[-1], ied=ied)
+                jn = int(np.nonzero(en == r)[0][0])
+                bexp += Res[jn, q]
+                for inn in range(9):
+                    for iq in range(6):
+                        i = pos[6 * int(en[inn]) + iq]
+                        val = A[inn * 6 + iq, jn * 6 + q]
+                        exp[i] = val if not seen[i] else exp[i] + val
+                        seen[i] = True
+                i = pos[N - 1]
+                exp[i] = Ac[jn * 6 + q] if not seen[i] else exp[i] + Ac[jn * 6 + q]
+                seen[i] = True
+            assert np.array_equal(vh[rowptr[row]:rowptr[row + 1]], exp), (r, q)
+            assert bh[row] == bexp, (r, q)
+
+
 def test_workload_jacobian_times_vector(big):
-    """A v ~ (R(x + d v) - R(x - d v)) / 2d for a smooth direction v (the FD
-    Jacobian is accurate to ~1e-7 relative)."""
+    """A v against a central directional difference of two GPU residuals: a
+    sanity check of the whole matrix at full size (the FD Jacobian carries
+    roundoff ~ eps*|R|/h per entry, so the bound is relative to each row's
+    sum of |A_ij v_j|)."""
     mesh, x, asm, xd, b, v = big
     rowptr, col = asm.pattern()
     N = asm.N
@@ -153,9 +197,11 @@ def test_workload_jacobian_times_vector(big):
         dirn[q:N - 1:6] = np.sin(rng.uniform(0.3, 1.0) * xy[:, 0] + rng.uniform(0, 6)) * np.cos(
             rng.uniform(0.3, 1.0) * xy[:, 1])
     dirn[-1] = 0.3
-    A = torch.sparse_csr_tensor(torch.from_numpy(rowptr), torch.from_numpy(col.astype(np.int64)), v.cpu(),
-                                size=(N, N), dtype=torch.float64)
-    Av = (A @ torch.from_numpy(dirn).unsqueeze(1)).squeeze(1).numpy()
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((v.cpu().numpy(), col, rowptr), shape=(N, N))
+    Av = A @ dirn
+    absAv = abs(A) @ np.abs(dirn)
     d = 1e-6
     bp, _ = asm.assemble(torch.from_numpy(x + d * dirn).cuda(), "MNR")
     bp = bp.cpu().numpy()
@@ -163,5 +209,35 @@ def test_workload_jacobian_times_vector(big):
     bm = bm.cpu().numpy()
     fd = (bp - bm) / (2 * d)
     err = np.abs(Av - fd)[:-1]
-    scale = np.abs(fd).max()
-    assert err.max() < 1e-5 * scale, (err.max(), scale)
+    s = absAv[:-1] + np.abs(fd[:-1]) + 1e-300
+    # forward differences with h = 1e-8 carry roundoff ~1e-16*|R|/h per entry;
+    # an indexing error in the scatter would give err ~ s
+    assert np.all(err <= 2e-2 * s), (err / s).max()
+    assert np.median(err / s) < 1e-6
+
+
+@pytest.mark.parametrize("increment", [1, 2, 3])
+def test_arclength_row(oracle, increment):
+    mesh = cylinder_channel(4, 2, 3, 4)
+    pset = PARAM_SETS["lptt"]
+    rng = np.random.default_rng(increment)
+    x = smooth_state(mesh, EU=0.7)
+    xo = x + 1e-3 * rng.standard_normal(x.size)
+    t = rng.standard_normal(x.size)
+    LN0, dArN, dS = 0.668713921947, -1.0e-3, 0.25
+    print(f"[gpu parity] arclength row increment={increment}", flush=True)
+    asm = Assembler(mesh, Params(**pset))
+    xd, xod, td = (torch.from_numpy(a).cuda() for a in (x, xo, t))
+    b, v = asm.assemble(xd, "NRP")
+    asm.arclength(xd, b, v, increment=increment, LN0=LN0, dArN=dArN, x_old=xod, tangent=td, dS=dS)
+    torch.cuda.synchronize()
+    b, v = b.cpu().numpy(), v.cpu().numpy()
+    rowptr, col = asm.pattern()
+    ob, ov = oracle.assemble(oracle.params(**pset), mesh, x, rowptr=rowptr, col=col)
+    oracle.arclength(0 if increment <= 2 else 1, mesh.n_nodes, x, xo, t, increment, LN0, dArN, dS, rowptr, col, ov, ob)
+    assert np.array_equal(v, ov)
+    assert np.array_equal(b[:-1], ob[:-1])
+    if increment <= 2:
+        assert b[-1] == ob[-1]
+    else:
+        assert abs(b[-1] - ob[-1]) <= 1e-13 * (np.abs(t * (x - xo)).sum() + dS)

@@ -100,6 +100,9 @@ def load_library():
     lib.vef_device_colind.restype = c_vp
     lib.vef_assemble.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]
     lib.vef_assemble.restype = ctypes.c_int
+    lib.vef_arclength.argtypes = [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, c_i32, c_vp, c_vp, c_vp]
+    lib.vef_arclength.restype = ctypes.c_int
     lib.vef_assemble_host.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp]
     lib.vef_assemble_host.restype = ctypes.c_int
     lib.vef_copy_residual_bank.argtypes = [c_vp, c_vp]
@@ -202,6 +205,22 @@ class Assembler:
                                    ctypes.c_void_p(stream.cuda_stream))
         self._check(rc, "vef_assemble")
         return b, (values if flag == VEF_RES_JAC else None)
+
+    def arclength(self, x, b, values=None, *, increment: int, LN0: float = 0.0, dArN: float = 0.0, x_old=None,
+                  tangent=None, dS: float = 0.0, flag_nr: str = "NRP", stream=None):
+        """Arclength row N-1 (src/Extra_Equations_mod.F90:698-828): pins lambda
+        for increments 1-2, pseudo-arclength row from increment 3 on."""
+        import torch
+
+        flag = self._flag(flag_nr)
+        mode = 0 if increment <= 2 else 1
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        rc = self.lib.vef_arclength(self._h, mode, ptr(x), ptr(x_old), ptr(tangent), int(increment), float(LN0),
+                                    float(dArN), float(dS), flag, ptr(b), ptr(values),
+                                    ctypes.c_void_p(stream.cuda_stream))
+        self._check(rc, "vef_arclength")
 
     def assemble_host(self, x: np.ndarray, flag_nr: str = "NRP"):
         """The same with numpy arrays (copies through the host)."""

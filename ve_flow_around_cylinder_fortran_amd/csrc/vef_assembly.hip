@@ -492,18 +492,7 @@ __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, 
   out6[5] = T5 * WET;
 }
 
-// Lane state: unknown j of the element is u[j] for j > p, fl(u+h) for j == p
-// and fl(fl(u+h)-h) for j < p — the in-place perturb/restore sequence of
-// storeBulk (Equations_mod.F90:267-284) seen by perturbation p.
-struct LaneState {
-  const double* u;  // [3][54] in LDS: original, restored (drifted), perturbed
-  int p;
-  __device__ __forceinline__ double operator()(int n, int q) const {
-    const int j = n * 6 + q;
-    const int sel = (j < p) ? 54 : ((j == p) ? 108 : 0);
-    return u[sel + j];
-  }
-};
+// Nodal unknowns of one evaluation: u[6*inod + q] in LDS.
 struct PlainState {
   const double* u;  // [54]
   __device__ __forceinline__ double operator()(int n, int q) const { return u[n * 6 + q]; }
@@ -524,32 +513,50 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   constexpr int NSLOT = EPB * KPB;
   static_assert(NSLOT == 7, "seven slots per wavefront");
   constexpr int BPE = NEVAL / KPB;  // workgroups per element group
-  constexpr int NSTATE = (NEVAL > 1) ? 3 : 1;
-  __shared__ double sU[EPB][NSTATE * 54];
+  __shared__ double sS[NSLOT][54];  // the nodal unknowns each slot evaluates at
   __shared__ double sX[EPB][18];
   __shared__ Geo sG[EPB][9];
   __shared__ double sRed[2][9][NSLOT][6];
 
   const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)(blockIdx.x / BPE) * EPB;
-  const int kg = blockIdx.x % BPE;
+  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+  // L2).  With BPE > 1 the BPE workgroups of one element are given ids that
+  // land on the same XCD (ids 64m + 8j + x, j = 0..7, for element 8m + x), so
+  // the element's inputs are fetched once per L2 and its bank rows, written in
+  // 7-evaluation pieces, merge in one L2 before reaching HBM.
+  int64_t egroup;
+  int kg;
+  if (BPE == 8) {
+    const int64_t b = blockIdx.x;
+    egroup = (b >> 6) * 8 + (b & 7);
+    kg = (int)((b >> 3) & 7);
+  } else {
+    egroup = blockIdx.x / BPE;
+    kg = blockIdx.x % BPE;
+  }
+  const int64_t e0 = egroup * EPB;
   const double EU = x[eu_index];
 
-  // Gather element unknowns and coordinates (copyToLocal).
-  for (int t = tid; t < EPB * 54; t += 64) {
-    const int el = t / 54, j = t % 54;
+  // Gather element unknowns (copyToLocal) and build each slot's state: for
+  // evaluation k of an element, unknown j is u_j for j > p, fl(u_p+h) at p and
+  // fl(fl(u_j+h)-h) for j < p, p = k-1 — the in-place perturb/restore sequence
+  // of storeBulk (Equations_mod.F90:267-306) as perturbation p sees it.
+  for (int t = tid; t < NSLOT * 54; t += 64) {
+    const int ss = t / 54, j = t % 54;
+    const int el = ss / KPB;
+    const int k = kg * KPB + ss % KPB;
     const int64_t e = e0 + el;
     double u = 0.0;
     if (e < n_el) {
       const int32_t gn = conn[e * 9 + j / 6];
       u = x[(int64_t)gn * 6 + (j % 6)];
     }
-    sU[el][j] = u;
-    if (NSTATE == 3) {
+    if (NEVAL > 1) {
+      const int p = k - 1;
       const double up = u + kEpsJac;
-      sU[el][108 + j] = up;
-      sU[el][54 + j] = up - kEpsJac;
+      u = (j < p) ? up - kEpsJac : ((j == p) ? up : u);
     }
+    sS[ss][j] = u;
   }
   for (int t = tid; t < EPB * 18; t += 64) {
     const int el = t / 18, c = t % 18;
@@ -578,13 +585,8 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   GP q;
   if (valid) {
     const double EUk = (NEVAL > 1 && k == NEVAL - 1) ? EU + kEpsJac : EU;
-    if (NEVAL > 1) {
-      LaneState st{sU[el], k - 1};
-      gauss_point<MODEL, DCS, INERTIA>(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
-    } else {
-      PlainState st{sU[el]};
-      gauss_point<MODEL, DCS, INERTIA>(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
-    }
+    PlainState st{sS[s]};
+    gauss_point<MODEL, DCS, INERTIA>(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
   }
   const double WET = sG[el][g].WET;
 
@@ -747,45 +749,68 @@ __device__ __forceinline__ double fd_quotient(double rk, double r0) {
   return qdiv((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh, true});
 }
 
+// Thread (pair pp = (r, c), column equation iq): the 6 slots of rows 6r+jq,
+// column 6c+iq.  The thread of the diagonal pair with iq = 0 also produces
+// b[6r+jq] and the dR/dEU entry (column N-1) of the 6 rows: the elements that
+// contribute to the diagonal pair are exactly those containing r.
 __global__ void __launch_bounds__(256)
-vef_gather_kernel(int64_t n_node_rows, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ pair_start,
+vef_gather_kernel(int64_t n_pairs, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ pair_start,
+                  const int32_t* __restrict__ pair_row, const int64_t* __restrict__ diag_pair,
                   const int32_t* __restrict__ cstart, const int32_t* __restrict__ contrib,
                   const int32_t* __restrict__ estart, const int32_t* __restrict__ econtrib,
-                  const double* __restrict__ bank, const double* __restrict__ ebank, double* __restrict__ values) {
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n_node_rows) return;
-  const int64_t r = row / 6;
-  const int jq = (int)(row % 6);
-  const int64_t p0 = pair_start[r];
-  const int64_t npos = (pair_start[r + 1] - p0) * 6;  // node-block columns; column N-1 follows
-  const int64_t base = rowptr[row];
-  for (int64_t pos = lane; pos < npos; pos += 64) {
-    const int64_t pp = p0 + pos / 6;
-    const int iq = (int)(pos % 6);
-    double acc = 0.0;
-    bool first = true;
-    for (int32_t c = cstart[pp]; c < cstart[pp + 1]; c++) {
-      const int32_t w = contrib[c];
-      const int64_t e = w / 81;
-      const int jn = (w / 9) % 9, in = w % 9;
-      const double* R = bank + (e * 54 + jn * 6 + jq) * kNeval;
-      const double v = fd_quotient(R[1 + in * 6 + iq], R[0]);
-      acc = first ? v : acc + v;
-      first = false;
-    }
-    if (jq < 2) {
-      for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
-        const int32_t w = econtrib[c];
-        const int64_t ed = w / 81;
-        const int jn = (w / 9) % 9, in = w % 9;
-        const double* R = ebank + (ed * 18 + jn * 2 + jq) * kNeval;
-        const double v = fd_quotient(R[1 + in * 6 + iq], R[0]);
-        acc = first ? v : acc + v;
-        first = false;
+                  const double* __restrict__ bank, const double* __restrict__ ebank, double* __restrict__ values,
+                  double* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_pairs * 6) return;
+  const int64_t pp = t / 6;
+  const int iq = (int)(t % 6);
+  const int64_t r = pair_row[pp];
+  const int64_t pos = pp - pair_start[r];
+  const bool rhs = (iq == 0) && (diag_pair[r] == pp);
+  double acc[6], accb[6], acca[6];
+  const int32_t c0 = cstart[pp], c1 = cstart[pp + 1];
+  for (int32_t c = c0; c < c1; c++) {
+    const int32_t w = contrib[c];
+    const int64_t e = w / 81;
+    const int jn = (w / 9) % 9, in = w % 9;
+    const double* R = bank + (e * 54 + jn * 6) * kNeval;
+    const int kk = 1 + in * 6 + iq;
+#pragma unroll
+    for (int jq = 0; jq < 6; jq++) {
+      const double r0 = R[jq * kNeval];
+      const double v = fd_quotient(R[jq * kNeval + kk], r0);
+      acc[jq] = (c == c0) ? v : acc[jq] + v;
+      if (rhs) {
+        const double va = fd_quotient(R[jq * kNeval + kNeval - 1], r0);
+        accb[jq] = (c == c0) ? r0 : accb[jq] + r0;
+        acca[jq] = (c == c0) ? va : acca[jq] + va;
       }
     }
-    values[base + pos] = acc;
+  }
+  for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
+    const int32_t w = econtrib[c];
+    const int64_t ed = w / 81;
+    const int jn = (w / 9) % 9, in = w % 9;
+    const double* R = ebank + (ed * 18 + jn * 2) * kNeval;
+    const int kk = 1 + in * 6 + iq;
+#pragma unroll
+    for (int jq = 0; jq < 2; jq++) {
+      const double r0 = R[jq * kNeval];
+      acc[jq] = acc[jq] + fd_quotient(R[jq * kNeval + kk], r0);
+      if (rhs) {
+        accb[jq] = accb[jq] + r0;
+        acca[jq] = acca[jq] + fd_quotient(R[jq * kNeval + kNeval - 1], r0);
+      }
+    }
+  }
+#pragma unroll
+  for (int jq = 0; jq < 6; jq++) {
+    const int64_t row = r * 6 + jq;
+    values[rowptr[row] + pos * 6 + iq] = acc[jq];
+    if (rhs) {
+      b[row] = accb[jq];
+      values[rowptr[row + 1] - 1] = acca[jq];
+    }
   }
 }
 
@@ -831,6 +856,68 @@ vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr,
   if (jac) values[rowptr[row + 1] - 1] = ra;
 }
 
+// ----------------------------------------------------------------------------
+// Arclength row (arclength / pseudoArclength, src/Extra_Equations_mod.F90:698-828)
+// ----------------------------------------------------------------------------
+// Increment <= 2: MatZeroRowsColumns(N-1) with diagonal 1.  Column N-1 is the
+// last stored entry of every row; row N-1 is dense.
+__global__ void vef_arc_pin_kernel(int64_t N, const int64_t* __restrict__ rowptr, double* __restrict__ values) {
+  const int64_t last = rowptr[N - 1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < N - 1) values[rowptr[i + 1] - 1] = 0.0;
+    values[last + i] = (i == N - 1) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void vef_arc_pin_rhs_kernel(int64_t N, const double* __restrict__ x, int32_t increment, double LN0,
+                                       double dArN, double* __restrict__ b) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) b[N - 1] = x[N - 1] - (LN0 + (double)(increment - 1) * dArN);
+}
+
+constexpr int kArcChunk = 4096;  // nodes per partial sum
+
+// Per-field partial sums of t*(x - xo) over fixed node chunks; fixed
+// reduction tree, so the result is deterministic.
+__global__ void __launch_bounds__(256)
+vef_arc_dot_kernel(int64_t n_nodes, const double* __restrict__ x, const double* __restrict__ xo,
+                   const double* __restrict__ t, double* __restrict__ partials) {
+  __shared__ double red[6][256];
+  const int64_t g0 = (int64_t)blockIdx.x * kArcChunk;
+  const int64_t g1 = (g0 + kArcChunk < n_nodes) ? g0 + kArcChunk : n_nodes;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += 256)
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc[q] = acc[q] + t[g * 6 + q] * (x[g * 6 + q] - xo[g * 6 + q]);
+#pragma unroll
+  for (int q = 0; q < 6; q++) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 6; q++) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void vef_arc_rhs_kernel(int64_t N, int nblk, const double* __restrict__ partials,
+                                   const double* __restrict__ x, const double* __restrict__ xo,
+                                   const double* __restrict__ t, double dSo, double* __restrict__ b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double value = 0.0;
+  for (int q = 0; q < 6; q++) {
+    double d = 0.0;
+    for (int i = 0; i < nblk; i++) d = d + partials[(int64_t)q * nblk + i];
+    value = value + d;
+  }
+  b[N - 1] = value + t[N - 1] * (x[N - 1] - xo[N - 1]) - dSo;
+}
+
+__global__ void vef_copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void vef_fill_kernel(double* __restrict__ p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = 0.0;
@@ -856,12 +943,16 @@ struct vef_handle {
   int32_t* d_col = nullptr;
   int64_t* d_pair_start = nullptr;
   int64_t* d_diag_pair = nullptr;
+  int32_t* d_pair_row = nullptr;
+  int64_t n_pairs = 0;
   int32_t* d_cstart = nullptr;
   int32_t* d_contrib = nullptr;
   int32_t* d_estart = nullptr;
   int32_t* d_econtrib = nullptr;
   double* d_bank = nullptr;   // n_el*54*56
   double* d_ebank = nullptr;  // n_edges*18*56
+  double* d_arc = nullptr;    // per-field partial sums of the arclength dot product
+  int arc_blocks = 0;
   double* d_tmp_x = nullptr;  // host-assemble staging
   double* d_tmp_b = nullptr;
   double* d_tmp_v = nullptr;
@@ -933,9 +1024,10 @@ int upload_tables(vef_handle* h) {
 namespace {
 template <int MODEL, bool DCS, bool INERTIA>
 void launch_bulk_t(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
-  if (jac)
-    hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>), dim3((unsigned)h->n_el * (kNeval / 7)),
-                       dim3(64), 0, st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_bank);
+  if (jac)  // 8 workgroups per element, elements padded to a multiple of 8 (XCD-aware ids)
+    hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>),
+                       dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_coords, h->d_conn, x, eu,
+                       h->n_el, h->dprm, h->d_bank);
   else
     hipLaunchKernelGGL((vef_bulk_kernel<7, 1, 1, MODEL, DCS, INERTIA>), dim3((unsigned)(h->n_el + 6) / 7), dim3(64), 0,
                        st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_bank);
@@ -980,7 +1072,8 @@ void vef_destroy(vef_handle* h) {
   for (hipEvent_t e : h->ev_bulk) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_scatter) (void)hipEventDestroy(e);
   void* ptrs[] = {h->d_coords, h->d_conn, h->d_edges, h->d_rowptr, h->d_col, h->d_pair_start, h->d_diag_pair,
-                  h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, h->d_tmp_x,
+                  h->d_pair_row,
+                  h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, h->d_arc, h->d_tmp_x,
                   h->d_tmp_b, h->d_tmp_v};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1070,6 +1163,10 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
     pair_start[r + 1] = (int64_t)pair_col.size();
   }
   const int64_t npairs = (int64_t)pair_col.size();
+  std::vector<int32_t> pair_row(npairs);
+  for (int32_t r = 0; r < n_nodes; r++)
+    for (int64_t pp = pair_start[r]; pp < pair_start[r + 1]; pp++) pair_row[pp] = r;
+  h->n_pairs = npairs;
   // Outlet-edge contributions: rows of the element's nodes, columns of its nodes.
   std::vector<int32_t> ecount(npairs + 1, 0);
   auto find_pair = [&](int32_t r, int32_t c) -> int64_t {
@@ -1134,12 +1231,15 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   UP(h->d_col, h->h_col.data(), (size_t)h->nnz);
   UP(h->d_pair_start, pair_start.data(), (size_t)n_nodes + 1);
   UP(h->d_diag_pair, diag_pair.data(), (size_t)n_nodes);
+  UP(h->d_pair_row, pair_row.data(), pair_row.size());
   UP(h->d_cstart, cstart.data(), cstart.size());
   UP(h->d_contrib, contrib.data(), contrib.size());
   UP(h->d_estart, estart.data(), estart.size());
   UP(h->d_econtrib, econtrib.data(), econtrib.size());
   UP(h->d_bank, (const double*)nullptr, (size_t)n_el * 54 * kNeval);
   UP(h->d_ebank, (const double*)nullptr, (size_t)std::max(n_edges, 1) * 18 * kNeval);
+  h->arc_blocks = (int)((n_nodes + kArcChunk - 1) / kArcChunk);
+  UP(h->d_arc, (const double*)nullptr, (size_t)6 * h->arc_blocks);
 #undef UP
   if (upload_tables(h)) {
     g_create_err = h->err;
@@ -1174,19 +1274,21 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
   const int64_t node_rows = (int64_t)h->n_nodes * 6;
   if (h->profiling) VEF_HIP(h, hipEventRecord(ev[2], st));
   if (jac) {
-    hipLaunchKernelGGL(vef_gather_kernel, dim3((unsigned)((node_rows + 3) / 4)), dim3(256), 0, st, node_rows,
-                       h->d_rowptr, h->d_pair_start, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
-                       h->d_ebank, values);
+    hipLaunchKernelGGL(vef_gather_kernel, dim3((unsigned)((h->n_pairs * 6 + 255) / 256)), dim3(256), 0, st,
+                       h->n_pairs, h->d_rowptr, h->d_pair_start, h->d_pair_row, h->d_diag_pair, h->d_cstart,
+                       h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, values, b);
     VEF_HIP(h, hipGetLastError());
     // Row N-1 (arclength row) is not touched by the bulk assembly: zero.
     const int64_t last0 = h->h_rowptr[h->N - 1];
     hipLaunchKernelGGL(vef_fill_kernel, dim3(1024), dim3(256), 0, st, values + last0, h->nnz - last0);
     VEF_HIP(h, hipGetLastError());
   }
-  hipLaunchKernelGGL(vef_rhs_kernel, dim3((unsigned)((node_rows + 255) / 256)), dim3(256), 0, st, node_rows, jac,
-                     h->d_rowptr, h->d_diag_pair, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
-                     jac ? kNeval : 1, h->d_ebank, values, b);
-  VEF_HIP(h, hipGetLastError());
+  if (!jac) {
+    hipLaunchKernelGGL(vef_rhs_kernel, dim3((unsigned)((node_rows + 255) / 256)), dim3(256), 0, st, node_rows, jac,
+                       h->d_rowptr, h->d_diag_pair, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
+                       1, h->d_ebank, values, b);
+    VEF_HIP(h, hipGetLastError());
+  }
   hipLaunchKernelGGL(vef_fill_kernel, dim3(1), dim3(64), 0, st, b + (h->N - 1), (int64_t)1);
   VEF_HIP(h, hipGetLastError());
   if (h->profiling) {
@@ -1235,6 +1337,39 @@ int vef_profile_read(vef_handle* h, double* bulk_ms, double* scatter_ms, int64_t
   *scatter_ms = ts;
   *launches = (int64_t)(h->ev_bulk.size() / 2);
   release_events(h);
+  return VEF_OK;
+}
+
+int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
+                  int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
+                  void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "bad flag");
+  if (flag == VEF_RES_JAC && !values) return set_err(h, VEF_ERR_ARG, "values is required for VEF_RES_JAC");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t N = h->N;
+  if (mode == VEF_ARC_PIN) {
+    hipLaunchKernelGGL(vef_arc_pin_rhs_kernel, dim3(1), dim3(64), 0, st, N, x, increment, LN0, dArN, b);
+    VEF_HIP(h, hipGetLastError());
+    if (flag == VEF_RES_JAC) {
+      hipLaunchKernelGGL(vef_arc_pin_kernel, dim3(1024), dim3(256), 0, st, N, h->d_rowptr, values);
+      VEF_HIP(h, hipGetLastError());
+    }
+    return VEF_OK;
+  }
+  if (mode != VEF_ARC_PSEUDO) return set_err(h, VEF_ERR_ARG, "mode must be VEF_ARC_PIN or VEF_ARC_PSEUDO");
+  if (!x_old || !tangent) return set_err(h, VEF_ERR_ARG, "x_old and tangent are required for VEF_ARC_PSEUDO");
+  hipLaunchKernelGGL(vef_arc_dot_kernel, dim3(h->arc_blocks), dim3(256), 0, st, (int64_t)h->n_nodes, x, x_old,
+                     tangent, h->d_arc);
+  VEF_HIP(h, hipGetLastError());
+  hipLaunchKernelGGL(vef_arc_rhs_kernel, dim3(1), dim3(64), 0, st, N, h->arc_blocks, h->d_arc, x, x_old, tangent, dS,
+                     b);
+  VEF_HIP(h, hipGetLastError());
+  if (flag == VEF_RES_JAC) {
+    hipLaunchKernelGGL(vef_copy_kernel, dim3(1024), dim3(256), 0, st, values + h->h_rowptr[N - 1], tangent, N);
+    VEF_HIP(h, hipGetLastError());
+  }
   return VEF_OK;
 }
 
