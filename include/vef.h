@@ -122,16 +122,11 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
 /* The same with host arrays (copies in, assembles, copies out). */
 int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values);
 
-/* Raw forward-difference bank of the last vef_assemble: for element e, row
- * j = 6*inod + q of the local residual and evaluation k (0: base state,
- * 1..54: unknown k-1 perturbed, 55: EU perturbed), the residual R_k(j) is at
- * bank[(e*54 + j)*56 + k].  Copies n_elements*54*56 doubles to host. */
-int vef_copy_residual_bank(const vef_handle* h, double* bank_host);
-
 /* Kernel timing, the counterpart of the reference's "Bulk EQs time" and
  * "Assembly time" prints (src/Newton_Raphson_mod.F90:337,398).  While enabled,
- * each vef_assemble records HIP events on its stream around the element
- * kernel (bulk) and around the scatter kernels.  vef_profile_read waits for
+ * each vef_assemble records HIP events on its stream around the
+ * forward-difference element kernel (bulk_ms) and around the kernels after it
+ * (ordered sums of shared entries, outlet-edge entries, b; scatter_ms).  vef_profile_read waits for
  * the recorded events, returns the summed milliseconds and the number of
  * assemblies since the last read, and clears them.  (Re-)enabling clears too. */
 int vef_profile(vef_handle* h, int32_t enable);

@@ -69,7 +69,8 @@ class Params:
 
 
 def library_path() -> str:
-    return os.path.join(os.path.dirname(os.path.abspath(__file__)), _LIB_NAME)
+    """The in-tree libvef.so, or the file VEF_LIBRARY names."""
+    return os.environ.get("VEF_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), _LIB_NAME)
 
 
 def load_library():
@@ -105,8 +106,6 @@ def load_library():
     lib.vef_arclength.restype = ctypes.c_int
     lib.vef_assemble_host.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp]
     lib.vef_assemble_host.restype = ctypes.c_int
-    lib.vef_copy_residual_bank.argtypes = [c_vp, c_vp]
-    lib.vef_copy_residual_bank.restype = ctypes.c_int
     lib.vef_profile.argtypes = [c_vp, c_i32]
     lib.vef_profile.restype = ctypes.c_int
     lib.vef_profile_read.argtypes = [c_vp, c_vp, c_vp, c_vp]
@@ -243,12 +242,6 @@ class Assembler:
         self._check(self.lib.vef_profile_read(self._h, ctypes.byref(tb), ctypes.byref(ts), ctypes.byref(n)),
                     "vef_profile_read")
         return tb.value, ts.value, n.value
-
-    def residual_bank(self) -> np.ndarray:
-        """R_k(e, row) of the last Jacobian assembly, shape (n_el, 54, 56)."""
-        out = np.empty((self.mesh.n_elements, 54, 56))
-        self._check(self.lib.vef_copy_residual_bank(self._h, _ptr(out)), "vef_copy_residual_bank")
-        return out
 
     @staticmethod
     def _flag(flag_nr: str) -> int:

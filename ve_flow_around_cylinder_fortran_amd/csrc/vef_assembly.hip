@@ -32,6 +32,7 @@
 // compiled with -ffp-contract=off, so the FP64 results equal the sequential
 // reference evaluation bit for bit.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <algorithm>
 #include <cfloat>
@@ -156,35 +157,77 @@ struct DevParams {
 struct Geo {
   double dFx[9], dFy[9], d2x[9], d2y[9];
   double WET, H2;
-  double H4, yH4;     // H2*H2 and RN(1/(H2*H2)) (denominator of the complex divisions by H2)
+  double H4, yH4;     // H2*H2 and RN(1/(H2*H2)) (denominator of the complex division by H2)
   double r3, t7;      // BvN/H2 and DfN/H2
 };
 
 // Complex division with zero imaginary parts: re((a+0i)/(c+0i)) = (a*c)/(c*c).
 __device__ __forceinline__ double cdiv(double a, double c) { return (a * c) / (c * c); }
 
-// Division by a denominator shared by many numerators.  With y = RN(1/d) and
-// q0 = RN(a*y), the residual r = a - d*q0 is exact in one FMA and
-// q1 = RN(q0 + r*y) is the correctly rounded a/d (Markstein's theorem for
-// round-to-nearest), so q1 equals the IEEE division bit for bit as long as no
-// intermediate leaves the normal range; outside it the IEEE division is used.
+// Division by a denominator shared by many numerators, in two modes.
+//
+// Fast (EXACT = false): with y = RN(1/d), q0 = RN(a*y), e = RN(d*q0 - a) and
+// q1 = RN(q0 - e*y) (two FMAs), q1 is the correctly rounded a/d: the
+// correction term q0 - (d*q0 - a)*y differs from a/d by |1 - d*y|*|a/d - q0|,
+// far below the distance of any quotient of two doubles from a rounding
+// midpoint (Markstein's correction step; checked exhaustively over all pairs
+// of significands at precisions 4..17).  The argument is scale-free, so it
+// holds while every quantity stays in the normal range with room for the
+// residual: d in [2^-60, 2^40] and |q0| in [2^-901, 2^900), or a = +-0 (then
+// q0 = +-0, e = +0 and q1 = q0 keeps the sign of a/d because d > 0).  Instead
+// of branching, each division widens the exponent interval [lo, hi] of its q0
+// in a DivCheck, and a denominator outside its range sets hi to kBadExp; a
+// non-finite q0 turns q1 into NaN, which reaches the element residual.  The
+// caller repeats the whole evaluation in the exact mode when the interval or
+// the residual says so.
+//
+// Exact (EXACT = true): the IEEE division a/d.
 struct Recip {
   double d, y;
-  bool fast;
 };
-__device__ __forceinline__ Recip make_recip(double d) {
-  const double ad = fabs(d);
-  return Recip{d, 1.0 / d, ad >= 0x1p-500 && ad <= 0x1p500};
-}
-__device__ __forceinline__ double qdiv(double a, const Recip& r) {
-  const double q0 = a * r.y;
-  const double aq = fabs(q0);
-  if (__builtin_expect(r.fast && aq >= 0x1p-900 && aq <= 0x1p900, 1)) {
-    const double e = fma(-r.d, q0, a);
-    return fma(e, r.y, q0);
+struct DivCheck {
+  int lo, hi;
+  __device__ __forceinline__ bool failed() const { return lo < -900 || hi > 900; }
+};
+constexpr int kBadExp = 4096;
+
+template <bool EXACT>
+__device__ __forceinline__ void check_denominator(double d, DivCheck& dc) {
+  if constexpr (!EXACT) {
+    if (!(d >= 0x1p-60 && d <= 0x1p40)) dc.hi = kBadExp;
   }
-  return a / r.d;
 }
+template <bool EXACT>
+__device__ __forceinline__ Recip make_recip(double d, DivCheck& dc) {
+  if constexpr (EXACT) return Recip{d, 0.0};
+  check_denominator<EXACT>(d, dc);
+  return Recip{d, 1.0 / d};
+}
+template <bool EXACT>
+__device__ __forceinline__ double qdiv(double a, const Recip& r, DivCheck& dc) {
+  if constexpr (EXACT) {
+    return a / r.d;
+  } else {
+    const double q0 = a * r.y;
+    const double e = fma(r.d, q0, -a);
+    const int ex = __builtin_amdgcn_frexp_exp(q0);  // 0 for +-0; the binade of q0 otherwise
+    dc.lo = min(dc.lo, ex);
+    dc.hi = max(dc.hi, ex);
+    return fma(-e, r.y, q0);
+  }
+}
+
+template <bool EXACT>
+__device__ __forceinline__ double fd_quotient(double rk, double r0, DivCheck& dc) {
+  // (dRes - Res)/eps_jac with a complex numerator: ((dR-R)*h)/(h*h)
+  constexpr double hh = kEpsJac * kEpsJac;
+  return qdiv<EXACT>((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh}, dc);
+}
+__device__ __forceinline__ double fd_quotient_ieee(double rk, double r0) {
+  DivCheck dc{0, 0};
+  return fd_quotient<true>(rk, r0, dc);
+}
+
 
 // basis2D for one Gauss point from the nine node coordinates; fills Geo.
 __device__ void geometry(const double* X /*[9][2]*/, const double* dc, const double* de, const double* dcc,
@@ -244,9 +287,9 @@ struct GP {
 // signed zeros whose additions leave every value unchanged, so they are not
 // computed; likewise S^3 and C:C (multiplied by f(3) = 0 for L-PTT) and the DCS
 // terms (tdcs = 0 when DfN > 1e-12).
-template <int MODEL, bool DCS, bool INERTIA, class V>
+template <int MODEL, bool DCS, bool INERTIA, bool EXACT, class V>
 __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const Geo& G, double EU,
-                                            const DevParams& prm, GP& o) {
+                                            const DevParams& prm, GP& o, DivCheck& dc) {
   // Interpolation, each sum sequential over the nodes (dot_product).
   double Ux, Uy, GU00, GU10, GU01, GU11, G2x0, G2y0, G2x1, G2y1, P, dPx, dPy;
   double S00, S01, S11, dS00x, dS00y, dS01x, dS01y, dS11x, dS11y;
@@ -319,8 +362,8 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   o.P11 = T11 + prm.BvN * Gd11 - P;
 
   // setUpperConvectedDerivativeSQRT: UCD = U.grad S - S.GU_tr - W.S
-  const Recip rW = make_recip(S00 + S11);
-  const double W12 = qdiv(-(S00 * Gt01) + S01 * (Gt00 - Gt11) + S11 * Gt10, rW);
+  const Recip rW = make_recip<EXACT>(S00 + S11, dc);
+  const double W12 = qdiv<EXACT>(-(S00 * Gt01) + S01 * (Gt00 - Gt11) + S11 * Gt10, rW, dc);
   const double W21 = -W12;
   double U00 = Ux * dS00x + Uy * dS00y, U01 = Ux * dS01x + Uy * dS01y;
   double U10 = U01, U11 = Ux * dS11x + Uy * dS11y;
@@ -330,8 +373,9 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   U11 = U11 - (S10 * Gt01 + S11 * Gt11) - W21 * S01;
 
   // setSInverse (inverseTensor2D)
-  const Recip rS = make_recip(S00 * S11 - S01 * S10);
-  const double SI00 = qdiv(S11, rS), SI01 = qdiv(-S01, rS), SI10 = qdiv(-S10, rS), SI11 = qdiv(S00, rS);
+  const Recip rS = make_recip<EXACT>(S00 * S11 - S01 * S10, dc);
+  const double SI00 = qdiv<EXACT>(S11, rS, dc), SI01 = qdiv<EXACT>(-S01, rS, dc);
+  const double SI10 = qdiv<EXACT>(-S10, rS, dc), SI11 = qdiv<EXACT>(S00, rS, dc);
 
   // term_S = f3 S^3 + f2 S - f1 S^-1 ; Con_EQ = WiN*UCD + term_S/2
   double tS00, tS01, tS10, tS11;
@@ -382,7 +426,8 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   Ha = cdiv(Ha, sqrt(1.0 + 0.5 * ddG));
   // (ReN/dt)**2 with dt = +huge is exactly zero and adds nothing.
   const double r3 = G.r3;
-  const double r4 = qdiv(Ha * H2, Recip{G.H4, G.yH4, true});
+  check_denominator<EXACT>(G.H4, dc);
+  const double r4 = qdiv<EXACT>(Ha * H2, Recip{G.H4, G.yH4}, dc);
   double tlsme;
   if constexpr (INERTIA) {
     const double r2 = prm.ReN * tsugn;
@@ -422,7 +467,7 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   o.CE00 = CE00; o.CE01 = CE01; o.CE10 = CE10; o.CE11 = CE11;
   o.S00 = S00; o.S01 = S01; o.S11 = S11;
   o.rW = rW;
-  o.rW2 = make_recip(WiN * WiN);
+  o.rW2 = make_recip<EXACT>(WiN * WiN, dc);
   o.dS00x = dS00x; o.dS00y = dS00y; o.dS01x = dS01x; o.dS01y = dS01y; o.dS11x = dS11x; o.dS11y = dS11y;
   o.WiN = WiN; o.f_T = f_T;
 }
@@ -430,9 +475,9 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
 // Test-node rows of one Gauss point (momentumMainSQRT, continuityBulkMain,
 // stressesBulkMainSQRT and their LSME/LSIC/LSCE/DCS terms, Residuals_mod.F90:
 // 90-110,161-234,295-490), already multiplied by WET.
-template <int MODEL, bool DCS, bool INERTIA>
+template <int MODEL, bool DCS, bool INERTIA, bool EXACT>
 __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, double fy, const DevParams& prm,
-                                           double WET, double* out6) {
+                                           double WET, double* out6, DivCheck& dc) {
   const double S10 = g.S01;
   const double Udf = g.Ux * fx + g.Uy * fy;
   const double lsme_m = INERTIA ? g.tlsme * prm.ReN * Udf : 0.0;
@@ -440,14 +485,14 @@ __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, 
   {
     // i = 1: GUT_lsce row 1 = grad(phi)
     const double lsic = (g.tlsic * fx) * g.Cont;
-    const double w = qdiv(g.S01 * fx + g.S11 * fy, g.rW);  // omegaNominator2D / omegaDenominator2D
+    const double w = qdiv<EXACT>(g.S01 * fx + g.S11 * fy, g.rW, dc);  // omegaNominator2D / omegaDenominator2D
     const double m00 = g.S00 * fx + g.S01 * fy, m10 = S10 * fx + g.S11 * fy;
     const double u00 = -m00 - w * S10, u01 = -(w * g.S11);
     const double u10 = -m10 - (-w) * g.S00, u11 = -((-w) * g.S01);
-    const double l00 = qdiv((g.tlsce * u00) * prm.omb * g.WiN, g.rW2);
-    const double l01 = qdiv((g.tlsce * u01) * prm.omb * g.WiN, g.rW2);
-    const double l10 = qdiv((g.tlsce * u10) * prm.omb * g.WiN, g.rW2);
-    const double l11 = qdiv((g.tlsce * u11) * prm.omb * g.WiN, g.rW2);
+    const double l00 = qdiv<EXACT>((g.tlsce * u00) * prm.omb * g.WiN, g.rW2, dc);
+    const double l01 = qdiv<EXACT>((g.tlsce * u01) * prm.omb * g.WiN, g.rW2, dc);
+    const double l10 = qdiv<EXACT>((g.tlsce * u10) * prm.omb * g.WiN, g.rW2, dc);
+    const double l11 = qdiv<EXACT>((g.tlsce * u11) * prm.omb * g.WiN, g.rW2, dc);
     const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
     if constexpr (INERTIA) T0 = g.RC0 * bifn + (g.P00 * fx + g.P01 * fy) + lsme_m * g.Mom0 + lsic + lsce;
     else T0 = (g.P00 * fx + g.P01 * fy) + lsic + lsce;
@@ -455,14 +500,14 @@ __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, 
   {
     // i = 2: GUT_lsce row 2 = grad(phi)
     const double lsic = (g.tlsic * fy) * g.Cont;
-    const double w = qdiv(-(g.S00 * fx) + g.S01 * (-fy), g.rW);
+    const double w = qdiv<EXACT>(-(g.S00 * fx) + g.S01 * (-fy), g.rW, dc);
     const double m01 = g.S00 * fx + g.S01 * fy, m11 = S10 * fx + g.S11 * fy;
     const double u00 = -(w * S10), u01 = -m01 - w * g.S11;
     const double u10 = -((-w) * g.S00), u11 = -m11 - (-w) * g.S01;
-    const double l00 = qdiv((g.tlsce * u00) * prm.omb * g.WiN, g.rW2);
-    const double l01 = qdiv((g.tlsce * u01) * prm.omb * g.WiN, g.rW2);
-    const double l10 = qdiv((g.tlsce * u10) * prm.omb * g.WiN, g.rW2);
-    const double l11 = qdiv((g.tlsce * u11) * prm.omb * g.WiN, g.rW2);
+    const double l00 = qdiv<EXACT>((g.tlsce * u00) * prm.omb * g.WiN, g.rW2, dc);
+    const double l01 = qdiv<EXACT>((g.tlsce * u01) * prm.omb * g.WiN, g.rW2, dc);
+    const double l10 = qdiv<EXACT>((g.tlsce * u10) * prm.omb * g.WiN, g.rW2, dc);
+    const double l11 = qdiv<EXACT>((g.tlsce * u11) * prm.omb * g.WiN, g.rW2, dc);
     const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
     if constexpr (INERTIA) T1 = g.RC1 * bifn + (g.P10 * fx + g.P11 * fy) + lsme_m * g.Mom1 + lsic + lsce;
     else T1 = (g.P10 * fx + g.P11 * fy) + lsic + lsce;
@@ -506,17 +551,38 @@ struct PlainState {
 // residual and FD Jacobian (8 workgroups per element), NEVAL = 1 the residual
 // only (7 elements per workgroup).
 // ----------------------------------------------------------------------------
+// Where the FD quotients go (NEVAL = 56).  For element e and its node pair
+// (jn, in), code[e*81 + jn*9 + in] is pos<<1 when the element is the only one
+// containing both nodes — the 36 matrix entries of the pair are then written
+// directly, at rowptr[6r] + jq*len(r) + pos*6 + iq (r = node jn, every row of a
+// node has the same length) — and (cidx<<1)|1 otherwise, when they go to
+// cbank[cidx*36 + jq*6 + iq] for the ordered sum of vef_shared_kernel.  The
+// dR/dEU quotients go to acbank[e*54 + row].
+struct FdOut {
+  const double* r0;      // [n_el][54] base residual rows (residual pass)
+  const int32_t* code;   // [n_el][81]
+  const int64_t* rowptr;
+  double* values;
+  double* cbank;
+  double* acbank;
+};
+
 template <int EPB, int KPB, int NEVAL, int MODEL, bool DCS, bool INERTIA>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
-                int64_t eu_index, int32_t n_el, DevParams prm, double* __restrict__ bank) {
+                int64_t eu_index, int32_t n_el, DevParams prm, double* __restrict__ bank, FdOut fo) {
   constexpr int NSLOT = EPB * KPB;
   static_assert(NSLOT == 7, "seven slots per wavefront");
   constexpr int BPE = NEVAL / KPB;  // workgroups per element group
+  constexpr bool FD = NEVAL > 1;
   __shared__ double sS[NSLOT][54];  // the nodal unknowns each slot evaluates at
   __shared__ double sX[EPB][18];
   __shared__ Geo sG[EPB][9];
-  __shared__ double sRed[2][9][NSLOT][6];
+  __shared__ double sRed[9][NSLOT][6];  // one buffer: 10.1 KB of LDS per workgroup -> 16 per CU
+  __shared__ double sR0[FD ? 54 : 1];
+  __shared__ int32_t sCode[FD ? 81 : 1];
+  __shared__ int64_t sRb[FD ? 9 : 1];
+  __shared__ int32_t sRl[FD ? 9 : 1];
 
   const int tid = threadIdx.x;
   // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -565,6 +631,16 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     if (e < n_el) v = coords[(int64_t)conn[e * 9 + c / 2] * 2 + (c & 1)];
     sX[el][c] = v;
   }
+  if (FD && e0 < n_el) {
+    for (int t = tid; t < 81; t += 64) sCode[t] = fo.code[e0 * 81 + t];
+    if (tid < 54) sR0[tid] = fo.r0[e0 * 54 + tid];
+    if (tid < 9) {
+      const int64_t r6 = (int64_t)conn[e0 * 9 + tid] * 6;
+      const int64_t rb = fo.rowptr[r6];
+      sRb[tid] = rb;
+      sRl[tid] = (int32_t)(fo.rowptr[r6 + 1] - rb);
+    }
+  }
   __syncthreads();
   for (int t = tid; t < EPB * 9; t += 64) {
     const int el = t / 9, g = t % 9;
@@ -582,35 +658,64 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   const int k = kg * KPB + s % KPB;
   const bool valid = active && (e0 + el < n_el);
 
-  GP q;
-  if (valid) {
-    const double EUk = (NEVAL > 1 && k == NEVAL - 1) ? EU + kEpsJac : EU;
-    PlainState st{sS[s]};
-    gauss_point<MODEL, DCS, INERTIA>(st, c_tab.bf[g], sG[el][g], EUk, prm, q);
-  }
-  const double WET = sG[el][g].WET;
-
-  for (int n = 0; n < 9; n++) {
-    const int buf = n & 1;
+  // One pass over the wavefront's 7 slots x 9 Gauss points: Gauss-point
+  // quantities, then per test node the 6 terms, their ordered sum over the
+  // Gauss points and its destination.  The fast pass divides with shared
+  // reciprocals; if any lane's DivCheck or any sum says the fast divisions may
+  // differ from IEEE ones, the exact pass rewrites every output of the
+  // workgroup (the branch is uniform: one wavefront per workgroup).
+  auto evaluate = [&](auto exact_tag) -> bool {
+    constexpr bool EXACT = decltype(exact_tag)::value;
+    DivCheck dc{0, 0};
+    GP q;
     if (valid) {
-      double r[6];
-      node_terms<MODEL, DCS, INERTIA>(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET, r);
-#pragma unroll
-      for (int c = 0; c < 6; c++) sRed[buf][g][s][c] = r[c];
+      const double EUk = (NEVAL > 1 && k == NEVAL - 1) ? EU + kEpsJac : EU;
+      PlainState st{sS[s]};
+      gauss_point<MODEL, DCS, INERTIA, EXACT>(st, c_tab.bf[g], sG[el][g], EUk, prm, q, dc);
     }
-    __syncthreads();
-    // Ordered sum over the Gauss points: Res = Res + TERM_RES*WET, ig = 1..9.
-    if (tid < 6 * NSLOT) {
-      const int c = tid / NSLOT, ss = tid % NSLOT;
-      const int eel = ss / KPB, kk = kg * KPB + ss % KPB;
-      if (e0 + eel < n_el) {
-        double acc = sRed[buf][0][ss][c];
+    const double WET = sG[el][g].WET;
+
+    for (int n = 0; n < 9; n++) {
+      __syncthreads();  // the previous sums have been read
+      if (valid) {
+        double r[6];
+        node_terms<MODEL, DCS, INERTIA, EXACT>(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET,
+                                               r, dc);
 #pragma unroll
-        for (int gg = 1; gg < 9; gg++) acc = acc + sRed[buf][gg][ss][c];
-        bank[((e0 + eel) * 54 + n * 6 + c) * NEVAL + kk] = acc;
+        for (int c = 0; c < 6; c++) sRed[g][s][c] = r[c];
+      }
+      __syncthreads();
+      // Ordered sum over the Gauss points: Res = Res + TERM_RES*WET, ig = 1..9.
+      if (tid < 6 * NSLOT) {
+        const int c = tid / NSLOT, ss = tid % NSLOT;
+        const int eel = ss / KPB, kk = kg * KPB + ss % KPB;
+        if (e0 + eel < n_el) {
+          double acc = sRed[0][ss][c];
+#pragma unroll
+          for (int gg = 1; gg < 9; gg++) acc = acc + sRed[gg][ss][c];
+          if (!EXACT && !__builtin_isfinite(acc)) dc.hi = kBadExp;
+          if (!FD) {
+            bank[(e0 + eel) * 54 + n * 6 + c] = acc;
+          } else if (kk > 0) {
+            // (dRes - Res)/eps_jac for column kk-1 (or EU for kk = 55)
+            const double qv = fd_quotient<EXACT>(acc, sR0[n * 6 + c], dc);
+            if (kk == NEVAL - 1) {
+              fo.acbank[e0 * 54 + n * 6 + c] = qv;
+            } else {
+              const int in = (kk - 1) / 6, iq = (kk - 1) % 6;
+              const int32_t code = sCode[n * 9 + in];
+              if ((code & 1) == 0)
+                fo.values[sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 1) * 6 + iq] = qv;
+              else
+                fo.cbank[(int64_t)(code >> 1) * 36 + c * 6 + iq] = qv;
+            }
+          }
+        }
       }
     }
-  }
+    return dc.failed();
+  };
+  if (__ballot(evaluate(std::false_type{})) != 0) evaluate(std::true_type{});
 }
 
 // ----------------------------------------------------------------------------
@@ -743,75 +848,45 @@ vef_edge_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
 // outlet-edge contributions packed as ed*81 + jn*9 + in, ascending in ed.
 // One wavefront per matrix row; lane l handles row positions l, l+64, ...
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ double fd_quotient(double rk, double r0) {
-  // (dRes - Res)/eps_jac with a complex numerator: ((dR-R)*h)/(h*h)
-  constexpr double hh = kEpsJac * kEpsJac;
-  return qdiv((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh, true});
+
+// Entries of node pairs shared by several elements: thread (shared pair su,
+// jq, iq) sums the contributions of its elements in ascending element order.
+__global__ void __launch_bounds__(256)
+vef_shared_kernel(int64_t n_shared, const int64_t* __restrict__ sbase, const int32_t* __restrict__ slen,
+                  const int32_t* __restrict__ ustart, const int32_t* __restrict__ ulist,
+                  const double* __restrict__ cbank, double* __restrict__ values) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_shared * 36) return;
+  const int64_t su = t / 36;
+  const int jq = (int)((t % 36) / 6), iq = (int)(t % 6);
+  const int32_t u0 = ustart[su], u1 = ustart[su + 1];
+  double acc = cbank[(int64_t)ulist[u0] * 36 + jq * 6 + iq];
+  for (int32_t u = u0 + 1; u < u1; u++) acc = acc + cbank[(int64_t)ulist[u] * 36 + jq * 6 + iq];
+  values[sbase[su] + (int64_t)jq * slen[su] + iq] = acc;
 }
 
-// Thread (pair pp = (r, c), column equation iq): the 6 slots of rows 6r+jq,
-// column 6c+iq.  The thread of the diagonal pair with iq = 0 also produces
-// b[6r+jq] and the dR/dEU entry (column N-1) of the 6 rows: the elements that
-// contribute to the diagonal pair are exactly those containing r.
+// Outlet-edge Jacobian entries, added after every bulk contribution in
+// ascending edge order: thread (edge pair ep, momentum row jq, iq).
 __global__ void __launch_bounds__(256)
-vef_gather_kernel(int64_t n_pairs, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ pair_start,
-                  const int32_t* __restrict__ pair_row, const int64_t* __restrict__ diag_pair,
-                  const int32_t* __restrict__ cstart, const int32_t* __restrict__ contrib,
-                  const int32_t* __restrict__ estart, const int32_t* __restrict__ econtrib,
-                  const double* __restrict__ bank, const double* __restrict__ ebank, double* __restrict__ values,
-                  double* __restrict__ b) {
+vef_edge_add_kernel(int64_t n_epairs, const int64_t* __restrict__ ep_pair, const int64_t* __restrict__ ebase,
+                    const int32_t* __restrict__ elen, const int32_t* __restrict__ estart,
+                    const int32_t* __restrict__ econtrib, const double* __restrict__ ebank,
+                    double* __restrict__ values) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_pairs * 6) return;
-  const int64_t pp = t / 6;
-  const int iq = (int)(t % 6);
-  const int64_t r = pair_row[pp];
-  const int64_t pos = pp - pair_start[r];
-  const bool rhs = (iq == 0) && (diag_pair[r] == pp);
-  double acc[6], accb[6], acca[6];
-  const int32_t c0 = cstart[pp], c1 = cstart[pp + 1];
-  for (int32_t c = c0; c < c1; c++) {
-    const int32_t w = contrib[c];
-    const int64_t e = w / 81;
-    const int jn = (w / 9) % 9, in = w % 9;
-    const double* R = bank + (e * 54 + jn * 6) * kNeval;
-    const int kk = 1 + in * 6 + iq;
-#pragma unroll
-    for (int jq = 0; jq < 6; jq++) {
-      const double r0 = R[jq * kNeval];
-      const double v = fd_quotient(R[jq * kNeval + kk], r0);
-      acc[jq] = (c == c0) ? v : acc[jq] + v;
-      if (rhs) {
-        const double va = fd_quotient(R[jq * kNeval + kNeval - 1], r0);
-        accb[jq] = (c == c0) ? r0 : accb[jq] + r0;
-        acca[jq] = (c == c0) ? va : acca[jq] + va;
-      }
-    }
-  }
+  if (t >= n_epairs * 12) return;
+  const int64_t ep = t / 12;
+  const int jq = (int)((t % 12) / 6), iq = (int)(t % 6);
+  const int64_t pp = ep_pair[ep];
+  const int64_t slot = ebase[ep] + (int64_t)jq * elen[ep] + iq;
+  double v = values[slot];
   for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
     const int32_t w = econtrib[c];
     const int64_t ed = w / 81;
     const int jn = (w / 9) % 9, in = w % 9;
-    const double* R = ebank + (ed * 18 + jn * 2) * kNeval;
-    const int kk = 1 + in * 6 + iq;
-#pragma unroll
-    for (int jq = 0; jq < 2; jq++) {
-      const double r0 = R[jq * kNeval];
-      acc[jq] = acc[jq] + fd_quotient(R[jq * kNeval + kk], r0);
-      if (rhs) {
-        accb[jq] = accb[jq] + r0;
-        acca[jq] = acca[jq] + fd_quotient(R[jq * kNeval + kNeval - 1], r0);
-      }
-    }
+    const double* R = ebank + (ed * 18 + jn * 2 + jq) * kNeval;
+    v = v + fd_quotient_ieee(R[1 + in * 6 + iq], R[0]);
   }
-#pragma unroll
-  for (int jq = 0; jq < 6; jq++) {
-    const int64_t row = r * 6 + jq;
-    values[rowptr[row] + pos * 6 + iq] = acc[jq];
-    if (rhs) {
-      b[row] = accb[jq];
-      values[rowptr[row + 1] - 1] = acca[jq];
-    }
-  }
+  values[slot] = v;
 }
 
 // Residual b and the dR/dEU column: one thread per node row, summing over the
@@ -820,7 +895,7 @@ __global__ void __launch_bounds__(256)
 vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ diag_pair,
                const int32_t* __restrict__ cstart, const int32_t* __restrict__ contrib,
                const int32_t* __restrict__ estart, const int32_t* __restrict__ econtrib,
-               const double* __restrict__ bank, int bank_stride, const double* __restrict__ ebank,
+               const double* __restrict__ r0bank, const double* __restrict__ acbank, const double* __restrict__ ebank,
                double* __restrict__ values, double* __restrict__ b) {
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n_node_rows) return;
@@ -828,19 +903,12 @@ vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr,
   const int jq = (int)(row % 6);
   const int64_t pp = diag_pair[r];
   double rb = 0.0, ra = 0.0;
-  bool first = true;
-  for (int32_t c = cstart[pp]; c < cstart[pp + 1]; c++) {
+  const int32_t c0 = cstart[pp];
+  for (int32_t c = c0; c < cstart[pp + 1]; c++) {
     const int32_t w = contrib[c];
-    const int64_t e = w / 81;
-    const int jn = (w / 9) % 9;
-    const double* R = bank + (e * 54 + jn * 6 + jq) * bank_stride;
-    const double vb = R[0];
-    rb = first ? vb : rb + vb;
-    if (jac) {
-      const double va = fd_quotient(R[kNeval - 1], R[0]);
-      ra = first ? va : ra + va;
-    }
-    first = false;
+    const int64_t idx = (w / 81) * 54 + ((w / 9) % 9) * 6 + jq;
+    rb = (c == c0) ? r0bank[idx] : rb + r0bank[idx];
+    if (jac) ra = (c == c0) ? acbank[idx] : ra + acbank[idx];
   }
   if (jq < 2) {
     for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
@@ -849,7 +917,7 @@ vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr,
       const int jn = (w / 9) % 9;
       const double* R = ebank + (ed * 18 + jn * 2 + jq) * kNeval;
       rb = rb + R[0];
-      if (jac) ra = ra + fd_quotient(R[kNeval - 1], R[0]);
+      if (jac) ra = ra + fd_quotient_ieee(R[kNeval - 1], R[0]);
     }
   }
   b[row] = rb;
@@ -941,15 +1009,26 @@ struct vef_handle {
   int32_t* d_edges = nullptr;
   int64_t* d_rowptr = nullptr;
   int32_t* d_col = nullptr;
-  int64_t* d_pair_start = nullptr;
   int64_t* d_diag_pair = nullptr;
-  int32_t* d_pair_row = nullptr;
   int64_t n_pairs = 0;
+  // direct scatter map and the ordered sums of shared entries
+  int32_t* d_code = nullptr;     // [n_el][81]
+  int64_t* d_sbase = nullptr;    // shared pairs: first slot
+  int32_t* d_slen = nullptr;     //   row length
+  int32_t* d_ustart = nullptr;   //   contribution list
+  int32_t* d_ulist = nullptr;
+  int64_t n_shared = 0, n_cidx = 0;
+  int64_t* d_ep_pair = nullptr;  // pairs with outlet-edge contributions
+  int64_t* d_ebase = nullptr;
+  int32_t* d_elen = nullptr;
+  int64_t n_epairs = 0;
+  double* d_r0bank = nullptr;    // [n_el][54] base residual rows
+  double* d_acbank = nullptr;    // [n_el][54] dR/dEU quotients
+  double* d_cbank = nullptr;     // [n_cidx][36] quotients of shared entries
   int32_t* d_cstart = nullptr;
   int32_t* d_contrib = nullptr;
   int32_t* d_estart = nullptr;
   int32_t* d_econtrib = nullptr;
-  double* d_bank = nullptr;   // n_el*54*56
   double* d_ebank = nullptr;  // n_edges*18*56
   double* d_arc = nullptr;    // per-field partial sums of the arclength dot product
   int arc_blocks = 0;
@@ -961,7 +1040,7 @@ struct vef_handle {
   // Optional kernel timing: event pairs around the element kernel and around
   // the scatter kernels of each assembly.
   bool profiling = false;
-  std::vector<hipEvent_t> ev_bulk, ev_scatter;  // start/stop pairs
+  std::vector<hipEvent_t> ev_bulk, ev_scatter, ev_other;  // start/stop pairs
 };
 
 namespace {
@@ -1023,31 +1102,32 @@ int upload_tables(vef_handle* h) {
 
 namespace {
 template <int MODEL, bool DCS, bool INERTIA>
-void launch_bulk_t(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
+void launch_bulk_t(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
+  FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank};
   if (jac)  // 8 workgroups per element, elements padded to a multiple of 8 (XCD-aware ids)
     hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>),
                        dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_coords, h->d_conn, x, eu,
-                       h->n_el, h->dprm, h->d_bank);
+                       h->n_el, h->dprm, (double*)nullptr, fo);
   else
     hipLaunchKernelGGL((vef_bulk_kernel<7, 1, 1, MODEL, DCS, INERTIA>), dim3((unsigned)(h->n_el + 6) / 7), dim3(64), 0,
-                       st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_bank);
+                       st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_r0bank, fo);
 }
 
 template <int MODEL, bool DCS>
-void launch_bulk_i(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
-  if (h->dprm.ReN != 0.0) launch_bulk_t<MODEL, DCS, true>(h, jac, x, eu, st);
-  else launch_bulk_t<MODEL, DCS, false>(h, jac, x, eu, st);
+void launch_bulk_i(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
+  if (h->dprm.ReN != 0.0) launch_bulk_t<MODEL, DCS, true>(h, jac, values, x, eu, st);
+  else launch_bulk_t<MODEL, DCS, false>(h, jac, values, x, eu, st);
 }
 
 template <int MODEL>
-void launch_bulk_d(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
-  if (h->dprm.dcs_on) launch_bulk_i<MODEL, true>(h, jac, x, eu, st);
-  else launch_bulk_i<MODEL, false>(h, jac, x, eu, st);
+void launch_bulk_d(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
+  if (h->dprm.dcs_on) launch_bulk_i<MODEL, true>(h, jac, values, x, eu, st);
+  else launch_bulk_i<MODEL, false>(h, jac, values, x, eu, st);
 }
 
-void launch_bulk(vef_handle* h, int jac, const double* x, int64_t eu, hipStream_t st) {
-  if (h->dprm.model == 0) launch_bulk_d<0>(h, jac, x, eu, st);
-  else launch_bulk_d<1>(h, jac, x, eu, st);
+void launch_bulk(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
+  if (h->dprm.model == 0) launch_bulk_d<0>(h, jac, values, x, eu, st);
+  else launch_bulk_d<1>(h, jac, values, x, eu, st);
 }
 }  // namespace
 
@@ -1069,12 +1149,13 @@ int vef_get_pattern(const vef_handle* h, int64_t* rowptr, int32_t* colind) {
 
 void vef_destroy(vef_handle* h) {
   if (!h) return;
-  for (hipEvent_t e : h->ev_bulk) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->ev_other) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_scatter) (void)hipEventDestroy(e);
-  void* ptrs[] = {h->d_coords, h->d_conn, h->d_edges, h->d_rowptr, h->d_col, h->d_pair_start, h->d_diag_pair,
-                  h->d_pair_row,
-                  h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, h->d_arc, h->d_tmp_x,
-                  h->d_tmp_b, h->d_tmp_v};
+  void* ptrs[] = {h->d_coords, h->d_conn,   h->d_edges,  h->d_rowptr,  h->d_col,     h->d_diag_pair,
+                  h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_code,    h->d_sbase,
+                  h->d_slen,   h->d_ustart,  h->d_ulist,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
+                  h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
+                  h->d_tmp_b,  h->d_tmp_v};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -1214,6 +1295,42 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
     for (int64_t c = 0; c < h->N; c++) h->h_col[p++] = (int32_t)c;
   }
 
+  // Direct-scatter codes, shared pairs and their contribution lists.
+  std::vector<int32_t> code((size_t)n_el * 81);
+  std::vector<int64_t> sbase;
+  std::vector<int32_t> slen, ustart(1, 0), ulist;
+  std::vector<int64_t> ep_pair, ebase;
+  std::vector<int32_t> elen;
+  for (int64_t pp = 0; pp < npairs; pp++) {
+    const int32_t r = pair_row[pp];
+    const int64_t pos = pp - pair_start[r];
+    const int64_t rb = h->h_rowptr[(int64_t)r * 6];
+    const int32_t rl = (int32_t)(h->h_rowptr[(int64_t)r * 6 + 1] - rb);
+    const int32_t c0 = cstart[pp], c1 = cstart[pp + 1];
+    if (c1 - c0 == 1) {
+      const int32_t w = contrib[c0];
+      code[(size_t)(w / 81) * 81 + w % 81] = (int32_t)(pos << 1);
+    } else {
+      sbase.push_back(rb + pos * 6);
+      slen.push_back(rl);
+      for (int32_t c = c0; c < c1; c++) {
+        const int32_t w = contrib[c];
+        const int32_t cidx = (int32_t)ulist.size();
+        code[(size_t)(w / 81) * 81 + w % 81] = (cidx << 1) | 1;
+        ulist.push_back(cidx);
+      }
+      ustart.push_back((int32_t)ulist.size());
+    }
+    if (estart[pp + 1] > estart[pp]) {
+      ep_pair.push_back(pp);
+      ebase.push_back(rb + pos * 6);
+      elen.push_back(rl);
+    }
+  }
+  h->n_shared = (int64_t)sbase.size();
+  h->n_cidx = (int64_t)ulist.size();
+  h->n_epairs = (int64_t)ep_pair.size();
+
   // Device copies.
 #define UP(dst, src, n)                         \
   do {                                          \
@@ -1229,14 +1346,22 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   UP(h->d_edges, edges, (size_t)n_edges * 2);
   UP(h->d_rowptr, h->h_rowptr.data(), (size_t)h->N + 1);
   UP(h->d_col, h->h_col.data(), (size_t)h->nnz);
-  UP(h->d_pair_start, pair_start.data(), (size_t)n_nodes + 1);
   UP(h->d_diag_pair, diag_pair.data(), (size_t)n_nodes);
-  UP(h->d_pair_row, pair_row.data(), pair_row.size());
   UP(h->d_cstart, cstart.data(), cstart.size());
   UP(h->d_contrib, contrib.data(), contrib.size());
   UP(h->d_estart, estart.data(), estart.size());
   UP(h->d_econtrib, econtrib.data(), econtrib.size());
-  UP(h->d_bank, (const double*)nullptr, (size_t)n_el * 54 * kNeval);
+  UP(h->d_code, code.data(), code.size());
+  UP(h->d_sbase, sbase.data(), sbase.size());
+  UP(h->d_slen, slen.data(), slen.size());
+  UP(h->d_ustart, ustart.data(), ustart.size());
+  UP(h->d_ulist, ulist.data(), ulist.size());
+  UP(h->d_ep_pair, ep_pair.data(), ep_pair.size());
+  UP(h->d_ebase, ebase.data(), ebase.size());
+  UP(h->d_elen, elen.data(), elen.size());
+  UP(h->d_r0bank, (const double*)nullptr, (size_t)n_el * 54);
+  UP(h->d_acbank, (const double*)nullptr, (size_t)n_el * 54);
+  UP(h->d_cbank, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
   UP(h->d_ebank, (const double*)nullptr, (size_t)std::max(n_edges, 1) * 18 * kNeval);
   h->arc_blocks = (int)((n_nodes + kArcChunk - 1) / kArcChunk);
   UP(h->d_arc, (const double*)nullptr, (size_t)6 * h->arc_blocks);
@@ -1263,40 +1388,56 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
     for (int i = 0; i < 4; i++) VEF_HIP(h, hipEventCreate(&ev[i]));
     VEF_HIP(h, hipEventRecord(ev[0], st));
   }
-  launch_bulk(h, jac, x, eu, st);
+  // 1. base residual rows of every element
+  launch_bulk(h, 0, values, x, eu, st);
   VEF_HIP(h, hipGetLastError());
-  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[1], st));
+  // 2. outlet edges (residual and FD bank)
   if (h->n_edges > 0) {
     hipLaunchKernelGGL(vef_edge_kernel, dim3(h->n_edges), dim3(192), 0, st, h->d_coords, h->d_conn, h->d_edges, x,
                        eu, h->n_edges, h->dprm, h->d_ebank);
     VEF_HIP(h, hipGetLastError());
   }
-  const int64_t node_rows = (int64_t)h->n_nodes * 6;
-  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[2], st));
+  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[1], st));
+  // 3. FD element Jacobians: single-contributor entries straight into values
   if (jac) {
-    hipLaunchKernelGGL(vef_gather_kernel, dim3((unsigned)((h->n_pairs * 6 + 255) / 256)), dim3(256), 0, st,
-                       h->n_pairs, h->d_rowptr, h->d_pair_start, h->d_pair_row, h->d_diag_pair, h->d_cstart,
-                       h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank, h->d_ebank, values, b);
+    launch_bulk(h, 1, values, x, eu, st);
     VEF_HIP(h, hipGetLastError());
+  }
+  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[2], st));
+  const int64_t node_rows = (int64_t)h->n_nodes * 6;
+  if (jac) {
+    // 4. shared entries in element order, 5. outlet-edge entries after them
+    if (h->n_shared > 0) {
+      hipLaunchKernelGGL(vef_shared_kernel, dim3((unsigned)((h->n_shared * 36 + 255) / 256)), dim3(256), 0, st,
+                         h->n_shared, h->d_sbase, h->d_slen, h->d_ustart, h->d_ulist, h->d_cbank, values);
+      VEF_HIP(h, hipGetLastError());
+    }
+    if (h->n_epairs > 0) {
+      hipLaunchKernelGGL(vef_edge_add_kernel, dim3((unsigned)((h->n_epairs * 12 + 255) / 256)), dim3(256), 0, st,
+                         h->n_epairs, h->d_ep_pair, h->d_ebase, h->d_elen, h->d_estart, h->d_econtrib, h->d_ebank,
+                         values);
+      VEF_HIP(h, hipGetLastError());
+    }
     // Row N-1 (arclength row) is not touched by the bulk assembly: zero.
     const int64_t last0 = h->h_rowptr[h->N - 1];
     hipLaunchKernelGGL(vef_fill_kernel, dim3(1024), dim3(256), 0, st, values + last0, h->nnz - last0);
     VEF_HIP(h, hipGetLastError());
   }
-  if (!jac) {
-    hipLaunchKernelGGL(vef_rhs_kernel, dim3((unsigned)((node_rows + 255) / 256)), dim3(256), 0, st, node_rows, jac,
-                       h->d_rowptr, h->d_diag_pair, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_bank,
-                       1, h->d_ebank, values, b);
-    VEF_HIP(h, hipGetLastError());
-  }
+  // 6. b and the dR/dEU column
+  hipLaunchKernelGGL(vef_rhs_kernel, dim3((unsigned)((node_rows + 255) / 256)), dim3(256), 0, st, node_rows, jac,
+                     h->d_rowptr, h->d_diag_pair, h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib,
+                     h->d_r0bank, h->d_acbank, h->d_ebank, values, b);
+  VEF_HIP(h, hipGetLastError());
   hipLaunchKernelGGL(vef_fill_kernel, dim3(1), dim3(64), 0, st, b + (h->N - 1), (int64_t)1);
   VEF_HIP(h, hipGetLastError());
   if (h->profiling) {
     VEF_HIP(h, hipEventRecord(ev[3], st));
-    h->ev_bulk.push_back(ev[0]);
     h->ev_bulk.push_back(ev[1]);
+    h->ev_bulk.push_back(ev[2]);
     h->ev_scatter.push_back(ev[2]);
     h->ev_scatter.push_back(ev[3]);
+    h->ev_other.push_back(ev[0]);
+    h->ev_other.push_back(ev[1]);
   }
   h->last_flag = flag;
   return VEF_OK;
@@ -1304,10 +1445,13 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
 
 namespace {
 void release_events(vef_handle* h) {
-  for (hipEvent_t e : h->ev_bulk) (void)hipEventDestroy(e);
+  // ev_other holds each assembly's first two events, ev_scatter its last two;
+  // ev_bulk repeats events of both.
+  for (hipEvent_t e : h->ev_other) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_scatter) (void)hipEventDestroy(e);
   h->ev_bulk.clear();
   h->ev_scatter.clear();
+  h->ev_other.clear();
 }
 }  // namespace
 
@@ -1387,15 +1531,6 @@ int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, d
   VEF_HIP(h, hipDeviceSynchronize());
   VEF_HIP(h, hipMemcpy(b, h->d_tmp_b, sizeof(double) * h->N, hipMemcpyDeviceToHost));
   if (flag == VEF_RES_JAC) VEF_HIP(h, hipMemcpy(values, h->d_tmp_v, sizeof(double) * h->nnz, hipMemcpyDeviceToHost));
-  return VEF_OK;
-}
-
-int vef_copy_residual_bank(const vef_handle* hc, double* out) {
-  vef_handle* h = const_cast<vef_handle*>(hc);
-  if (!h || !out) return VEF_ERR_ARG;
-  if (h->last_flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "no Jacobian assembly has run");
-  VEF_HIP(h, hipDeviceSynchronize());
-  VEF_HIP(h, hipMemcpy(out, h->d_bank, sizeof(double) * (size_t)h->n_el * 54 * kNeval, hipMemcpyDeviceToHost));
   return VEF_OK;
 }
 
