@@ -115,14 +115,17 @@ def test_bitwise_all_exact_divisions(oracle):
 
 
 def test_bitwise_some_exact_divisions(oracle):
-    """An indefinite S (det S < 0) on a few nodes sends only the wavefronts of
-    the elements around them through the IEEE-division pass."""
+    """An indefinite S (det S < 0) on a few nodes, and S entries of size 2^70
+    (outside the range that makes the test-node divisions safe) on others,
+    send only the wavefronts of the elements around them through the
+    IEEE-division pass."""
     mesh = cylinder_channel(4, 2, 3, 4)
     x = smooth_state(mesh, EU=0.8)
     u = x[:-1].reshape(-1, 6)
     u[mesh.conn[[3, 17, 40], 8], 3] *= -1.0
+    u[mesh.conn[[9, 31], 8], 4] *= 2.0 ** 70
     x[:-1] = u.ravel()
-    check_bitwise(oracle, mesh, PARAM_SETS["giesekus"], x, "indefinite S on 3 nodes")
+    check_bitwise(oracle, mesh, PARAM_SETS["giesekus"], x, "indefinite S on 3 nodes, 2^70 S01 on 2")
 
 
 def test_deterministic_repeats():

@@ -2,25 +2,30 @@
 // assembly for the 2D square-root-conformation viscoelastic cylinder problem,
 // behind the C ABI of include/vef.h.
 //
-// What runs where
-//   vef_bulk_kernel<1,7,56> eight single-wavefront workgroups per element, each
-//                          63 lanes = 9 Gauss points x 7 of the 56 evaluations
-//                          (base state, the 54 nodal perturbations, the EU
-//                          perturbation).  Each lane evaluates the Gauss-point
-//                          strong form and the 9 test-node residual rows at its
-//                          state; an ordered LDS reduction over the Gauss
-//                          points gives the 54 element residual rows of every
-//                          evaluation, written as the bank R[e][row][k].
-//   vef_bulk_kernel<7,1,1> residual only: one wavefront per 7 elements.
-//   vef_edge_kernel        outlet-edge traction (openBC) bank, 3 edge Gauss
-//                          points x 56 evaluations per outlet edge.
-//   vef_gather_kernel      CSR values: each matrix slot sums the
-//                          forward-difference quotients of its contributing
-//                          elements in ascending element order, then outlet
-//                          edges in ascending edge order — the order of
-//                          PETSc's ADD_VALUES calls in the reference — so the
-//                          result is deterministic and bitwise reproducible.
-//   vef_rhs_kernel         residual vector b in the same order.
+// What runs where (one assembly, in stream order)
+//   vef_bulk_kernel<7,1,1>  residual pass: one wavefront per 7 elements, lane =
+//                           (Gauss point, element); element residual rows R_0
+//                           into r0bank[e][54].
+//   vef_edge_kernel         outlet-edge traction (openBC), 3 edge Gauss points
+//                           x 56 evaluations per outlet edge, into ebank.
+//   vef_bulk_kernel<1,7,56> Jacobian pass: eight single-wavefront workgroups
+//                           per element, 63 lanes = 9 Gauss points x 7 of the
+//                           54 nodal perturbations and the EU perturbation.
+//                           Each lane evaluates the Gauss-point strong form and
+//                           the 9 test-node rows; an ordered LDS reduction over
+//                           the Gauss points gives each evaluation's 54 rows,
+//                           and the quotients (R_k - R_0)/h go straight to the
+//                           CSR values when the node pair belongs to this
+//                           element only, else to cbank, and dR/dEU to acbank.
+//   vef_shared_kernel       values of node pairs shared by several elements:
+//                           their cbank entries summed in ascending element
+//                           order (the order of the reference's ADD_VALUES).
+//   vef_edge_add_kernel     outlet-edge quotients added after the bulk ones,
+//                           in ascending edge order.
+//   vef_rhs_kernel          b and the EU column: per node row, elements then
+//                           edges in ascending order.
+// Every sum starts from +0 as ADD_VALUES into zeroed storage does, so even
+// signed zeros match.
 //
 // The arithmetic of each lane restates src/Element_Calculations_mod.F90,
 // src/Constitutive_Models_SQRT_mod.F90 and src/Residuals_mod.F90 in 2D form:
@@ -159,6 +164,7 @@ struct Geo {
   double WET, H2;
   double H4, yH4;     // H2*H2 and RN(1/(H2*H2)) (denominator of the complex division by H2)
   double r3, t7;      // BvN/H2 and DfN/H2
+  int in_range;       // every dFx, dFy is 0 or has magnitude in [2^-64, 2^64)
 };
 
 // Complex division with zero imaginary parts: re((a+0i)/(c+0i)) = (a*c)/(c*c).
@@ -186,10 +192,21 @@ struct Recip {
   double d, y;
 };
 struct DivCheck {
-  int lo, hi;
-  __device__ __forceinline__ bool failed() const { return lo < -900 || hi > 900; }
+  int lo, hi;    // binades of the tracked quotients q0: must stay in [-900, 900]
+  int blo, bhi;  // binades of the base quantities: must stay in [-63, 64]
+  __device__ __forceinline__ bool failed() const { return lo < -900 || hi > 900 || blo < -63 || bhi > 64; }
 };
 constexpr int kBadExp = 4096;
+
+// Base quantities of the untracked divisions: x = 0 or 2^-64 <= |x| < 2^64.
+template <bool EXACT>
+__device__ __forceinline__ void base_range(double x, DivCheck& dc) {
+  if constexpr (!EXACT) {
+    const int ex = __builtin_amdgcn_frexp_exp(x);
+    dc.blo = min(dc.blo, ex);
+    dc.bhi = max(dc.bhi, ex);
+  }
+}
 
 template <bool EXACT>
 __device__ __forceinline__ void check_denominator(double d, DivCheck& dc) {
@@ -217,6 +234,20 @@ __device__ __forceinline__ double qdiv(double a, const Recip& r, DivCheck& dc) {
   }
 }
 
+// Fast division whose window is guaranteed by the ranges of its inputs rather
+// than tracked (the test-node divisions of node_terms; the bound is derived
+// there).
+template <bool EXACT>
+__device__ __forceinline__ double qdiv_ranged(double a, const Recip& r) {
+  if constexpr (EXACT) {
+    return a / r.d;
+  } else {
+    const double q0 = a * r.y;
+    const double e = fma(r.d, q0, -a);
+    return fma(-e, r.y, q0);
+  }
+}
+
 template <bool EXACT>
 __device__ __forceinline__ double fd_quotient(double rk, double r0, DivCheck& dc) {
   // (dRes - Res)/eps_jac with a complex numerator: ((dR-R)*h)/(h*h)
@@ -224,7 +255,7 @@ __device__ __forceinline__ double fd_quotient(double rk, double r0, DivCheck& dc
   return qdiv<EXACT>((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh}, dc);
 }
 __device__ __forceinline__ double fd_quotient_ieee(double rk, double r0) {
-  DivCheck dc{0, 0};
+  DivCheck dc{0, 0, 0, 0};
   return fd_quotient<true>(rk, r0, dc);
 }
 
@@ -261,6 +292,13 @@ __device__ void geometry(const double* X /*[9][2]*/, const double* dc, const dou
   G.WET = wgt * fabs(detJ);
   G.H4 = G.H2 * G.H2;
   G.yH4 = 1.0 / G.H4;
+  int lo = 0, hi = 0;
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    lo = min(lo, min(__builtin_amdgcn_frexp_exp(G.dFx[n]), __builtin_amdgcn_frexp_exp(G.dFy[n])));
+    hi = max(hi, max(__builtin_amdgcn_frexp_exp(G.dFx[n]), __builtin_amdgcn_frexp_exp(G.dFy[n])));
+  }
+  G.in_range = lo >= -63 && hi <= 64;
 }
 
 // Gauss-point quantities that the test-node terms read (GaussPointQuantities,
@@ -462,6 +500,14 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   const double t7 = G.t7;
   sum = sum + t7 * t7;
   o.tlsce = cdiv(1.0, sqrt(sum));
+  // Inputs of the test-node divisions (node_terms), range-checked once here.
+  base_range<EXACT>(S00, dc);
+  base_range<EXACT>(S01, dc);
+  base_range<EXACT>(S11, dc);
+  base_range<EXACT>(o.tlsce, dc);
+  base_range<EXACT>(WiN, dc);
+  base_range<EXACT>(prm.omb, dc);
+  if (!EXACT && !G.in_range) dc.hi = kBadExp;
 
   o.Ux = Ux; o.Uy = Uy;
   o.CE00 = CE00; o.CE01 = CE01; o.CE10 = CE10; o.CE11 = CE11;
@@ -475,9 +521,23 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
 // Test-node rows of one Gauss point (momentumMainSQRT, continuityBulkMain,
 // stressesBulkMainSQRT and their LSME/LSIC/LSCE/DCS terms, Residuals_mod.F90:
 // 90-110,161-234,295-490), already multiplied by WET.
+//
+// Why the divisions here need no tracking.  Write Z(a, b) for {0} together
+// with the magnitudes in [2^a, 2^b].  In the fast pass gauss_point has checked
+// S00, S01, S11, tlsce, WiN, omb and (through Geo::in_range) fx, fy to lie in
+// Z(-64, 64), and make_recip has checked rW and rW2 to lie in [2^-60, 2^40],
+// so y = RN(1/d) lies in [2^-41, 2^61].  A rounded product of values in
+// Z(a, b) and Z(c, d) lies in Z(a+c, b+d); a rounded sum or difference of
+// values in Z(a, b) lies in Z(a-52, b+1) (both operands are multiples of
+// 2^(a-52)).  Hence: the omega numerators and m00, m10 lie in Z(-180, 129),
+// their q0 in Z(-221, 190) and w in Z(-220, 189); the u's (-m - w*S or
+// -(w*S)) lie in Z(-336, 254); the l numerators ((tlsce*u)*omb)*WiN in
+// Z(-528, 446) and their q0 in Z(-569, 507).  Every q0 is inside the window
+// [2^-901, 2^900) of the two-FMA division, and nothing leaves the normal
+// range, so each quotient is the IEEE one.
 template <int MODEL, bool DCS, bool INERTIA, bool EXACT>
 __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, double fy, const DevParams& prm,
-                                           double WET, double* out6, DivCheck& dc) {
+                                           double WET, double* out6) {
   const double S10 = g.S01;
   const double Udf = g.Ux * fx + g.Uy * fy;
   const double lsme_m = INERTIA ? g.tlsme * prm.ReN * Udf : 0.0;
@@ -485,14 +545,14 @@ __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, 
   {
     // i = 1: GUT_lsce row 1 = grad(phi)
     const double lsic = (g.tlsic * fx) * g.Cont;
-    const double w = qdiv<EXACT>(g.S01 * fx + g.S11 * fy, g.rW, dc);  // omegaNominator2D / omegaDenominator2D
+    const double w = qdiv_ranged<EXACT>(g.S01 * fx + g.S11 * fy, g.rW);  // omegaNominator2D / omegaDenominator2D
     const double m00 = g.S00 * fx + g.S01 * fy, m10 = S10 * fx + g.S11 * fy;
     const double u00 = -m00 - w * S10, u01 = -(w * g.S11);
     const double u10 = -m10 - (-w) * g.S00, u11 = -((-w) * g.S01);
-    const double l00 = qdiv<EXACT>((g.tlsce * u00) * prm.omb * g.WiN, g.rW2, dc);
-    const double l01 = qdiv<EXACT>((g.tlsce * u01) * prm.omb * g.WiN, g.rW2, dc);
-    const double l10 = qdiv<EXACT>((g.tlsce * u10) * prm.omb * g.WiN, g.rW2, dc);
-    const double l11 = qdiv<EXACT>((g.tlsce * u11) * prm.omb * g.WiN, g.rW2, dc);
+    const double l00 = qdiv_ranged<EXACT>((g.tlsce * u00) * prm.omb * g.WiN, g.rW2);
+    const double l01 = qdiv_ranged<EXACT>((g.tlsce * u01) * prm.omb * g.WiN, g.rW2);
+    const double l10 = qdiv_ranged<EXACT>((g.tlsce * u10) * prm.omb * g.WiN, g.rW2);
+    const double l11 = qdiv_ranged<EXACT>((g.tlsce * u11) * prm.omb * g.WiN, g.rW2);
     const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
     if constexpr (INERTIA) T0 = g.RC0 * bifn + (g.P00 * fx + g.P01 * fy) + lsme_m * g.Mom0 + lsic + lsce;
     else T0 = (g.P00 * fx + g.P01 * fy) + lsic + lsce;
@@ -500,14 +560,14 @@ __device__ __forceinline__ void node_terms(const GP& g, double bifn, double fx, 
   {
     // i = 2: GUT_lsce row 2 = grad(phi)
     const double lsic = (g.tlsic * fy) * g.Cont;
-    const double w = qdiv<EXACT>(-(g.S00 * fx) + g.S01 * (-fy), g.rW, dc);
+    const double w = qdiv_ranged<EXACT>(-(g.S00 * fx) + g.S01 * (-fy), g.rW);
     const double m01 = g.S00 * fx + g.S01 * fy, m11 = S10 * fx + g.S11 * fy;
     const double u00 = -(w * S10), u01 = -m01 - w * g.S11;
     const double u10 = -((-w) * g.S00), u11 = -m11 - (-w) * g.S01;
-    const double l00 = qdiv<EXACT>((g.tlsce * u00) * prm.omb * g.WiN, g.rW2, dc);
-    const double l01 = qdiv<EXACT>((g.tlsce * u01) * prm.omb * g.WiN, g.rW2, dc);
-    const double l10 = qdiv<EXACT>((g.tlsce * u10) * prm.omb * g.WiN, g.rW2, dc);
-    const double l11 = qdiv<EXACT>((g.tlsce * u11) * prm.omb * g.WiN, g.rW2, dc);
+    const double l00 = qdiv_ranged<EXACT>((g.tlsce * u00) * prm.omb * g.WiN, g.rW2);
+    const double l01 = qdiv_ranged<EXACT>((g.tlsce * u01) * prm.omb * g.WiN, g.rW2);
+    const double l10 = qdiv_ranged<EXACT>((g.tlsce * u10) * prm.omb * g.WiN, g.rW2);
+    const double l11 = qdiv_ranged<EXACT>((g.tlsce * u11) * prm.omb * g.WiN, g.rW2);
     const double lsce = l00 * g.CE00 + l01 * g.CE10 + l10 * g.CE01 + l11 * g.CE11;
     if constexpr (INERTIA) T1 = g.RC1 * bifn + (g.P10 * fx + g.P11 * fy) + lsme_m * g.Mom1 + lsic + lsce;
     else T1 = (g.P10 * fx + g.P11 * fy) + lsic + lsce;
@@ -666,7 +726,7 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   // workgroup (the branch is uniform: one wavefront per workgroup).
   auto evaluate = [&](auto exact_tag) -> bool {
     constexpr bool EXACT = decltype(exact_tag)::value;
-    DivCheck dc{0, 0};
+    DivCheck dc{0, 0, 0, 0};
     GP q;
     if (valid) {
       const double EUk = (NEVAL > 1 && k == NEVAL - 1) ? EU + kEpsJac : EU;
@@ -680,7 +740,7 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
       if (valid) {
         double r[6];
         node_terms<MODEL, DCS, INERTIA, EXACT>(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET,
-                                               r, dc);
+                                               r);
 #pragma unroll
         for (int c = 0; c < 6; c++) sRed[g][s][c] = r[c];
       }
@@ -705,7 +765,7 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
               const int in = (kk - 1) / 6, iq = (kk - 1) % 6;
               const int32_t code = sCode[n * 9 + in];
               if ((code & 1) == 0)
-                fo.values[sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 1) * 6 + iq] = qv;
+                fo.values[sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 1) * 6 + iq] = 0.0 + qv;
               else
                 fo.cbank[(int64_t)(code >> 1) * 36 + c * 6 + iq] = qv;
             }
@@ -842,26 +902,32 @@ vef_edge_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
 
 // ----------------------------------------------------------------------------
 // Scatter (storeResidual / storeJacobian / storeJacobian_Ac with ADD_VALUES,
-// Storage_mod.F90:10-115) as a deterministic gather.  Node pair (r, c) of the
-// pattern has the list of bulk contributions (element e, local index jn of r,
-// local index in of c), ascending in e, packed as e*81 + jn*9 + in, then the
-// outlet-edge contributions packed as ed*81 + jn*9 + in, ascending in ed.
-// One wavefront per matrix row; lane l handles row positions l, l+64, ...
+// Storage_mod.F90:10-115) as deterministic ordered sums.  Node pair (r, c) of
+// the pattern has the list of bulk contributions (element e, local index jn
+// of r, local index in of c), ascending in e, packed as e*81 + jn*9 + in, then
+// the outlet-edge contributions packed as ed*81 + jn*9 + in, ascending in ed.
 // ----------------------------------------------------------------------------
 
 // Entries of node pairs shared by several elements: thread (shared pair su,
 // jq, iq) sums the contributions of its elements in ascending element order.
+// The contributions of pair su are cbank rows ustart[su] .. ustart[su+1]-1,
+// already in that order; the first four loads are issued together.
 __global__ void __launch_bounds__(256)
 vef_shared_kernel(int64_t n_shared, const int64_t* __restrict__ sbase, const int32_t* __restrict__ slen,
-                  const int32_t* __restrict__ ustart, const int32_t* __restrict__ ulist,
-                  const double* __restrict__ cbank, double* __restrict__ values) {
+                  const int32_t* __restrict__ ustart, const double* __restrict__ cbank, double* __restrict__ values) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_shared * 36) return;
   const int64_t su = t / 36;
   const int jq = (int)((t % 36) / 6), iq = (int)(t % 6);
   const int32_t u0 = ustart[su], u1 = ustart[su + 1];
-  double acc = cbank[(int64_t)ulist[u0] * 36 + jq * 6 + iq];
-  for (int32_t u = u0 + 1; u < u1; u++) acc = acc + cbank[(int64_t)ulist[u] * 36 + jq * 6 + iq];
+  const double* c = cbank + (int64_t)u0 * 36 + jq * 6 + iq;
+  const int32_t cnt = u1 - u0;  // >= 2
+  const double c0 = c[0], c1 = c[36];
+  const double c2 = cnt > 2 ? c[72] : 0.0, c3 = cnt > 3 ? c[108] : 0.0;
+  double acc = (0.0 + c0) + c1;
+  if (cnt > 2) acc = acc + c2;
+  if (cnt > 3) acc = acc + c3;
+  for (int32_t u = 4; u < cnt; u++) acc = acc + c[(int64_t)u * 36];
   values[sbase[su] + (int64_t)jq * slen[su] + iq] = acc;
 }
 
@@ -907,8 +973,8 @@ vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr,
   for (int32_t c = c0; c < cstart[pp + 1]; c++) {
     const int32_t w = contrib[c];
     const int64_t idx = (w / 81) * 54 + ((w / 9) % 9) * 6 + jq;
-    rb = (c == c0) ? r0bank[idx] : rb + r0bank[idx];
-    if (jac) ra = (c == c0) ? acbank[idx] : ra + acbank[idx];
+    rb = rb + r0bank[idx];
+    if (jac) ra = ra + acbank[idx];
   }
   if (jq < 2) {
     for (int32_t c = estart[pp]; c < estart[pp + 1]; c++) {
@@ -1015,8 +1081,7 @@ struct vef_handle {
   int32_t* d_code = nullptr;     // [n_el][81]
   int64_t* d_sbase = nullptr;    // shared pairs: first slot
   int32_t* d_slen = nullptr;     //   row length
-  int32_t* d_ustart = nullptr;   //   contribution list
-  int32_t* d_ulist = nullptr;
+  int32_t* d_ustart = nullptr;   //   first cbank row of each shared pair
   int64_t n_shared = 0, n_cidx = 0;
   int64_t* d_ep_pair = nullptr;  // pairs with outlet-edge contributions
   int64_t* d_ebase = nullptr;
@@ -1153,7 +1218,7 @@ void vef_destroy(vef_handle* h) {
   for (hipEvent_t e : h->ev_scatter) (void)hipEventDestroy(e);
   void* ptrs[] = {h->d_coords, h->d_conn,   h->d_edges,  h->d_rowptr,  h->d_col,     h->d_diag_pair,
                   h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_code,    h->d_sbase,
-                  h->d_slen,   h->d_ustart,  h->d_ulist,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
+                  h->d_slen,   h->d_ustart,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
                   h->d_tmp_b,  h->d_tmp_v};
   for (void* p : ptrs)
@@ -1298,7 +1363,8 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   // Direct-scatter codes, shared pairs and their contribution lists.
   std::vector<int32_t> code((size_t)n_el * 81);
   std::vector<int64_t> sbase;
-  std::vector<int32_t> slen, ustart(1, 0), ulist;
+  std::vector<int32_t> slen, ustart(1, 0);
+  int64_t n_cidx = 0;
   std::vector<int64_t> ep_pair, ebase;
   std::vector<int32_t> elen;
   for (int64_t pp = 0; pp < npairs; pp++) {
@@ -1315,11 +1381,10 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
       slen.push_back(rl);
       for (int32_t c = c0; c < c1; c++) {
         const int32_t w = contrib[c];
-        const int32_t cidx = (int32_t)ulist.size();
-        code[(size_t)(w / 81) * 81 + w % 81] = (cidx << 1) | 1;
-        ulist.push_back(cidx);
+        code[(size_t)(w / 81) * 81 + w % 81] = (int32_t)((n_cidx << 1) | 1);
+        n_cidx++;
       }
-      ustart.push_back((int32_t)ulist.size());
+      ustart.push_back((int32_t)n_cidx);
     }
     if (estart[pp + 1] > estart[pp]) {
       ep_pair.push_back(pp);
@@ -1328,7 +1393,7 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
     }
   }
   h->n_shared = (int64_t)sbase.size();
-  h->n_cidx = (int64_t)ulist.size();
+  h->n_cidx = n_cidx;
   h->n_epairs = (int64_t)ep_pair.size();
 
   // Device copies.
@@ -1355,7 +1420,6 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   UP(h->d_sbase, sbase.data(), sbase.size());
   UP(h->d_slen, slen.data(), slen.size());
   UP(h->d_ustart, ustart.data(), ustart.size());
-  UP(h->d_ulist, ulist.data(), ulist.size());
   UP(h->d_ep_pair, ep_pair.data(), ep_pair.size());
   UP(h->d_ebase, ebase.data(), ebase.size());
   UP(h->d_elen, elen.data(), elen.size());
@@ -1409,7 +1473,7 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
     // 4. shared entries in element order, 5. outlet-edge entries after them
     if (h->n_shared > 0) {
       hipLaunchKernelGGL(vef_shared_kernel, dim3((unsigned)((h->n_shared * 36 + 255) / 256)), dim3(256), 0, st,
-                         h->n_shared, h->d_sbase, h->d_slen, h->d_ustart, h->d_ulist, h->d_cbank, values);
+                         h->n_shared, h->d_sbase, h->d_slen, h->d_ustart, h->d_cbank, values);
       VEF_HIP(h, hipGetLastError());
     }
     if (h->n_epairs > 0) {
