@@ -8,7 +8,7 @@ traction, and the deterministic scatter into the fixed CSR pattern.  Inputs
 (mesh, pattern, state) are resident in HBM before the timed region.
 
 Workload: channel x in [-15, 15], y in [-2, 2], cylinder R = 1, 100,000 9-node
-elements (400,848 nodes, N = 2,411,089 unknowns); Oldroyd-B (L-PTT with
+elements (401,848 nodes, N = 2,411,089 unknowns); Oldroyd-B (L-PTT with
 VeN = 0), Wi = WiN0*EU = 0.5, beta = 0.59, Re = 0, DfN = 1e-3; state = smooth
 divergence-free flow + 8-mode Fourier perturbation (amplitude 1e-2, seed
 20261203) on every field.
