@@ -119,6 +119,28 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
                   int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
                   void* hip_stream);
 
+/* Drag on a boundary, the Fd the reference prints after each converged step
+ * (calculateEdgeQuantities with ieq_ed = 1, src/Post_Process_mod.F90:1796-1890;
+ * the edges of the Cylinder boundary from setBoundaryEdges,
+ * src/Boundary_mod.F90:477-522).  vef_set_drag_edges stores the boundary's
+ * edges (host int32[n][2]: element 0-based, edge id 1..4), in the order the
+ * reference lists them (ascending element, then edge id).  vef_drag computes
+ *   Fd = sum over the edges of term_el,
+ *   term_el = sum over the 3 edge Gauss points of dot(-normal, P_tot(:,1))*WET,
+ * both sums sequential, with P_tot at WiN = WiN0*x[N-1].  Fd: device double[1];
+ * per_edge: device double[n] for the term_el values, or NULL. */
+int vef_set_drag_edges(vef_handle* h, int32_t n_edges, const int32_t* edges);
+int vef_drag(vef_handle* h, const double* x, double* Fd, double* per_edge, void* hip_stream);
+
+/* Nodal stress tensor of the square-root formulation at every node
+ * (nodalStressTensorSQRT, src/Post_Process_mod.F90:2176-2236, the wall-stress
+ * output of calculateNodalQuantities, :1992-2168):
+ *   T = ((1-BvN)/WiN)*(C - I), C = S.S,
+ * for the L-PTT and Giesekus model names (g1 = g2 = 1 there).  WiN is the value
+ * of the reference's module variable at post-processing time, passed by the
+ * caller.  T: device double[n_nodes][4] = (Txx, Txy, Tyy, Tzz). */
+int vef_nodal_stress(vef_handle* h, const double* x, double WiN, double* T, void* hip_stream);
+
 /* The same with host arrays (copies in, assembles, copies out). */
 int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values);
 

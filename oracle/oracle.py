@@ -53,6 +53,9 @@ def lib(flops: bool = False):
                                      vp, vp, vp]
             L.vo_arclength.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp]
+            L.vo_drag.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int,
+                                  vp, vp, vp]
+            L.vo_nodal_stress.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp]
             L.vo_pattern_nnz.argtypes = [ctypes.c_int, ctypes.c_int, vp]
             L.vo_pattern_nnz.restype = ctypes.c_int64
             L.vo_pattern.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp]
@@ -171,3 +174,25 @@ def arclength(mode, n_nodes, x, xo, t, increment, LN0, dArN, dSo, rowptr, col, v
     x, xo, t = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, xo, t))
     lib().vo_arclength(mode, n_nodes, _p(x), _p(xo), _p(t), increment, LN0, dArN, dSo, 1 if jac else 0, _p(rowptr),
                        _p(col), _p(vals), _p(b))
+
+
+def drag(prm, mesh, x, edges):
+    """Drag term over the given boundary edges (element, edge id 1..4), in the
+    reference's edge order; returns (Fd, per-edge terms)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    edges = np.ascontiguousarray(edges, dtype=np.int32).reshape(-1, 2)
+    Fd = np.zeros(1)
+    per = np.zeros(max(len(edges), 1))
+    rc = lib().vo_drag(ctypes.byref(prm), mesh.n_nodes, mesh.n_elements, _p(np.ascontiguousarray(mesh.coords)),
+                       _p(np.ascontiguousarray(mesh.conn)), _p(x), len(edges), _p(edges), _p(Fd), _p(per))
+    if rc != 0:
+        raise ValueError("vo_drag failed")
+    return float(Fd[0]), per[:len(edges)]
+
+
+def nodal_stress(BvN, WiN, n_nodes, x):
+    """[n_nodes][4] = T(1,1), T(1,2), T(2,2), T(3,3) of the SQRT formulation."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((n_nodes, 4))
+    lib().vo_nodal_stress(float(BvN), float(WiN), n_nodes, _p(x), _p(out))
+    return out

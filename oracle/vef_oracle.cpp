@@ -1124,6 +1124,94 @@ int vo_arclength(int mode, int n_nodes, const double* x, const double* xo, const
   return 0;
 }
 
+// f4: drag on a boundary (calculateEdgeQuantities with ieq_ed = 1,
+// src/Post_Process_mod.F90:1796-1890): term = sum over the boundary's edges,
+// in (element, edge) order, of term_el = sum over the 3 edge Gauss points of
+// dot_product(-normal, P_tot(:,1))*WET, with the Gauss-point quantities of
+// setFlowQuantities and setEdgeQuantities as in the outlet residual.
+// per_edge (optional) receives each term_el.
+int vo_drag(const vo_params* p, int n_nodes, int n_el, const double* coords, const int32_t* conn, const double* x,
+            int n_edges, const int32_t* edges, double* Fd, double* per_edge) {
+  (void)n_el;
+  Phys ph = make_phys(p);
+  const Tables& T = tables();
+  vreal I[3][3];
+  identity(I);
+  const double EU = x[(size_t)n_nodes * 6];
+  vreal term = Z();
+  for (int k = 0; k < n_edges; k++) {
+    const int e = edges[2 * k], ied = edges[2 * k + 1];
+    if (ied < 1 || ied > 4) return -1;
+    Elem E;
+    copyToLocal(E, coords, conn, e, x, EU);
+    const int ed = ied - 1;
+    vreal term_el = Z();
+    for (int g = 0; g < 3; g++) {
+      E.bfn = T.bfnE[ed][g];
+      E.dfdc = T.dfdcE[ed][g];
+      E.dfde = T.dfdeE[ed][g];
+      E.d2fdc2 = T.d2fdc2E[ed][g];
+      E.d2fde2 = T.d2fde2E[ed][g];
+      E.d2fdce = T.d2fdceE[ed][g];
+      GsPt G;
+      basis2D(E, G);
+      setFlowQuantities(ph, E, G, I);
+      vreal dxdc = G.dXidci[0][0], dydc = G.dXidci[0][1];
+      vreal dxde = G.dXidci[1][0], dyde = G.dXidci[1][1];
+      G.normal[2] = Z();
+      switch (ied) {
+        case 1: G.normal[0] = dydc; G.normal[1] = -dxdc; break;
+        case 2: G.normal[0] = dyde; G.normal[1] = -dxde; break;
+        case 3: G.normal[0] = -dydc; G.normal[1] = dxdc; break;
+        default: G.normal[0] = -dyde; G.normal[1] = dxde; break;
+      }
+      G.dL = vsqrt(G.normal[0] * G.normal[0] + G.normal[1] * G.normal[1]);
+      G.normal[0] = G.normal[0] / G.dL;
+      G.normal[1] = G.normal[1] / G.dL;
+      G.WET = vreal(T.W1[g]) * G.dL;
+      vreal mn[3] = {-G.normal[0], -G.normal[1], -G.normal[2]};
+      vreal col[3] = {G.P_tot[0][0], G.P_tot[1][0], G.P_tot[2][0]};
+      term_el = term_el + dot3(mn, col) * G.WET;
+    }
+    if (per_edge) per_edge[k] = val(term_el);
+    term = term + term_el;
+  }
+  *Fd = val(term);
+  return 0;
+}
+
+// f4: nodal stress tensor of the SQRT formulation (nodalStressTensorSQRT,
+// src/Post_Process_mod.F90:2176-2236) for the model names whose g1 = g2 = 1
+// (every name but m-L-PTT, m-e-PTT and the FENE models, so L-PTT and
+// GIESEKUS): S from TL(4:6) (getStressComponent, Tools_mod.F90:769-793),
+// C = matmul(S,S), T = ((1-BvN)/WiN)*(g2*C - g1*I).  WiN is the module value
+// the caller holds.  out[g][4] = T(1,1), T(1,2), T(2,2), T(3,3).
+int vo_nodal_stress(double BvN, double WiN, int n_nodes, const double* x, double* out) {
+  for (int g = 0; g < n_nodes; g++) {
+    double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    S[0][0] = x[(size_t)g * 6 + 3];
+    S[0][1] = x[(size_t)g * 6 + 4];
+    S[1][0] = S[0][1];
+    S[1][1] = x[(size_t)g * 6 + 5];
+    double C[3][3], Tt[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++) s = s + S[i][k] * S[k][j];
+        C[i][j] = s;
+      }
+    const double g1 = 1.0, g2 = 1.0;
+    const double a = (1.0 - BvN) / WiN;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) Tt[i][j] = a * (g2 * C[i][j] - g1 * (i == j ? 1.0 : 0.0));
+    out[(size_t)g * 4 + 0] = Tt[0][0];
+    out[(size_t)g * 4 + 1] = Tt[0][1];
+    out[(size_t)g * 4 + 2] = Tt[1][1];
+    out[(size_t)g * 4 + 3] = Tt[2][2];
+  }
+  return 0;
+}
+
 #else  // VEF_COUNT_FLOPS
 
 // Operation counts of one element-residual evaluation (bulk or edge) at a given

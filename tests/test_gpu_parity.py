@@ -128,6 +128,17 @@ def test_bitwise_some_exact_divisions(oracle):
     check_bitwise(oracle, mesh, PARAM_SETS["giesekus"], x, "indefinite S on 3 nodes, 2^70 S01 on 2")
 
 
+@pytest.mark.parametrize("turns", [1, 2, 3])
+def test_bitwise_rotated_local_numbering(oracle, turns):
+    """Every element's local numbering turned by a quarter turn or more, so the
+    outlet edges carry edge ids 1, 3 and 4 instead of 2."""
+    m = cylinder_channel(4, 2, 3, 4)
+    perm = [(i + turns) % 4 for i in range(4)] + [4 + (i + turns) % 4 for i in range(4)] + [8]
+    mesh = finish_mesh(m.coords, m.conn[:, perm], m.geometry)
+    assert set(mesh.outlet_edges[:, 1].tolist()) == {(2 - turns - 1) % 4 + 1}
+    check_bitwise(oracle, mesh, PARAM_SETS["inertia"], smooth_state(mesh, EU=0.8), f"rotated by {turns}")
+
+
 def test_deterministic_repeats():
     mesh = cylinder_channel(4, 2, 3, 4)
     x = torch.from_numpy(smooth_state(mesh, EU=0.6)).cuda()

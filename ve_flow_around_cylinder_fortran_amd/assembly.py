@@ -104,6 +104,12 @@ def load_library():
     lib.vef_arclength.argtypes = [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, c_i32, c_vp, c_vp, c_vp]
     lib.vef_arclength.restype = ctypes.c_int
+    lib.vef_set_drag_edges.argtypes = [c_vp, c_i32, c_ip]
+    lib.vef_set_drag_edges.restype = ctypes.c_int
+    lib.vef_drag.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp]
+    lib.vef_drag.restype = ctypes.c_int
+    lib.vef_nodal_stress.argtypes = [c_vp, c_vp, ctypes.c_double, c_vp, c_vp]
+    lib.vef_nodal_stress.restype = ctypes.c_int
     lib.vef_assemble_host.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp]
     lib.vef_assemble_host.restype = ctypes.c_int
     lib.vef_profile.argtypes = [c_vp, c_i32]
@@ -220,6 +226,44 @@ class Assembler:
                                     float(dArN), float(dS), flag, ptr(b), ptr(values),
                                     ctypes.c_void_p(stream.cuda_stream))
         self._check(rc, "vef_arclength")
+
+    def set_drag_edges(self, edges: np.ndarray):
+        """Boundary edges (element, edge id 1..4) of the drag integral, in the
+        reference's order (mesh.boundary_edges(mesh, "Cylinder"))."""
+        e = np.ascontiguousarray(edges, dtype=np.int32).reshape(-1, 2)
+        self._check(self.lib.vef_set_drag_edges(self._h, len(e), _ptr(e)), "vef_set_drag_edges")
+        self._n_drag = len(e)
+
+    def drag(self, x, per_edge: bool = False, stream=None):
+        """Fd over the drag edges (calculateEdgeQuantities, ieq_ed = 1) as a
+        one-element device tensor; with per_edge, also each edge's term."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        Fd = torch.empty(1, dtype=torch.float64, device=x.device)
+        per = torch.empty(max(getattr(self, "_n_drag", 0), 1), dtype=torch.float64, device=x.device) \
+            if per_edge else None
+        rc = self.lib.vef_drag(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(Fd.data_ptr()),
+                               ctypes.c_void_p(per.data_ptr()) if per is not None else None,
+                               ctypes.c_void_p(stream.cuda_stream))
+        self._check(rc, "vef_drag")
+        if per_edge:
+            return Fd, per[:getattr(self, "_n_drag", 0)]
+        return Fd
+
+    def nodal_stress(self, x, WiN: float, stream=None):
+        """[n_nodes][4] device tensor (Txx, Txy, Tyy, Tzz) of the square-root
+        formulation (nodalStressTensorSQRT) at the given WiN."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        T = torch.empty((self.mesh.n_nodes, 4), dtype=torch.float64, device=x.device)
+        rc = self.lib.vef_nodal_stress(self._h, ctypes.c_void_p(x.data_ptr()), float(WiN),
+                                       ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
+        self._check(rc, "vef_nodal_stress")
+        return T
 
     def assemble_host(self, x: np.ndarray, flag_nr: str = "NRP"):
         """The same with numpy arrays (copies through the host)."""

@@ -783,6 +783,22 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
 // Gauss points, with setEdgeQuantities normals (Element_Calculations_mod.F90:887-916).
 // Bank layout: ebank[(edge*18 + n*2 + i)*56 + k], rows (node n, momentum i).
 // ----------------------------------------------------------------------------
+// Unnormalised outward normal of edge ied (setEdgeQuantities, Quadrangle,
+// Element_Calculations_mod.F90:887-905): edges 1 (eta = -1) and 3 (eta = +1)
+// run along xi, 2 and 4 along eta; 1 and 2 take (t_y, -t_x), 3 and 4
+// (-t_y, t_x) of the tangent t.  Written with selects: with a four-way switch
+// on ied, hipcc (ROCm 7.2) produced wrong normals for edge 4 in the drag
+// kernel although the optimised IR was right.
+__device__ __forceinline__ void edge_normal(int ied, double dxdc, double dydc, double dxde, double dyde,
+                                            double& n0, double& n1) {
+  const bool along_xi = (ied & 1) != 0;
+  const double tx = along_xi ? dxdc : dxde;
+  const double ty = along_xi ? dydc : dyde;
+  const bool first = ied <= 2;
+  n0 = first ? ty : -ty;
+  n1 = first ? -tx : tx;
+}
+
 struct EdgeGeo {
   double dFx[9], dFy[9];
   double n0, n1, WET;
@@ -829,12 +845,7 @@ vef_edge_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
       sG[g].dFy[n] = dcdy * dc[n] + dedy * de[n];
     }
     double n0, n1;
-    switch (ied) {
-      case 1: n0 = dydc; n1 = -dxdc; break;
-      case 2: n0 = dyde; n1 = -dxde; break;
-      case 3: n0 = -dydc; n1 = dxdc; break;
-      default: n0 = -dyde; n1 = dxde; break;
-    }
+    edge_normal(ied, dxdc, dydc, dxde, dyde, n0, n1);
     const double dL = sqrt(n0 * n0 + n1 * n1);
     sG[g].n0 = n0 / dL;
     sG[g].n1 = n1 / dL;
@@ -991,6 +1002,112 @@ vef_rhs_kernel(int64_t n_node_rows, int jac, const int64_t* __restrict__ rowptr,
 }
 
 // ----------------------------------------------------------------------------
+// Post-processing: drag on a boundary (calculateEdgeQuantities with ieq_ed = 1,
+// src/Post_Process_mod.F90:1796-1890) and the nodal stress tensor
+// (nodalStressTensorSQRT, :2176-2236).
+// ----------------------------------------------------------------------------
+constexpr int kDragEdgesPerBlock = 21;  // 63 lanes = 21 edges x 3 edge Gauss points
+
+// Lane (edge, g): dot_product(-normal, P_tot(:,1))*WET at edge Gauss point g,
+// then lane g = 0 of each edge forms term_el = ((0 + t0) + t1) + t2.
+__global__ void __launch_bounds__(64)
+vef_drag_edge_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn,
+                     const int32_t* __restrict__ edges, const double* __restrict__ x, int64_t eu_index,
+                     int32_t n_edges, DevParams prm, double* __restrict__ term_el) {
+  __shared__ double sT[kDragEdgesPerBlock][3];
+  const int tid = threadIdx.x;
+  const int le = tid / 3, g = tid % 3;
+  const int64_t k = (int64_t)blockIdx.x * kDragEdgesPerBlock + le;
+  const bool valid = tid < 3 * kDragEdgesPerBlock && k < n_edges;
+  if (valid) {
+    const int32_t e = edges[2 * k];
+    const int ied = edges[2 * k + 1];
+    const int ed = ied - 1;
+    double X[18], u[54];
+    for (int n = 0; n < 9; n++) {
+      const int64_t gn = conn[(int64_t)e * 9 + n];
+      X[2 * n] = coords[gn * 2];
+      X[2 * n + 1] = coords[gn * 2 + 1];
+      for (int q = 0; q < 6; q++) u[n * 6 + q] = x[gn * 6 + q];
+    }
+    const double* dc = c_tab.dcE[ed][g];
+    const double* de = c_tab.deE[ed][g];
+    const double* bf = c_tab.bfE[ed][g];
+    double dxdc = dc[0] * X[0], dydc = dc[0] * X[1], dxde = de[0] * X[0], dyde = de[0] * X[1];
+    for (int n = 1; n < 9; n++) {
+      dxdc = dxdc + dc[n] * X[2 * n];
+      dydc = dydc + dc[n] * X[2 * n + 1];
+      dxde = dxde + de[n] * X[2 * n];
+      dyde = dyde + de[n] * X[2 * n + 1];
+    }
+    const double detJ = dxdc * dyde - dxde * dydc;
+    const double dcdx = dyde / detJ, dedx = -(dydc / detJ), dcdy = -(dxde / detJ), dedy = dxdc / detJ;
+    double GU00 = 0, GU10 = 0, GU01 = 0, GU11 = 0, P = 0, S00 = 0, S01 = 0, S11 = 0;
+    for (int n = 0; n < 9; n++) {
+      const double fx = dcdx * dc[n] + dedx * de[n];
+      const double fy = dcdy * dc[n] + dedy * de[n];
+      const double b = bf[n];
+      if (n == 0) {
+        GU00 = u[0] * fx; GU10 = u[0] * fy; GU01 = u[1] * fx; GU11 = u[1] * fy;
+        P = u[2] * b; S00 = u[3] * b; S01 = u[4] * b; S11 = u[5] * b;
+      } else {
+        GU00 = GU00 + u[n * 6] * fx; GU10 = GU10 + u[n * 6] * fy;
+        GU01 = GU01 + u[n * 6 + 1] * fx; GU11 = GU11 + u[n * 6 + 1] * fy;
+        P = P + u[n * 6 + 2] * b; S00 = S00 + u[n * 6 + 3] * b;
+        S01 = S01 + u[n * 6 + 4] * b; S11 = S11 + u[n * 6 + 5] * b;
+      }
+    }
+    const double S10 = S01;
+    const double WiN = prm.WiN0 * x[eu_index];
+    const double tr = GU00 + GU11;
+    const double htr = (tr * 2.0) / (2.0 * 2.0);
+    const double Gt00 = GU00 - htr, Gt10 = GU10, Gt01 = GU01;
+    const double Gd00 = Gt00 + Gt00, Gd10 = Gt10 + Gt01;
+    const double C00 = S00 * S00 + S01 * S10;
+    const double C10 = S10 * S00 + S11 * S10;
+    const double a = prm.omb / WiN;
+    const double P00 = a * (C00 - 1.0) + prm.BvN * Gd00 - P;
+    const double P10 = a * (C10 - 0.0) + prm.BvN * Gd10;
+    double n0, n1;
+    edge_normal(ied, dxdc, dydc, dxde, dyde, n0, n1);
+    const double dL = sqrt(n0 * n0 + n1 * n1);
+    n0 = n0 / dL;
+    n1 = n1 / dL;
+    const double WET = c_tab.W1[g] * dL;
+    const double d = (0.0 + (-n0) * P00) + (-n1) * P10;  // the third term, (-0)*0, adds nothing
+    sT[le][g] = d * WET;
+  }
+  __syncthreads();
+  if (valid && g == 0) term_el[k] = ((0.0 + sT[le][0]) + sT[le][1]) + sT[le][2];
+}
+
+// term = term + term_el over the edges in order (one thread; a few hundred edges).
+__global__ void vef_drag_sum_kernel(int32_t n_edges, const double* __restrict__ term_el, double* __restrict__ Fd) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t = 0.0;
+  for (int32_t k = 0; k < n_edges; k++) t = t + term_el[k];
+  Fd[0] = t;
+}
+
+// T = ((1-BvN)/WiN)*(C - I), C = S.S summed from zero over k = 1..3 (g1 = g2 = 1).
+__global__ void __launch_bounds__(256)
+vef_nodal_stress_kernel(int64_t n_nodes, const double* __restrict__ x, double omb, double WiN,
+                        double* __restrict__ T) {
+  const int64_t gnod = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gnod >= n_nodes) return;
+  const double S00 = x[gnod * 6 + 3], S01 = x[gnod * 6 + 4], S11 = x[gnod * 6 + 5];
+  const double S10 = S01;
+  const double C00 = (0.0 + S00 * S00) + S01 * S10;
+  const double C01 = (0.0 + S00 * S01) + S01 * S11;
+  const double C11 = (0.0 + S10 * S01) + S11 * S11;
+  const double a = omb / WiN;
+  T[gnod * 4 + 0] = a * (1.0 * C00 - 1.0 * 1.0);
+  T[gnod * 4 + 1] = a * (1.0 * C01 - 1.0 * 0.0);
+  T[gnod * 4 + 2] = a * (1.0 * C11 - 1.0 * 1.0);
+  T[gnod * 4 + 3] = a * (1.0 * 0.0 - 1.0 * 1.0);
+}
+
+// ----------------------------------------------------------------------------
 // Arclength row (arclength / pseudoArclength, src/Extra_Equations_mod.F90:698-828)
 // ----------------------------------------------------------------------------
 // Increment <= 2: MatZeroRowsColumns(N-1) with diagonal 1.  Column N-1 is the
@@ -1096,6 +1213,9 @@ struct vef_handle {
   int32_t* d_econtrib = nullptr;
   double* d_ebank = nullptr;  // n_edges*18*56
   double* d_arc = nullptr;    // per-field partial sums of the arclength dot product
+  int32_t* d_drag_edges = nullptr;  // boundary edges of the drag integral, [n][2]
+  double* d_drag_terms = nullptr;   // term_el per edge
+  int32_t n_drag_edges = 0;
   int arc_blocks = 0;
   double* d_tmp_x = nullptr;  // host-assemble staging
   double* d_tmp_b = nullptr;
@@ -1220,6 +1340,7 @@ void vef_destroy(vef_handle* h) {
                   h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_code,    h->d_sbase,
                   h->d_slen,   h->d_ustart,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
+                  h->d_drag_edges, h->d_drag_terms,
                   h->d_tmp_b,  h->d_tmp_v};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1576,6 +1697,54 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
   VEF_HIP(h, hipGetLastError());
   if (flag == VEF_RES_JAC) {
     hipLaunchKernelGGL(vef_copy_kernel, dim3(1024), dim3(256), 0, st, values + h->h_rowptr[N - 1], tangent, N);
+    VEF_HIP(h, hipGetLastError());
+  }
+  return VEF_OK;
+}
+
+int vef_set_drag_edges(vef_handle* h, int32_t n_edges, const int32_t* edges) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (n_edges < 0 || (n_edges > 0 && !edges)) return set_err(h, VEF_ERR_ARG, "bad edge list");
+  for (int32_t k = 0; k < n_edges; k++) {
+    if (edges[2 * k] < 0 || edges[2 * k] >= h->n_el) return set_err(h, VEF_ERR_MESH, "drag edge element out of range");
+    if (edges[2 * k + 1] < 1 || edges[2 * k + 1] > 4) return set_err(h, VEF_ERR_MESH, "drag edge id must be 1..4");
+  }
+  if (h->d_drag_edges) (void)hipFree(h->d_drag_edges);
+  if (h->d_drag_terms) (void)hipFree(h->d_drag_terms);
+  h->d_drag_edges = nullptr;
+  h->d_drag_terms = nullptr;
+  h->n_drag_edges = n_edges;
+  if (n_edges == 0) return VEF_OK;
+  VEF_HIP(h, hipMalloc((void**)&h->d_drag_edges, sizeof(int32_t) * 2 * (size_t)n_edges));
+  VEF_HIP(h, hipMalloc((void**)&h->d_drag_terms, sizeof(double) * (size_t)n_edges));
+  VEF_HIP(h, hipMemcpy(h->d_drag_edges, edges, sizeof(int32_t) * 2 * (size_t)n_edges, hipMemcpyHostToDevice));
+  return VEF_OK;
+}
+
+int vef_drag(vef_handle* h, const double* x, double* Fd, double* per_edge, void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !Fd) return set_err(h, VEF_ERR_ARG, "x and Fd are required");
+  hipStream_t st = (hipStream_t)stream;
+  const int32_t n = h->n_drag_edges;
+  double* terms = per_edge ? per_edge : h->d_drag_terms;
+  if (n > 0) {
+    const int blocks = (n + kDragEdgesPerBlock - 1) / kDragEdgesPerBlock;
+    hipLaunchKernelGGL(vef_drag_edge_kernel, dim3(blocks), dim3(64), 0, st, h->d_coords, h->d_conn,
+                       h->d_drag_edges, x, h->N - 1, n, h->dprm, terms);
+    VEF_HIP(h, hipGetLastError());
+  }
+  hipLaunchKernelGGL(vef_drag_sum_kernel, dim3(1), dim3(64), 0, st, n, terms, Fd);
+  VEF_HIP(h, hipGetLastError());
+  return VEF_OK;
+}
+
+int vef_nodal_stress(vef_handle* h, const double* x, double WiN, double* T, void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !T) return set_err(h, VEF_ERR_ARG, "x and T are required");
+  const int64_t n = h->n_nodes;
+  if (n > 0) {
+    hipLaunchKernelGGL(vef_nodal_stress_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, x, h->dprm.omb, WiN, T);
     VEF_HIP(h, hipGetLastError());
   }
   return VEF_OK;

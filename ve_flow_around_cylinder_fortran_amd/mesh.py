@@ -178,6 +178,36 @@ def finish_mesh(coords: np.ndarray, conn: np.ndarray, geometry: Dict[str, float]
                 boundary_nodes={k: v.astype(np.int64) for k, v in bnd.items()}, geometry=dict(geometry))
 
 
+def boundary_edges(mesh: Mesh, name: str) -> np.ndarray:
+    """Edges (element, edge id 1..4) whose three nodes lie on boundary `name`,
+    ascending in element then edge id — the order of setBoundaryEdges
+    (src/Boundary_mod.F90:477-522) — with the distance tests of
+    distanceToBNDCylinder (:1383-1414): Inlet |x + L|, Outlet |x - L|,
+    Wall |y -+ H|, Cylinder |x^2 + y^2 - R^2|, each below the mesh tolerance."""
+    g = mesh.geometry
+    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+    tol = MESH_TOLERANCE
+    if name == "Inlet":
+        on = np.abs(x + g["L_up"]) < tol
+    elif name == "Outlet":
+        on = np.abs(x - g["L_down"]) < tol
+    elif name == "Wall":
+        on = (np.abs(y + g["H"]) < tol) | (np.abs(y - g["H"]) < tol)
+    elif name == "Cylinder":
+        on = np.abs(x * x + y * y - g["R"] * g["R"]) < tol
+    else:
+        raise ValueError(f"unknown boundary {name!r}")
+    if name == "Wall":  # the two walls are separate boundaries; an edge lies on one of them
+        lo = np.abs(y + g["H"]) < tol
+        hi = np.abs(y - g["H"]) < tol
+        hit = np.stack([np.all(lo[mesh.conn[:, list(EDGE_NODES[ied])]], axis=1)
+                        | np.all(hi[mesh.conn[:, list(EDGE_NODES[ied])]], axis=1) for ied in (1, 2, 3, 4)], axis=1)
+    else:
+        hit = np.stack([np.all(on[mesh.conn[:, list(EDGE_NODES[ied])]], axis=1) for ied in (1, 2, 3, 4)], axis=1)
+    el, ed = np.nonzero(hit)
+    return np.stack([el, ed + 1], axis=1).astype(np.int32).reshape(-1, 2)
+
+
 def rcm_reorder(coords: np.ndarray, conn: np.ndarray):
     """Reverse Cuthill-McKee node numbering; elements sorted by lowest node."""
     import scipy.sparse as sp
