@@ -119,6 +119,18 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
                   int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
                   void* hip_stream);
 
+/* Dirichlet rows, applied after the assembly and the arclength row
+ * (applyDirichletBCFullyDeveloped and applyDirichletBC in assemblePetscObjects,
+ * src/Dirichlet_mod.F90:104-190, src/Newton_Raphson_mod.F90:384-394).
+ * vef_set_dirichlet stores rows (host int64[n]: global rows 6*gnod + ieq) with
+ * their prescribed values (host double[n]); a row listed more than once keeps
+ * its last value, as the reference's boundary-by-boundary INSERT_VALUES does.
+ * vef_apply_dirichlet sets b[row] = x[row] - value and, with VEF_RES_JAC,
+ * zeroes the row's stored values and puts 1 on its diagonal (MatZeroRows with
+ * diag = 1). */
+int vef_set_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows, const double* values);
+int vef_apply_dirichlet(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* hip_stream);
+
 /* Drag on a boundary, the Fd the reference prints after each converged step
  * (calculateEdgeQuantities with ieq_ed = 1, src/Post_Process_mod.F90:1796-1890;
  * the edges of the Cylinder boundary from setBoundaryEdges,

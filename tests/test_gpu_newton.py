@@ -1,0 +1,88 @@
+"""Newton counterpart on the GPU: Dirichlet rows against their numpy
+restatement (bit for bit), the re-factorisation LU against scipy, and a full
+fixed-Wi Newton solve against the same iteration on the CPU with the oracle's
+assembly and SuperLU (same flags and iteration count; states within 1e-8)."""
+import numpy as np
+import pytest
+
+from ve_flow_around_cylinder_fortran_amd import Assembler, Params, cylinder_channel, smooth_state
+from ve_flow_around_cylinder_fortran_amd.mesh import boundary_edges
+from ve_flow_around_cylinder_fortran_amd.newton import NewtonSolver, cylinder_dirichlet, initial_state
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+OLDROYD_B = dict(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1e-3, WiN0=1.0, model="L-PTT")
+
+
+def test_dirichlet_rows_bitwise(oracle):
+    from newton_cpu import apply_dirichlet, dirichlet_rows
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    x = smooth_state(mesh, EU=0.3)
+    rows, vals = cylinder_dirichlet(mesh, 0.3)
+    asm = Assembler(mesh, Params(**OLDROYD_B))
+    asm.set_dirichlet(rows, vals)
+    xd = torch.from_numpy(x).cuda()
+    b, v = asm.assemble(xd, "NRP")
+    asm.apply_dirichlet(xd, b, v, "NRP")
+    torch.cuda.synchronize()
+    rowptr, col = asm.pattern()
+    ob, ov = oracle.assemble(oracle.params(**OLDROYD_B), mesh, x, rowptr=rowptr, col=col)
+    r, vv = dirichlet_rows(rows, vals, asm.N)
+    apply_dirichlet(r, vv, x, ob, ov, rowptr, col)
+    assert np.array_equal(b.cpu().numpy(), ob) and np.array_equal(v.cpu().numpy(), ov)
+
+
+def test_refactorization_lu_matches_scipy(oracle):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from newton_cpu import apply_dirichlet, dirichlet_rows
+    from ve_flow_around_cylinder_fortran_amd.linsolve import RfLU
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    asm = Assembler(mesh, Params(**OLDROYD_B))
+    rows, vals = cylinder_dirichlet(mesh, 0.1)
+    asm.set_dirichlet(rows, vals)
+    rowptr, col = asm.pattern()
+    lu = RfLU(rowptr, col)
+    for k, x in enumerate([initial_state(mesh, 0.1), smooth_state(mesh, EU=0.1)]):
+        print(f"[gpu newton] LU matrix {k}", flush=True)
+        xd = torch.from_numpy(x).cuda()
+        b, v = asm.assemble(xd, "NRP")
+        asm.arclength(xd, b, v, increment=1, LN0=0.1)
+        asm.apply_dirichlet(xd, b, v, "NRP")
+        lu.factor(v)
+        s = lu.solve(b)
+        torch.cuda.synchronize()
+        A = sp.csr_matrix((v.cpu().numpy(), col, rowptr), shape=(asm.N, asm.N))
+        ref = sla.spsolve(A.tocsc(), b.cpu().numpy())
+        err = np.abs(s.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < 1e-9, err
+    assert lu.analyses == 1 and lu.refactorizations == 2 and lu.host_fallbacks == 0
+
+
+def test_newton_matches_cpu_iteration(oracle):
+    from newton_cpu import newton_cpu
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    LN0 = 0.1
+    x0 = initial_state(mesh, LN0)
+    print("[gpu newton] full solve, small mesh", flush=True)
+    solver = NewtonSolver(mesh, Params(**OLDROYD_B))
+    xg, hist = solver.solve(x0, LN0)
+    xc, hc = newton_cpu(oracle, mesh, OLDROYD_B, x0, LN0)
+    assert [h.flag for h in hist] == [h[1] for h in hc]
+    xg = xg.cpu().numpy()
+    assert np.abs(xg - xc).max() <= 1e-8 * np.abs(xc).max()
+    assert hist[-1].cor_norm < 1e-8 and hist[-1].res_norm < 1e-8
+    # quadratic convergence at the end: each NRP correction below the square of the previous (scaled)
+    cors = [h.cor_norm for h in hist]
+    assert cors[-1] < 1e-8 and cors[-2] < 1e-4
+    # the drag of the converged state, GPU against the oracle on the CPU state
+    edges = boundary_edges(mesh, "Cylinder")
+    solver.asm.set_drag_edges(edges)
+    Fd = solver.asm.drag(torch.from_numpy(xg).cuda()).item()
+    oFd, _ = oracle.drag(oracle.params(**OLDROYD_B), mesh, xc, edges)
+    assert abs(Fd - oFd) <= 1e-8 * abs(oFd)

@@ -104,6 +104,10 @@ def load_library():
     lib.vef_arclength.argtypes = [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, c_i32, c_vp, c_vp, c_vp]
     lib.vef_arclength.restype = ctypes.c_int
+    lib.vef_set_dirichlet.argtypes = [c_vp, c_i64, c_vp, c_vp]
+    lib.vef_set_dirichlet.restype = ctypes.c_int
+    lib.vef_apply_dirichlet.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]
+    lib.vef_apply_dirichlet.restype = ctypes.c_int
     lib.vef_set_drag_edges.argtypes = [c_vp, c_i32, c_ip]
     lib.vef_set_drag_edges.restype = ctypes.c_int
     lib.vef_drag.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp]
@@ -226,6 +230,27 @@ class Assembler:
                                     float(dArN), float(dS), flag, ptr(b), ptr(values),
                                     ctypes.c_void_p(stream.cuda_stream))
         self._check(rc, "vef_arclength")
+
+    def set_dirichlet(self, rows: np.ndarray, values: np.ndarray):
+        """Dirichlet rows (global 6*gnod + ieq) and their prescribed values; a
+        row listed twice keeps its last value."""
+        r = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if r.shape != v.shape:
+            raise VefError("rows and values differ in length")
+        self._check(self.lib.vef_set_dirichlet(self._h, len(r), _ptr(r), _ptr(v)), "vef_set_dirichlet")
+
+    def apply_dirichlet(self, x, b, values=None, flag_nr: str = "NRP", stream=None):
+        """b[row] = x[row] - value; with 'NRP' the rows of values become unit rows."""
+        import torch
+
+        flag = self._flag(flag_nr)
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        rc = self.lib.vef_apply_dirichlet(self._h, ctypes.c_void_p(x.data_ptr()), flag, ctypes.c_void_p(b.data_ptr()),
+                                          ctypes.c_void_p(values.data_ptr()) if values is not None else None,
+                                          ctypes.c_void_p(stream.cuda_stream))
+        self._check(rc, "vef_apply_dirichlet")
 
     def set_drag_edges(self, edges: np.ndarray):
         """Boundary edges (element, edge id 1..4) of the drag integral, in the

@@ -1108,6 +1108,24 @@ vef_nodal_stress_kernel(int64_t n_nodes, const double* __restrict__ x, double om
 }
 
 // ----------------------------------------------------------------------------
+// Dirichlet rows (applyDirichletBC / applyDirichletBCFullyDeveloped,
+// src/Dirichlet_mod.F90:104-190): b[r] = x[r] - value and, with the Jacobian,
+// MatZeroRows with diagonal 1.  One wavefront per row; the rows are distinct.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+vef_dirichlet_kernel(int64_t n_rows, const int64_t* __restrict__ rows, const double* __restrict__ target,
+                     const double* __restrict__ x, int jac, const int64_t* __restrict__ rowptr,
+                     const int32_t* __restrict__ col, double* __restrict__ b, double* __restrict__ values) {
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (i >= n_rows) return;
+  const int64_t r = rows[i];
+  if (lane == 0) b[r] = x[r] - target[i];
+  if (!jac) return;
+  for (int64_t p = rowptr[r] + lane; p < rowptr[r + 1]; p += 64) values[p] = (col[p] == r) ? 1.0 : 0.0;
+}
+
+// ----------------------------------------------------------------------------
 // Arclength row (arclength / pseudoArclength, src/Extra_Equations_mod.F90:698-828)
 // ----------------------------------------------------------------------------
 // Increment <= 2: MatZeroRowsColumns(N-1) with diagonal 1.  Column N-1 is the
@@ -1213,6 +1231,9 @@ struct vef_handle {
   int32_t* d_econtrib = nullptr;
   double* d_ebank = nullptr;  // n_edges*18*56
   double* d_arc = nullptr;    // per-field partial sums of the arclength dot product
+  int64_t* d_bc_rows = nullptr;     // Dirichlet rows, distinct
+  double* d_bc_values = nullptr;    // their target values
+  int64_t n_bc = 0;
   int32_t* d_drag_edges = nullptr;  // boundary edges of the drag integral, [n][2]
   double* d_drag_terms = nullptr;   // term_el per edge
   int32_t n_drag_edges = 0;
@@ -1340,7 +1361,7 @@ void vef_destroy(vef_handle* h) {
                   h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_code,    h->d_sbase,
                   h->d_slen,   h->d_ustart,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
-                  h->d_drag_edges, h->d_drag_terms,
+                  h->d_drag_edges, h->d_drag_terms, h->d_bc_rows, h->d_bc_values,
                   h->d_tmp_b,  h->d_tmp_v};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1699,6 +1720,49 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
     hipLaunchKernelGGL(vef_copy_kernel, dim3(1024), dim3(256), 0, st, values + h->h_rowptr[N - 1], tangent, N);
     VEF_HIP(h, hipGetLastError());
   }
+  return VEF_OK;
+}
+
+int vef_set_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows, const double* values) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (n_rows < 0 || (n_rows > 0 && (!rows || !values))) return set_err(h, VEF_ERR_ARG, "bad Dirichlet list");
+  // The reference inserts boundary by boundary, so a row listed twice keeps its last value.
+  std::vector<int64_t> last(h->N, -1);
+  for (int64_t i = 0; i < n_rows; i++) {
+    if (rows[i] < 0 || rows[i] >= h->N) return set_err(h, VEF_ERR_ARG, "Dirichlet row out of range");
+    last[rows[i]] = i;
+  }
+  std::vector<int64_t> r;
+  std::vector<double> v;
+  for (int64_t row = 0; row < h->N; row++)
+    if (last[row] >= 0) {
+      r.push_back(row);
+      v.push_back(values[last[row]]);
+    }
+  if (h->d_bc_rows) (void)hipFree(h->d_bc_rows);
+  if (h->d_bc_values) (void)hipFree(h->d_bc_values);
+  h->d_bc_rows = nullptr;
+  h->d_bc_values = nullptr;
+  h->n_bc = (int64_t)r.size();
+  if (h->n_bc == 0) return VEF_OK;
+  VEF_HIP(h, hipMalloc((void**)&h->d_bc_rows, sizeof(int64_t) * r.size()));
+  VEF_HIP(h, hipMalloc((void**)&h->d_bc_values, sizeof(double) * v.size()));
+  VEF_HIP(h, hipMemcpy(h->d_bc_rows, r.data(), sizeof(int64_t) * r.size(), hipMemcpyHostToDevice));
+  VEF_HIP(h, hipMemcpy(h->d_bc_values, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+  return VEF_OK;
+}
+
+int vef_apply_dirichlet(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "bad flag");
+  if (flag == VEF_RES_JAC && !values) return set_err(h, VEF_ERR_ARG, "values is required for VEF_RES_JAC");
+  if (h->n_bc == 0) return VEF_OK;
+  const int64_t blocks = (h->n_bc + 3) / 4;
+  hipLaunchKernelGGL(vef_dirichlet_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, h->n_bc,
+                     h->d_bc_rows, h->d_bc_values, x, flag == VEF_RES_JAC ? 1 : 0, h->d_rowptr, h->d_col, b,
+                     values);
+  VEF_HIP(h, hipGetLastError());
   return VEF_OK;
 }
 

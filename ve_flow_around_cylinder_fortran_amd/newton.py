@@ -1,0 +1,167 @@
+"""Newton counterpart for one continuation step with the arclength unknown
+pinned (the reference's increments 1-2), built on the GPU assembler.
+
+Per iteration, in the order of assemblePetscObjects
+(src/Newton_Raphson_mod.F90:281-400): residual (and, for 'NRP', the
+forward-difference Jacobian) on the GPU; the arclength row pinning EU
+(src/Extra_Equations_mod.F90:698-760); the Dirichlet rows — the fully
+developed inflow first, then Outlet P = 0, Wall and Cylinder U = 0
+(src/Dirichlet_mod.F90:104-190, src/Boundary_mod.F90:218-245); ||b||; the
+sparse solve (GPU re-factorisation, `linsolve.RfLU`; 'MNR' iterations reuse the
+last factors as the reference reuses its factored matrix); ||s||; the flag
+logic of checkNewtonConvergence (:470-528); x <- x - xF*s (updateNewton,
+:618-680).  Convergence is the reference's test at the top of the loop:
+||s|| < 1e-8 and ||b|| < 1e-8 (Cor_Norm_eps, src/Parameters_mod.F90:214).
+
+The fully developed inflow: the reference solves a separate 1D problem for it
+(loopNewtonRaphsonInflow); here it is the closed-form Oldroyd-B channel
+solution with mean velocity 1 (the flow rate 2*Height of Inflow_1D,
+src/Physical_mod.F90:48-49), so this driver takes Oldroyd-B (VeN = 0) only.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from typing import List
+
+import numpy as np
+
+from .assembly import Assembler, Params
+from .mesh import MESH_TOLERANCE, Mesh
+
+COR_NORM_EPS = 1.0e-8
+MNR_EPS = 1.0e-1
+COR_NORM_DIV = 1.0e8
+NITER = 50
+
+
+def oldroyd_b_inflow(y: np.ndarray, H: float, WiN: float):
+    """Fully developed Oldroyd-B channel flow, mean velocity 1:
+    U = 1.5 (1 - (y/H)^2), gamma = dU/dy, C = [[1 + 2 (Wi gamma)^2, Wi gamma],
+    [Wi gamma, 1]] and S = C^(1/2) (the symmetric positive root).  Returns
+    (Ux, Uy, Sxx, Sxy, Syy)."""
+    y = np.asarray(y, dtype=np.float64)
+    U = 1.5 * (1.0 - (y / H) ** 2)
+    g = -3.0 * y / H ** 2
+    c11 = 1.0 + 2.0 * (WiN * g) ** 2
+    c12 = WiN * g
+    c22 = np.ones_like(y)
+    s = np.sqrt(c11 * c22 - c12 * c12)
+    t = np.sqrt(c11 + c22 + 2.0 * s)
+    return U, np.zeros_like(y), (c11 + s) / t, c12 / t, (c22 + s) / t
+
+
+def cylinder_dirichlet(mesh: Mesh, WiN: float):
+    """Rows and values of the Cylinder_2D Dirichlet conditions, in the order
+    the reference inserts them (a row listed twice keeps the later value)."""
+    g = mesh.geometry
+    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+    tol = MESH_TOLERANCE
+    rows: List[np.ndarray] = []
+    vals: List[np.ndarray] = []
+    inlet = np.nonzero(np.abs(x + g["L_up"]) < tol)[0]
+    prof = oldroyd_b_inflow(y[inlet], g["H"], WiN)
+    for q, v in zip((0, 1, 3, 4, 5), prof):  # Ux, Uy, Sxx, Sxy, Syy (P is skipped)
+        rows.append(6 * inlet + q)
+        vals.append(v)
+    outlet = np.nonzero(np.abs(x - g["L_down"]) < tol)[0]
+    rows.append(6 * outlet + 2)
+    vals.append(np.zeros(len(outlet)))
+    for on in (np.abs(y + g["H"]) < tol, np.abs(y - g["H"]) < tol,
+               np.abs(x * x + y * y - g["R"] * g["R"]) < tol):
+        nodes = np.nonzero(on)[0]
+        for q in (0, 1):
+            rows.append(6 * nodes + q)
+            vals.append(np.zeros(len(nodes)))
+    return np.concatenate(rows).astype(np.int64), np.concatenate(vals)
+
+
+def initial_state(mesh: Mesh, EU: float) -> np.ndarray:
+    """The rest state U = 0, P = 0, S = I with the arclength unknown EU: the
+    first Newton iteration from it is the Stokes-like solve with the boundary
+    values, the start the reference takes from its Newtonian state."""
+    x = np.zeros(mesh.n_unknowns)
+    u = x[:-1].reshape(-1, 6)
+    u[:, 3] = 1.0
+    u[:, 5] = 1.0
+    x[-1] = EU
+    return x
+
+
+@dataclasses.dataclass
+class NewtonRecord:
+    iteration: int
+    flag: str
+    res_norm: float
+    cor_norm: float
+    seconds: float
+
+
+class NewtonError(RuntimeError):
+    pass
+
+
+class NewtonSolver:
+    """Fixed-Wi Newton solve (arclength unknown pinned at LN0) on the GPU."""
+
+    def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0"):
+        import torch
+
+        from .linsolve import RfLU
+
+        if params.VeN != 0.0 or params.model != "L-PTT":
+            raise NewtonError("the closed-form inflow covers Oldroyd-B (L-PTT with VeN = 0) only")
+        self.torch = torch
+        self.mesh = mesh
+        self.params = params
+        self.device = device
+        self.asm = Assembler(mesh, params)
+        rowptr, col = self.asm.pattern()
+        self.lu = RfLU(rowptr, col, device=device)
+        self.N = self.asm.N
+        self.b = torch.empty(self.N, dtype=torch.float64, device=device)
+        self.values = torch.empty(self.asm.nnz, dtype=torch.float64, device=device)
+
+    def solve(self, x0: np.ndarray, LN0: float, xF: float = 1.0, max_iter: int = NITER):
+        """Newton from x0 with EU pinned at LN0; returns (x as a device tensor,
+        list of NewtonRecord)."""
+        torch = self.torch
+        WiN = self.params.WiN0 * LN0
+        rows, vals = cylinder_dirichlet(self.mesh, WiN)
+        self.asm.set_dirichlet(rows, vals)
+        x = torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
+        flag = "NRP"
+        res_norm = cor_new = cor_old = 1.0
+        it = 0
+        hist: List[NewtonRecord] = []
+        while True:
+            if cor_new < COR_NORM_EPS and res_norm < COR_NORM_EPS:
+                break
+            t0 = time.perf_counter()
+            it += 1
+            if it % 10 == 0:
+                flag = "NRP"
+            jac = flag == "NRP"
+            self.asm.assemble(x, flag, b=self.b, values=self.values if jac else None)
+            self.asm.arclength(x, self.b, self.values if jac else None, increment=1, LN0=LN0, dArN=0.0,
+                               flag_nr=flag)
+            self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
+            res_norm = float(torch.linalg.vector_norm(self.b))
+            if jac:
+                self.lu.factor(self.values)
+            s = self.lu.solve(self.b)
+            cor_new = float(torch.linalg.vector_norm(s))
+            if cor_new != cor_new:
+                raise NewtonError("NaN correction")
+            hist.append(NewtonRecord(it, flag, res_norm, cor_new, 0.0))
+            if flag == "NRP":
+                flag = "MNR" if cor_new < MNR_EPS else "NRP"
+            else:
+                flag = "NRP" if cor_new > cor_old else "MNR"
+            if cor_new > COR_NORM_DIV or it > max_iter:
+                raise NewtonError(f"no convergence after {it} iterations (||s|| = {cor_new:.3e})")
+            cor_old = cor_new
+            x = x - xF * s
+            torch.cuda.synchronize()
+            hist[-1].seconds = time.perf_counter() - t0
+        return x, hist
