@@ -1,0 +1,141 @@
+"""TEST INFRASTRUCTURE ONLY (part of the oracle): the Newton iteration and the
+arclength continuation of ve_flow_around_cylinder_fortran_amd.newton restated
+on the CPU with the oracle's assembly, MatZeroRows-style Dirichlet rows in
+numpy and scipy's SuperLU, as the reference's loopNewtonRaphson
+(src/Newton_Raphson_mod.F90:21-172) and continuationStepArclength
+(src/Continuation_Arclength_mod.F90:77-377) run them.  Used by tests/ as the
+checker and by bench.py's CPU baseline."""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from ve_flow_around_cylinder_fortran_amd.newton import (COR_NORM_DIV, COR_NORM_EPS, MNR_EPS, NITER,
+                                                        cylinder_dirichlet)
+
+
+def dirichlet_rows(rows, vals, N):
+    """Last value wins for repeated rows (INSERT_VALUES order)."""
+    last = {}
+    for r, v in zip(rows.tolist(), vals.tolist()):
+        last[r] = v
+    r = np.array(sorted(last), dtype=np.int64)
+    return r, np.array([last[i] for i in r])
+
+
+def apply_dirichlet(rows, vals, x, b, values, rowptr, col):
+    b[rows] = x[rows] - vals
+    if values is not None:
+        for r in rows:
+            s, e = rowptr[r], rowptr[r + 1]
+            values[s:e] = np.where(col[s:e] == r, 1.0, 0.0)
+
+
+def newton_cpu(oracle, mesh, pset, x0, LN0, max_iter=NITER, nthreads=8, xF=1.0):
+    prm = oracle.params(**pset)
+    rowptr, col = oracle.pattern(mesh)
+    N = mesh.n_unknowns
+    rows, vals = dirichlet_rows(*cylinder_dirichlet(mesh, pset["WiN0"] * LN0), N)
+    x = np.array(x0, dtype=np.float64)
+    flag, res_norm, cor_new, cor_old, it = "NRP", 1.0, 1.0, 1.0, 0
+    lu = None
+    hist = []
+    while not (cor_new < COR_NORM_EPS and res_norm < COR_NORM_EPS):
+        it += 1
+        if it % 10 == 0:
+            flag = "NRP"
+        jac = flag == "NRP"
+        b, v = oracle.assemble(prm, mesh, x, rowptr=rowptr, col=col, jac=jac, nthreads=nthreads)
+        vv = v if jac else np.zeros(len(col))
+        oracle.arclength(0, mesh.n_nodes, x, x, x, 1, LN0, 0.0, 0.0, rowptr, col, vv, b, jac=jac)
+        apply_dirichlet(rows, vals, x, b, v if jac else None, rowptr, col)
+        res_norm = float(np.linalg.norm(b))
+        if jac:
+            lu = sla.splu(sp.csr_matrix((v, col, rowptr), shape=(N, N)).tocsc(), permc_spec="MMD_AT_PLUS_A",
+                          options=dict(DiagPivotThresh=0.01))
+        s = lu.solve(b)
+        cor_new = float(np.linalg.norm(s))
+        hist.append((it, flag, res_norm, cor_new))
+        if flag == "NRP":
+            flag = "MNR" if cor_new < MNR_EPS else "NRP"
+        else:
+            flag = "NRP" if cor_new > cor_old else "MNR"
+        if cor_new > COR_NORM_DIV or it > max_iter:
+            raise RuntimeError(f"no convergence: {hist}")
+        cor_old = cor_new
+        x = x - xF * s
+    return x, hist
+
+
+def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent, dS, nthreads=8):
+    """One Newton solve of a continuation step (pinned for increments 1-2,
+    pseudo-arclength after); the inflow values follow the current EU.
+    Returns (x, history, last SuperLU factorisation)."""
+    prm = oracle.params(**pset)
+    rowptr, col = oracle.pattern(mesh)
+    N = mesh.n_unknowns
+    x = np.array(x0, dtype=np.float64)
+    flag, res_norm, cor_new, cor_old, it = "NRP", 1.0, 1.0, 1.0, 0
+    lu = None
+    hist = []
+    mode = 0 if increment <= 2 else 1
+    xo = x_old if x_old is not None else x
+    t = tangent if tangent is not None else x
+    while not (cor_new < COR_NORM_EPS and res_norm < COR_NORM_EPS):
+        it += 1
+        if it % 10 == 0:
+            flag = "NRP"
+        jac = flag == "NRP"
+        rows, vals = dirichlet_rows(*cylinder_dirichlet(mesh, pset["WiN0"] * x[-1]), N)
+        b, v = oracle.assemble(prm, mesh, x, rowptr=rowptr, col=col, jac=jac, nthreads=nthreads)
+        vv = v if jac else np.zeros(len(col))
+        oracle.arclength(mode, mesh.n_nodes, x, xo, t, increment, LN0, dArN, dS, rowptr, col, vv, b, jac=jac)
+        apply_dirichlet(rows, vals, x, b, v if jac else None, rowptr, col)
+        res_norm = float(np.linalg.norm(b))
+        if jac:
+            lu = sla.splu(sp.csr_matrix((v, col, rowptr), shape=(N, N)).tocsc(), permc_spec="MMD_AT_PLUS_A",
+                          options=dict(DiagPivotThresh=0.01))
+        s = lu.solve(b)
+        cor_new = float(np.linalg.norm(s))
+        hist.append((it, flag, res_norm, cor_new))
+        if flag == "NRP":
+            flag = "MNR" if cor_new < MNR_EPS else "NRP"
+        else:
+            flag = "NRP" if cor_new > cor_old else "MNR"
+        if cor_new > COR_NORM_DIV or it > NITER:
+            raise RuntimeError(f"no convergence: {hist}")
+        cor_old = cor_new
+        x = x - s
+    return x, hist, lu
+
+
+def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=50.0, eps_pr=1.0e-3):
+    """continuationStepArclength restated with newton_cpu_arc; returns the
+    converged states and the step sizes used."""
+    x = np.array(x0, dtype=np.float64)
+    x_old = tangent = None
+    dS = dS0
+    states, dss = [], []
+    for k in range(1, steps + 1):
+        predicted = x.copy()
+        dss.append(dS)
+        xc, hist, lu = newton_cpu_arc(oracle, mesh, pset, x, LN0, k, dArN, x_old, tangent, dS)
+        if k > 1:
+            if k == 2:
+                d = xc - x_old
+                tangent = d / np.linalg.norm(d)
+            else:
+                e = np.zeros(len(x))
+                e[-1] = 1.0
+                tt = lu.solve(e)
+                tangent = tt / np.linalg.norm(tt)
+            N1 = len(x) - 1
+            dd = (xc[:N1] - predicted[:N1]).reshape(-1, 6)
+            vv = xc[:N1].reshape(-1, 6)
+            err = max(float(np.sqrt((dd * dd).sum(0) / (vv * vv).sum(0)).max()),
+                      abs(xc[-1] - predicted[-1]) / abs(xc[-1]))
+            trns = min(np.sqrt(eps_pr / err), 2.0)
+            dS = min(dS * trns, dS_max)
+        x_old = xc
+        x = xc + tangent * dS if k > 1 else xc.copy()
+        states.append(xc)
+    return states, dss

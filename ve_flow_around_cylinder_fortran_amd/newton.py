@@ -21,6 +21,7 @@ src/Physical_mod.F90:48-49), so this driver takes Oldroyd-B (VeN = 0) only.
 from __future__ import annotations
 
 import dataclasses
+import math
 import time
 from typing import List
 
@@ -122,18 +123,21 @@ class NewtonSolver:
         self.b = torch.empty(self.N, dtype=torch.float64, device=device)
         self.values = torch.empty(self.asm.nnz, dtype=torch.float64, device=device)
 
-    def solve(self, x0: np.ndarray, LN0: float, xF: float = 1.0, max_iter: int = NITER):
-        """Newton from x0 with EU pinned at LN0; returns (x as a device tensor,
-        list of NewtonRecord)."""
+    def solve(self, x0, LN0: float, xF: float = 1.0, max_iter: int = NITER, increment: int = 1, dArN: float = 0.0,
+              x_old=None, tangent=None, dS: float = 0.0):
+        """Newton from x0 (numpy or device tensor); returns (x as a device
+        tensor, list of NewtonRecord).  Increments 1-2 pin EU at
+        LN0 + (increment-1)*dArN; from increment 3 the pseudo-arclength row
+        tangent.(x - x_old) = dS closes the system (x_old, tangent: device
+        tensors).  The inflow values follow the current EU, as the reference
+        re-solves its inflow problem inside every arclength assembly."""
         torch = self.torch
-        WiN = self.params.WiN0 * LN0
-        rows, vals = cylinder_dirichlet(self.mesh, WiN)
-        self.asm.set_dirichlet(rows, vals)
-        x = torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
+        x = x0.clone() if torch.is_tensor(x0) else torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
         flag = "NRP"
         res_norm = cor_new = cor_old = 1.0
         it = 0
         hist: List[NewtonRecord] = []
+        last_EU = None
         while True:
             if cor_new < COR_NORM_EPS and res_norm < COR_NORM_EPS:
                 break
@@ -142,9 +146,14 @@ class NewtonSolver:
             if it % 10 == 0:
                 flag = "NRP"
             jac = flag == "NRP"
+            EU = float(x[-1])
+            if EU != last_EU:
+                rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * EU)
+                self.asm.set_dirichlet(rows, vals)
+                last_EU = EU
             self.asm.assemble(x, flag, b=self.b, values=self.values if jac else None)
-            self.asm.arclength(x, self.b, self.values if jac else None, increment=1, LN0=LN0, dArN=0.0,
-                               flag_nr=flag)
+            self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
+                               x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
             self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
             res_norm = float(torch.linalg.vector_norm(self.b))
             if jac:
@@ -164,4 +173,63 @@ class NewtonSolver:
             x = x - xF * s
             torch.cuda.synchronize()
             hist[-1].seconds = time.perf_counter() - t0
+        return x, hist
+
+
+class ArclengthContinuation:
+    """Arclength continuation in WiN = WiN0*EU (continuationStepArclength,
+    src/Continuation_Arclength_mod.F90:77-377): increments 1-2 pin EU at
+    LN0 + (increment-1)*dArN; then the direction is the secant (increment 2)
+    or the normalised solution of J t = e_N with the last factorised Jacobian
+    (increment >= 3, setDirectionVectors); the step dS is scaled by
+    sqrt(eps_pr/err_pr) (at most doubled, at most dS_max) from the relative
+    prediction error (adjustContinuationStep); the predictor is x + t*dS
+    (updateSolutionArclength)."""
+
+    def __init__(self, solver: NewtonSolver, LN0: float, dArN: float, dS0: float = 1.0, dS_max: float = 50.0,
+                 eps_pr: float = 1.0e-3):
+        self.s = solver
+        self.LN0, self.dArN, self.dS, self.dS_max, self.eps_pr = LN0, dArN, dS0, dS_max, eps_pr
+        self.increment = 0
+        self.x = None       # current predictor / state (device)
+        self.x_old = None   # last converged state
+        self.tangent = None
+
+    def start(self, x0):
+        torch = self.s.torch
+        self.x = torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.s.device)
+
+    def _field_errors(self, x, xp):
+        """max over the 6 fields and EU of ||x - xp|| / ||x|| (per field)."""
+        torch = self.s.torch
+        N1 = self.s.N - 1
+        d = (x[:N1] - xp[:N1]).view(-1, 6)
+        v = x[:N1].view(-1, 6)
+        e = torch.sqrt((d * d).sum(0) / (v * v).sum(0))
+        eu = abs(float(x[-1] - xp[-1])) / abs(float(x[-1]))
+        return max(float(e.max()), eu)
+
+    def step(self):
+        """One continuation step; returns (converged state, Newton records)."""
+        torch = self.s.torch
+        self.increment += 1
+        k = self.increment
+        predicted = self.x.clone()
+        x, hist = self.s.solve(self.x, self.LN0, increment=k, dArN=self.dArN, x_old=self.x_old,
+                               tangent=self.tangent, dS=self.dS)
+        if k > 1:
+            if k == 2:
+                d = x - self.x_old
+                self.tangent = d / torch.linalg.vector_norm(d)
+            else:
+                e = torch.zeros(self.s.N, dtype=torch.float64, device=self.s.device)
+                e[-1] = 1.0
+                t = self.s.lu.solve(e)
+                self.tangent = t / torch.linalg.vector_norm(t)
+            err = self._field_errors(x, predicted)
+            trns = math.sqrt(self.eps_pr / err) if err > 0 else 2.0
+            trns = min(trns, 2.0)
+            self.dS = min(self.dS * trns, self.dS_max)
+        self.x_old = x
+        self.x = x + self.tangent * self.dS if k > 1 else x.clone()
         return x, hist
