@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--elements", type=int, default=N_ELEMENTS)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--newton", action="store_true",
+                    help="measure seconds per converged arclength step (fixed-pattern GPU Newton + continuation on "
+                         "the 1,600-element mesh) instead of the assembly throughput")
+    ap.add_argument("--arc-steps", type=int, default=10, help="continuation steps for --newton (2 pinned + the rest)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference launch for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -64,6 +68,81 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}), flush=True)
+
+
+def newton_bench(args):
+    """Arclength continuation in Wi on the 1,600-element mesh: two pinned steps
+    (Wi = 0.1, 0.12), then pseudo-arclength steps; value = mean wall time of a
+    converged pseudo-arclength step (assembly, arclength and Dirichlet rows,
+    GPU re-factorisation and solves, tangent, predictor), the state resident on
+    the GPU.  The CPU baseline runs the same continuation with the oracle's
+    assembly and SuperLU on the host cores."""
+    import numpy as np
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd import Params, workload_mesh
+    from ve_flow_around_cylinder_fortran_amd.mesh import boundary_edges
+    from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation, NewtonSolver, initial_state
+
+    pset = dict(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1.0e-3, WiN0=1.0, model="L-PTT")
+    LN0, dArN = 0.1, 0.02
+    mesh = workload_mesh(1600)
+    x0 = initial_state(mesh, LN0)
+    solver = NewtonSolver(mesh, Params(**pset))
+    cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN)
+    cont.start(x0)
+    times, iters, EU = [], [], []
+    for _ in range(args.arc_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, hist = cont.step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        iters.append("".join(h.flag[0] for h in hist))
+        EU.append(float(x[-1]))
+    solver.asm.set_drag_edges(boundary_edges(mesh, "Cylinder"))
+    Fd = float(solver.asm.drag(x).item())
+    arc = times[2:]
+    out = {
+        "metric": "s per converged arclength step",
+        "value": float(np.mean(arc)),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": len(arc),
+        "warmup": 2,
+        "ms_per_step": float(np.mean(arc)) * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "cylinder-in-channel Q2 mesh, 1600 elements, Oldroyd-B beta=0.59 Re=0 DfN=1e-3, arclength "
+                        f"continuation in Wi from {LN0} (pinned steps {LN0}, {LN0 + dArN}), Newton tolerance 1e-8 on "
+                        "||b|| and ||s||",
+            "elements": mesh.n_elements, "unknowns": mesh.n_unknowns,
+            "step_seconds": times, "newton_flags_per_step": iters, "EU_per_step": EU,
+            "factor_pattern_nnz": solver.lu.nnzT, "host_fallbacks": solver.lu.host_fallbacks,
+            "Fd_last_step": Fd,
+        },
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        from newton_cpu import continuation_cpu
+
+        threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1), 16))
+        n_cpu = min(args.arc_steps, 6)
+        cpu_times = []
+        states, _ = continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, n_cpu, nthreads=threads, times=cpu_times)
+        cpu_arc = float(np.mean(cpu_times[2:]))
+        out["cpu_baseline"] = {"value": cpu_arc, "unit": "s", "cores": threads, "kind": "port",
+                               "sample": f"the same continuation, steps 3..{n_cpu}: oracle assembly on {threads} "
+                                         "threads, SuperLU (minimum degree on A+A^T) on the host"}
+        out["config"]["EU_cpu_last"] = float(states[-1][-1])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(mesh, x, target_s):
@@ -109,6 +188,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.newton:
+        newton_bench(args)
         return
 
     import numpy as np  # noqa: F401

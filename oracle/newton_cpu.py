@@ -108,17 +108,21 @@ def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent,
     return x, hist, lu
 
 
-def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=50.0, eps_pr=1.0e-3):
+def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=50.0, eps_pr=1.0e-3, nthreads=8,
+                     times=None):
     """continuationStepArclength restated with newton_cpu_arc; returns the
-    converged states and the step sizes used."""
+    converged states and the step sizes used (and appends each step's wall
+    time to `times` when given)."""
+    import time
     x = np.array(x0, dtype=np.float64)
     x_old = tangent = None
     dS = dS0
     states, dss = [], []
     for k in range(1, steps + 1):
+        t0 = time.perf_counter()
         predicted = x.copy()
         dss.append(dS)
-        xc, hist, lu = newton_cpu_arc(oracle, mesh, pset, x, LN0, k, dArN, x_old, tangent, dS)
+        xc, hist, lu = newton_cpu_arc(oracle, mesh, pset, x, LN0, k, dArN, x_old, tangent, dS, nthreads=nthreads)
         if k > 1:
             if k == 2:
                 d = xc - x_old
@@ -138,4 +142,6 @@ def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=5
         x_old = xc
         x = xc + tangent * dS if k > 1 else xc.copy()
         states.append(xc)
+        if times is not None:
+            times.append(time.perf_counter() - t0)
     return states, dss
