@@ -103,6 +103,25 @@ def newton_bench(args):
     solver.asm.set_drag_edges(boundary_edges(mesh, "Cylinder"))
     Fd = float(solver.asm.drag(x).item())
     arc = times[2:]
+    # the same continuation with the learned predictor (per-node MLP on the matrix cores, trained online)
+    from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
+
+    lcont = ArclengthContinuation(NewtonSolver(mesh, Params(**pset)), LN0=LN0, dArN=dArN,
+                                  predictor=LearnedPredictor(mesh, seed=0))
+    lcont.start(x0)
+    ltimes, liters, lEU = [], [], []
+    for _ in range(args.arc_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xl, hist = lcont.step()
+        torch.cuda.synchronize()
+        ltimes.append(time.perf_counter() - t0)
+        liters.append("".join(h.flag[0] for h in hist))
+        lEU.append(float(xl[-1]))
+    learned = {"step_seconds": ltimes, "newton_flags_per_step": liters, "EU_per_step": lEU,
+               "choices": lcont.choices,
+               "Wi_gain_per_second": (lEU[-1] - lEU[1]) * pset["WiN0"] / sum(ltimes[2:]),
+               "tangent_Wi_gain_per_second": (EU[-1] - EU[1]) * pset["WiN0"] / sum(arc)}
     out = {
         "metric": "s per converged arclength step",
         "value": float(np.mean(arc)),
@@ -124,6 +143,7 @@ def newton_bench(args):
             "step_seconds": times, "newton_flags_per_step": iters, "EU_per_step": EU,
             "factor_pattern_nnz": solver.lu.nnzT, "host_fallbacks": solver.lu.host_fallbacks,
             "Fd_last_step": Fd,
+            "learned_predictor": learned,
         },
     }
     if not args.no_cpu_baseline:

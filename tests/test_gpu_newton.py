@@ -111,3 +111,36 @@ def test_arclength_continuation_matches_cpu(oracle):
         assert abs(g[-1] - c[-1]) <= 2e-8
         assert np.abs(g - c).max() <= 1e-7 * np.abs(c).max()
     assert gpu_states[-1][-1] > gpu_states[2][-1] > 0.12
+
+
+def _continue_to(mesh, target_EU, predictor, max_steps=25):
+    from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation
+
+    cont = ArclengthContinuation(NewtonSolver(mesh, Params(**OLDROYD_B)), LN0=0.1, dArN=0.02, predictor=predictor)
+    cont.start(initial_state(mesh, 0.1))
+    iters = 0
+    for k in range(max_steps):
+        x, hist = cont.step()
+        if k >= 2:
+            iters += len(hist)
+        if float(x[-1]) >= target_EU:
+            return k + 1, iters, cont
+    raise AssertionError(f"EU {float(x[-1])} below {target_EU} after {max_steps} steps")
+
+
+def test_learned_predictor_gains_over_tangent():
+    """From Wi = 0.1 to Wi = 0.145 on the small mesh: with the online-trained
+    per-node correction of the tangent predictor the continuation needs fewer
+    steps and fewer Newton iterations than with the tangent predictor alone
+    (the learned proposal is taken only where its residual is smaller)."""
+    from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    print("[gpu newton] continuation to Wi 0.145, tangent predictor", flush=True)
+    steps_t, iters_t, _ = _continue_to(mesh, 0.145, None)
+    print("[gpu newton] continuation to Wi 0.145, learned predictor", flush=True)
+    steps_l, iters_l, cont = _continue_to(mesh, 0.145, LearnedPredictor(mesh, seed=0))
+    print(f"[gpu newton] tangent: {steps_t} steps, {iters_t} Newton iterations; learned: {steps_l} steps, "
+          f"{iters_l} iterations; choices {cont.choices}", flush=True)
+    assert "learned" in cont.choices
+    assert steps_l < steps_t and iters_l < iters_t

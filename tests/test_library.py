@@ -14,8 +14,8 @@ from ve_flow_around_cylinder_fortran_amd import assembly
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    text = open(os.path.join(ROOT, "include", "vef.h")).read()
+def declared_functions(header="vef.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(vef_[a-z_]+)\s*\(", text)))
 
@@ -26,6 +26,20 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 12
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_mlp_library_exports_every_declared_symbol():
+    from ve_flow_around_cylinder_fortran_amd import learned
+
+    lib = learned.load_library()
+    names = declared_functions("vef_mlp.h")
+    assert len(names) == 9
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_every_header_is_checked():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["vef.h", "vef_mlp.h"]
 
 
 def test_fortran_binding_names_match_header():

@@ -1,6 +1,7 @@
 """Build steps of the package (run by __graft_entry__.build()).
 
-* libvef.so: the HIP assembler, hipcc for gfx950, in-tree.  FP64 results must
+* libvef.so: the HIP assembler, and libvef_mlp.so: the per-node MLP of the
+  learned predictor; hipcc for gfx950, in-tree.  FP64 results must
   follow the source's operation order, so contraction into FMA is off
   (-ffp-contract=off) and no fast-math is used; packed FP32 instructions are
   disabled for all HIP code.
@@ -19,6 +20,8 @@ ROOT = os.path.dirname(PKG)
 INCLUDE = os.path.join(ROOT, "include")
 HIP_SRC = os.path.join(PKG, "csrc", "vef_assembly.hip")
 LIB = os.path.join(PKG, "libvef.so")
+MLP_SRC = os.path.join(PKG, "csrc", "vef_mlp.hip")
+MLP_LIB = os.path.join(PKG, "libvef_mlp.so")
 FORTRAN_DIR = os.path.join(PKG, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
 
@@ -43,12 +46,13 @@ def hipcc() -> str:
 
 
 def build_hip(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, HIP_SRC]
-    if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
-    os.replace(tmp, LIB)
+    for src, lib in ((HIP_SRC, LIB), (MLP_SRC, MLP_LIB)):
+        tmp = lib + ".tmp"
+        cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, src]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        os.replace(tmp, lib)
     return LIB
 
 

@@ -175,6 +175,19 @@ class NewtonSolver:
             hist[-1].seconds = time.perf_counter() - t0
         return x, hist
 
+    def residual_norm(self, x, LN0: float, increment: int, dArN: float = 0.0, x_old=None, tangent=None,
+                      dS: float = 0.0) -> float:
+        """||b|| of the step's system at x (residual, arclength and Dirichlet
+        rows; no Jacobian): how far a predicted state is from the solution."""
+        torch = self.torch
+        rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * float(x[-1]))
+        self.asm.set_dirichlet(rows, vals)
+        self.asm.assemble(x, "MNR", b=self.b)
+        self.asm.arclength(x, self.b, None, increment=increment, LN0=LN0, dArN=dArN, x_old=x_old, tangent=tangent,
+                           dS=dS, flag_nr="MNR")
+        self.asm.apply_dirichlet(x, self.b, None, "MNR")
+        return float(torch.linalg.vector_norm(self.b))
+
 
 class ArclengthContinuation:
     """Arclength continuation in WiN = WiN0*EU (continuationStepArclength,
@@ -187,13 +200,17 @@ class ArclengthContinuation:
     (updateSolutionArclength)."""
 
     def __init__(self, solver: NewtonSolver, LN0: float, dArN: float, dS0: float = 1.0, dS_max: float = 50.0,
-                 eps_pr: float = 1.0e-3):
+                 eps_pr: float = 1.0e-3, predictor=None):
         self.s = solver
         self.LN0, self.dArN, self.dS, self.dS_max, self.eps_pr = LN0, dArN, dS0, dS_max, eps_pr
         self.increment = 0
         self.x = None       # current predictor / state (device)
         self.x_old = None   # last converged state
         self.tangent = None
+        self.predictor = predictor  # optional learned predictor (learned.LearnedPredictor)
+        self.choices = []   # per step: "tangent" or "learned" for the next prediction
+        self.residuals = []  # per learned proposal: (||b|| at the tangent prediction, at the proposal)
+        self._pending = None  # (features, tangent prediction) of the last step
 
     def start(self, x0):
         torch = self.s.torch
@@ -232,4 +249,21 @@ class ArclengthContinuation:
             self.dS = min(self.dS * trns, self.dS_max)
         self.x_old = x
         self.x = x + self.tangent * self.dS if k > 1 else x.clone()
+        choice = "tangent"
+        if self.predictor is not None and k >= 2:
+            if self._pending is not None:
+                # the error the previous tangent prediction made
+                self.predictor.observe(self._pending[0], x - self._pending[1])
+            X = self.predictor.features(x, self.x)
+            self._pending = (X, self.x.clone())
+            if self.predictor.ready():
+                cand = self.predictor.propose(X, self.x, self.tangent)
+                nxt = k + 1
+                r_t = self.s.residual_norm(self.x, self.LN0, nxt, self.dArN, self.x_old, self.tangent, self.dS)
+                r_l = self.s.residual_norm(cand, self.LN0, nxt, self.dArN, self.x_old, self.tangent, self.dS)
+                self.residuals.append((r_t, r_l))
+                if r_l < r_t:
+                    self.x = cand
+                    choice = "learned"
+        self.choices.append(choice)
         return x, hist
