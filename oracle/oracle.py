@@ -43,6 +43,7 @@ def lib(flops: bool = False):
         vp = ctypes.c_void_p
         if flops:
             L.vo_count_flops.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, vp]
+            L.vo_count_flops_stability.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, vp, vp]
         else:
             L.vo_tables.argtypes = [vp]
             L.vo_basis2d.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -62,6 +63,16 @@ def lib(flops: bool = False):
             L.vo_assemble.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, vp,
                                       vp, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]
+            L.vo_stab_plant.argtypes = [ctypes.c_int]
+            L.vo_stab_element.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, vp, vp]
+            L.vo_stab_residual.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, vp, vp, vp]
+            L.vo_stab_linearization.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, vp, ctypes.c_int,
+                                                ctypes.c_double, vp, vp]
+            L.vo_stab_pattern_nnz.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+            L.vo_stab_pattern_nnz.restype = ctypes.c_int64
+            L.vo_stab_pattern.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp]
+            L.vo_stab_assemble.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
+                                           vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _libs[name] = L
     return _libs[name]
 
@@ -196,3 +207,91 @@ def nodal_stress(BvN, WiN, n_nodes, x):
     out = np.zeros((n_nodes, 4))
     lib().vo_nodal_stress(float(BvN), float(WiN), n_nodes, _p(x), _p(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Linear stability assembly (Cylinder_Stability_2D, vef_oracle.cpp's f3 part).
+# ---------------------------------------------------------------------------
+def _stab_rc(rc, what):
+    if rc == -2:
+        raise ValueError(f"{what}: the stability assembly needs DfN > 1e-12 (discontinuity capturing off)")
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc})")
+
+
+def stab_plant(which: int):
+    """Plants a known mistake in the stability restatement (0 removes it)."""
+    lib().vo_stab_plant(int(which))
+
+
+def stab_element(prm, xy, TLb, EU):
+    """A, B [54 columns (inod,ieq)][54 rows (jnod,jeq)] of storeBulkStability."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    TLb = np.ascontiguousarray(TLb, dtype=np.float64)
+    A = np.zeros((54, 54))
+    B = np.zeros((54, 54))
+    _stab_rc(lib().vo_stab_element(ctypes.byref(prm), _p(xy), _p(TLb), float(EU), _p(A), _p(B)), "vo_stab_element")
+    return A, B
+
+
+def stab_residual(prm, xy, TLb, EU, TLd):
+    """Res_a[9][6], Res_b[9][6] of residualBulkStability for the disturbance TLd."""
+    xy, TLb, TLd = (np.ascontiguousarray(a, dtype=np.float64) for a in (xy, TLb, TLd))
+    RA = np.zeros((9, 6))
+    RB = np.zeros((9, 6))
+    _stab_rc(lib().vo_stab_residual(ctypes.byref(prm), _p(xy), _p(TLb), float(EU), _p(TLd), _p(RA), _p(RB)),
+             "vo_stab_residual")
+    return RA, RB
+
+
+def stab_linearization(prm, xy, TLb, EU, TLd, gp, h):
+    """(stability test-node terms, central difference of the frozen-coefficient
+    base terms) at bulk Gauss point gp, each [9][6]."""
+    xy, TLb, TLd = (np.ascontiguousarray(a, dtype=np.float64) for a in (xy, TLb, TLd))
+    st = np.zeros((9, 6))
+    fd = np.zeros((9, 6))
+    _stab_rc(lib().vo_stab_linearization(ctypes.byref(prm), _p(xy), _p(TLb), float(EU), _p(TLd), int(gp), float(h),
+                                         _p(st), _p(fd)), "vo_stab_linearization")
+    return st, fd
+
+
+def stab_pattern(mesh):
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    nnz = lib().vo_stab_pattern_nnz(mesh.n_nodes, mesh.n_elements, _p(conn))
+    rowptr = np.zeros(mesh.n_nodes * 6 + 1, dtype=np.int64)
+    col = np.zeros(nnz, dtype=np.int32)
+    lib().vo_stab_pattern(mesh.n_nodes, mesh.n_elements, _p(conn), _p(rowptr), _p(col))
+    return rowptr, col
+
+
+def stab_assemble(prm, mesh, x, rowptr=None, col=None, el_range=None, nthreads=0):
+    """setEquationsBulkStability in the reference's order: (A values, B values)."""
+    if rowptr is None:
+        rowptr, col = stab_pattern(mesh)
+    coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    A = np.zeros(int(rowptr[-1]))
+    B = np.zeros(int(rowptr[-1]))
+    e0, e1 = (0, mesh.n_elements) if el_range is None else el_range
+    _stab_rc(lib().vo_stab_assemble(ctypes.byref(prm), mesh.n_nodes, mesh.n_elements, _p(coords), _p(conn), _p(x),
+                                    _p(rowptr), _p(col), _p(A), _p(B), e0, e1, nthreads), "vo_stab_assemble")
+    return A, B
+
+
+def stab_dirichlet(rowptr, col, A, B, rows):
+    """applyDirichletBCStability (src/Dirichlet_mod.F90:287-316): MatZeroRows
+    of A with diagonal 1 and of B with diagonal 0, in place."""
+    for r in np.unique(np.asarray(rows, dtype=np.int64)):
+        s, e = rowptr[r], rowptr[r + 1]
+        A[s:e] = 0.0
+        B[s:e] = 0.0
+        A[s:e][col[s:e] == r] = 1.0
+
+
+def count_flops_stability(prm, xy, TLb, EU, TLd):
+    out = np.zeros(10, dtype=np.uint64)
+    xy, TLb, TLd = (np.ascontiguousarray(a, dtype=np.float64) for a in (xy, TLb, TLd))
+    lib(flops=True).vo_count_flops_stability(ctypes.byref(prm), _p(xy), _p(TLb), float(EU), _p(TLd), _p(out))
+    keys = ("add", "mul", "div", "sqrt", "total")
+    return dict(zip(keys, map(int, out[:5]))), dict(zip(keys, map(int, out[5:])))

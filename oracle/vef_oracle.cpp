@@ -839,6 +839,395 @@ static int scatter_element(const int32_t* nodes, const double* Res54, const doub
   return 0;
 }
 
+// ----------------------------------------------------------------------------
+// f3: linear stability assembly of Cylinder_Stability_2D (Neq = 6, Nex = 0)
+// with Stress_Reform = 'SQRT'.
+//   assembleSlepcObjects      src/Stability/Stability_mod.F90:97-199
+//   setEquationsBulkStability src/Stability/Equations_Stability_mod.F90:20-57
+//   residualBulkStability     :61-96, equationsResidualsBulkStability :98-131
+//   storeBulkStability        :135-226
+//   setFlowQuantitiesStability, setFlowUnknownsStability,
+//   strongFormMomentumStability src/Stability/Element_Calculations_Stability_mod.F90:19-102,186-262,302-329
+//   strongFormConEQ_Stability_SQRT and helpers
+//                             src/Stability/Constitutive_Models_SQRT_Stability_mod.F90:87-487
+//   momentumStabilitySQRT, continuityStability, stressStabilitySQRT
+//                             src/Stability/Residuals_Stability_mod.F90:125-153,209-313
+// The base state b is TEMP_TL_b = TL (real), the disturbance d is TEMP_TL_d
+// (complex; zero in storeBulkStability's base evaluation, Stability_mod.F90:58-62).
+// Every disturbance value has a zero imaginary part on this path (the
+// spanwise terms are commented out, Element_Calculations_Stability_mod.F90:51-81,
+// and eps_jac is the real part of perturbVariable's 1e-8+1e-8i, assigned to a
+// PetscReal, Equations_Stability_mod.F90:155,178), so the real restatement
+// with the division rule cdiv() reproduces the reference's values.
+// A (Res_a) and B (Res_b, negated) are forward differences in the
+// disturbance.  The discontinuity-capturing terms multiply tdcs = 0 (DfN >
+// 1e-12 is required here), so they add signed zeros only and are skipped.
+// ----------------------------------------------------------------------------
+// Planted mistakes for the tests that show the linearisation check catches
+// them (0: none; 1: sign of n_b*d_d in W_d; 2: d.U_f.grad(S_b) left out of UCD_d).
+static int g_stab_plant = 0;
+
+struct GsPtD {
+  vreal U_f[3], GU[3][3], GUT[3][3], G_dot[3][3], G2U[3][3], dPdXi[3], P;
+  vreal S[3][3], dSdXi[3][3][3];
+  vreal C[3][3], dCdXi[3][3][3], f[3], g[2], dgdxi[2][3], f_T;
+  vreal T_ve[3][3], P_tot[3][3], dTdXi[3][3][3];
+  vreal Con_EQ[3][3], Con_EQ_time[3][3];
+  vreal Convection[3], Momentum[3], Momentum_time[3], Continuity;
+};
+
+// setFlowUnknownsStabilityComplex (Element_Calculations_Stability_mod.F90:186-262)
+// for the disturbance nodal values TLd[9][6].
+static void setFlowUnknownsD(const Elem& E, const vreal TLd[9][6], GsPtD& d) {
+  auto dot9 = [&](int ieq, const vreal* v) {
+    vreal s = Z();
+    for (int n = 0; n < 9; n++) s = s + TLd[n][ieq] * v[n];
+    return s;
+  };
+  auto dot9d = [&](int ieq, const double* v) {
+    vreal s = Z();
+    for (int n = 0; n < 9; n++) s = s + TLd[n][ieq] * vreal(v[n]);
+    return s;
+  };
+  for (int i = 0; i < 3; i++) d.U_f[i] = Z(), d.dPdXi[i] = Z();
+  zero33(d.GU);
+  zero33(d.G2U);
+  for (int i = 0; i < 2; i++) {
+    d.U_f[i] = dot9d(i, E.bfn);
+    for (int j = 0; j < 3; j++) {
+      d.GU[j][i] = dot9(i, E.dFdXi[j]);
+      d.G2U[j][i] = dot9(i, E.d2FdXi2[j]);
+    }
+    d.dPdXi[i] = dot9(2, E.dFdXi[i]);
+  }
+  d.P = dot9d(2, E.bfn);
+  zero33(d.S);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      for (int c = 0; c < 3; c++) d.dSdXi[a][b][c] = Z();
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  for (int s = 0; s < 3; s++) {
+    int j = comp[s][0], k = comp[s][1];
+    d.S[j][k] = dot9d(3 + s, E.bfn);
+    d.S[k][j] = d.S[j][k];
+    for (int c = 0; c < 3; c++) d.dSdXi[j][k][c] = dot9(3 + s, E.dFdXi[c]);
+    for (int c = 0; c < 3; c++) d.dSdXi[k][j][c] = d.dSdXi[j][k][c];
+  }
+}
+
+// setFlowKinematics (Element_Calculations_mod.F90:605-634) for the disturbance:
+// GUT and G_dot are what the stability terms read.
+static void setFlowKinematicsD(GsPtD& d) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.GUT[i][j] = d.GU[j][i];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.G_dot[i][j] = d.GU[i][j] + d.GUT[i][j];
+}
+
+// omegaNominator2D (Constitutive_Models_SQRT_mod.F90:540-584) on the leading
+// 2x2 blocks of S and GUT.
+static vreal omegaNom(const vreal S[3][3], const vreal GUT[3][3]) {
+  return -(S[0][0] * GUT[1][0]) + S[0][1] * (GUT[0][0] - GUT[1][1]) + S[1][1] * GUT[0][1];
+}
+
+// strongFormConEQ_Stability_SQRT (Constitutive_Models_SQRT_Stability_mod.F90:87-125)
+// with setCTensorStabilitySQRT :129-147, conEQ_L_PTT_Stability_SQRT :151-175,
+// conEQ_Giesekus_Stability_SQRT :276-297, setStressTensorStabilitySQRT :431-459,
+// setUCDStabilitySQRT :376-413, WTensor2DStability :463-486,
+// setSInverseStability :417-427.
+static void strongFormConEQStability(const Phys& ph, const GsPt& b, GsPtD& d, const vreal I[3][3]) {
+  vreal M1[3][3], M2[3][3], M3[3][3], M4[3][3];
+  matmul33(b.S, d.S, M1);
+  matmul33(d.S, b.S, M2);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.C[i][j] = M1[i][j] + M2[i][j];
+  for (int k = 0; k < 3; k++) {
+    vreal bdS[3][3], ddS[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) bdS[i][j] = b.dSdXi[i][j][k], ddS[i][j] = d.dSdXi[i][j][k];
+    matmul33(bdS, d.S, M1);
+    matmul33(b.S, ddS, M2);
+    matmul33(ddS, b.S, M3);
+    matmul33(d.S, bdS, M4);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) d.dCdXi[i][j][k] = M1[i][j] + M2[i][j] + M3[i][j] + M4[i][j];
+  }
+  if (ph.model == 0) {
+    vreal trC_d = trace33(d.C);
+    d.f_T = ph.VeN * trC_d;
+    d.f[0] = d.f_T;
+    d.f[1] = d.f[0];
+    d.f[2] = Z();
+  } else {
+    d.f_T = Z();
+    d.f[0] = Z();
+    d.f[1] = Z();
+    d.f[2] = Z();
+  }
+  d.g[0] = Z();
+  d.g[1] = Z();
+  for (int c = 0; c < 3; c++) d.dgdxi[0][c] = Z(), d.dgdxi[1][c] = Z();
+
+  const vreal omb = vreal(1.0) - ph.BvN;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.T_ve[i][j] = b.g[1] * d.C[i][j] + d.g[1] * b.C[i][j] - d.g[0] * I[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.T_ve[i][j] = cdiv(omb * d.T_ve[i][j], b.WiN);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.P_tot[i][j] = d.T_ve[i][j] + ph.BvN * d.G_dot[i][j] - d.P * I[i][j];
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        d.dTdXi[i][j][k] = b.dgdxi[1][k] * d.C[i][j] + d.dgdxi[1][k] * b.C[i][j] + b.g[1] * d.dCdXi[i][j][k] +
+                           d.g[1] * b.dCdXi[i][j][k] - d.dgdxi[0][k] * I[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) d.dTdXi[i][j][k] = cdiv(omb * d.dTdXi[i][j][k], b.WiN);
+
+  // UCD_d
+  vreal UCD[3][3];
+  matmul33(b.S, d.GU, M1);
+  matmul33(d.S, b.GU, M2);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) UCD[i][j] = -M1[i][j] - M2[i][j];
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        UCD[i][j] = g_stab_plant == 2 ? UCD[i][j] + b.U_f[k] * d.dSdXi[i][j][k]
+                                      : UCD[i][j] + b.U_f[k] * d.dSdXi[i][j][k] + d.U_f[k] * b.dSdXi[i][j][k];
+  vreal W_b[3][3], W_d[3][3];
+  zero33(W_b);
+  W_b[0][1] = omega12(b.S, b.GUT_tr);  // setOmegaTensor(GsPt_b)
+  W_b[1][0] = -W_b[0][1];
+  {
+    vreal n_b = omegaNom(b.S, b.GUT);
+    vreal d_b = b.S[0][0] + b.S[1][1];
+    vreal n_d = omegaNom(b.S, d.GUT) + omegaNom(d.S, b.GUT);
+    vreal d_d = d.S[0][0] + d.S[1][1];
+    zero33(W_d);
+    W_d[0][1] = cdiv(g_stab_plant == 1 ? n_d * d_b + n_b * d_d : n_d * d_b - n_b * d_d, d_b * d_b);
+    W_d[1][0] = -W_d[0][1];
+  }
+  matmul33(W_b, d.S, M1);
+  matmul33(W_d, b.S, M2);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) UCD[i][j] = UCD[i][j] - M1[i][j] - M2[i][j];
+
+  // SI_b = setSInverse(real(S_b)) (inverseTensor2D, real divisions); SI_d
+  vreal SI_b[3][3], SI_d[3][3];
+  zero33(SI_b);
+  {
+    vreal detA = b.S[0][0] * b.S[1][1] - b.S[0][1] * b.S[1][0];
+    SI_b[0][0] = b.S[1][1] / detA;
+    SI_b[0][1] = -b.S[0][1] / detA;
+    SI_b[1][0] = -b.S[1][0] / detA;
+    SI_b[1][1] = b.S[0][0] / detA;
+  }
+  matmul33(d.S, SI_b, M1);
+  matmul33(SI_b, M1, M2);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) SI_d[i][j] = -M2[i][j];
+  // S3_b = S.(S.S); S3_d = (S.S).S_d + (S.S_d).S + S_d.(S.S)
+  vreal SS[3][3], S3_b[3][3], S3_d[3][3];
+  matmul33(b.S, b.S, SS);
+  matmul33(b.S, SS, S3_b);
+  matmul33(SS, d.S, M1);
+  matmul33(b.S, d.S, M3);
+  matmul33(M3, b.S, M2);
+  matmul33(d.S, SS, M4);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) S3_d[i][j] = M1[i][j] + M2[i][j] + M4[i][j];
+  const vreal two = vreal(2.0);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      vreal tS = b.f[2] * S3_d[i][j] + d.f[2] * S3_b[i][j] + d.f[1] * b.S[i][j] + b.f[1] * d.S[i][j] -
+                 b.f[0] * SI_d[i][j] - d.f[0] * SI_b[i][j];
+      d.Con_EQ[i][j] = b.WiN * UCD[i][j] + cdiv(tS, two);
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) d.Con_EQ_time[i][j] = b.WiN * d.S[i][j];
+}
+
+// strongFormMomentumStability (Element_Calculations_Stability_mod.F90:302-329)
+// and strongFormContinuity (Element_Calculations_mod.F90:687-697).
+static void strongFormMomentumStability(const Phys& ph, const GsPt& b, GsPtD& d) {
+  vreal div_Tve[3], div_Ptot[3];
+  for (int i = 0; i < 3; i++) {
+    div_Tve[i] = Z();
+    for (int j = 0; j < 3; j++) div_Tve[i] = div_Tve[i] + d.dTdXi[j][i][j];
+  }
+  for (int i = 0; i < 3; i++)
+    div_Ptot[i] = -d.dPdXi[i] + div_Tve[i] + ph.BvN * (d.G2U[0][i] + d.G2U[1][i] + d.G2U[2][i]);
+  for (int j = 0; j < 3; j++) {
+    vreal m1 = Z(), m2 = Z();
+    for (int k = 0; k < 3; k++) m1 = m1 + b.U_f[k] * d.GU[k][j];
+    for (int k = 0; k < 3; k++) m2 = m2 + d.U_f[k] * b.GU[k][j];
+    d.Convection[j] = m1 + m2;
+  }
+  for (int i = 0; i < 3; i++) d.Momentum[i] = ph.ReN * d.Convection[i] - div_Ptot[i];
+  for (int i = 0; i < 3; i++) d.Momentum_time[i] = ph.ReN * d.U_f[i];
+  d.Continuity = trace33(d.GU);
+}
+
+// setFlowQuantitiesStability's disturbance part for a given base Gauss point.
+static void setFlowQuantitiesD(const Phys& ph, const Elem& E, const vreal TLd[9][6], const GsPt& b, GsPtD& d,
+                               const vreal I[3][3]) {
+  setFlowUnknownsD(E, TLd, d);
+  setFlowKinematicsD(d);
+  strongFormConEQStability(ph, b, d, I);
+  strongFormMomentumStability(ph, b, d);
+}
+
+// LSCEMomentumSQRT (Residuals_mod.F90:388-433) of a Gauss point's quantities.
+static void lsceMomentum(const Phys& ph, const GsPt& G, int i, const vreal dF[3], vreal lsce[3][3]) {
+  vreal GUT_l[3][3], GU_l[3][3];
+  zero33(GUT_l);
+  for (int k = 0; k < 3; k++) GUT_l[i][k] = dF[k];
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) GU_l[a][c] = GUT_l[c][a];
+  vreal W_l[3][3];
+  zero33(W_l);
+  W_l[0][1] = omega12(G.S, GUT_l);
+  W_l[1][0] = -W_l[0][1];
+  vreal M1[3][3], M2[3][3];
+  matmul33(G.S, GU_l, M1);
+  matmul33(W_l, G.S, M2);
+  const vreal omb = vreal(1.0) - ph.BvN;
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) {
+      vreal U = -M1[a][c] - M2[a][c];
+      lsce[a][c] = cdiv(G.tlsce * U * omb, G.WiN);
+    }
+}
+
+// momentumStabilitySQRT, continuityStability, stressStabilitySQRT
+// (Residuals_Stability_mod.F90:125-153,209-313) at test node n; TB already
+// negated (equationsResidualsBulkStability, Equations_Stability_mod.F90:123).
+static void nodeResidualStability(const Phys& ph, const Elem& E, const GsPt& b, const GsPtD& d, int n, vreal TA[6],
+                                  vreal TB[6]) {
+  vreal bifn = vreal(E.bfn[n]);
+  vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
+  for (int i = 0; i < 2; i++) {
+    vreal lsme_term = b.tlsme * ph.ReN * dot3(b.U_f, dF);  // LSMEMomentum(dFdXi, GsPt_b)
+    vreal lsic_term = b.tlsic * dF[i];
+    lsic_term = lsic_term * d.Continuity;
+    vreal lsce[3][3];
+    lsceMomentum(ph, b, i, dF, lsce);
+    vreal lsce_term = ddot33(lsce, d.Con_EQ);
+    vreal Prow[3] = {d.P_tot[i][0], d.P_tot[i][1], d.P_tot[i][2]};
+    TA[i] = ph.ReN * d.Convection[i] * bifn + dot3(Prow, dF) + lsme_term * d.Momentum[i] + lsic_term + lsce_term;
+    lsce_term = ddot33(lsce, d.Con_EQ_time);
+    TB[i] = d.Momentum_time[i] * bifn + lsme_term * d.Momentum_time[i] + lsce_term;
+  }
+  TA[2] = d.Continuity * bifn + b.tlsme * dot3(dF, d.Momentum);
+  TB[2] = b.tlsme * dot3(dF, d.Momentum_time);
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  vreal lsce_s = b.WiN * dot3(b.U_f, dF);  // LSCEStressSQRT(bifn, dFdXi, GsPt_b)
+  lsce_s = lsce_s + cdiv(b.f_T * bifn, vreal(2.0));
+  lsce_s = b.tlsce * lsce_s;
+  for (int s = 0; s < 3; s++) {
+    int j = comp[s][0], k = comp[s][1];
+    vreal lsme = -(dF[k] * d.Momentum[j]);  // LSMEStressSQRT(j,k,dFdXi,tlsme_b,Momentum_d)
+    if (j != k) lsme = lsme - dF[j] * d.Momentum[k];
+    lsme = b.tlsme * lsme * b.WiN;
+    vreal dS[3] = {d.dSdXi[j][k][0], d.dSdXi[j][k][1], d.dSdXi[j][k][2]};
+    TA[3 + s] = d.Con_EQ[j][k] * bifn + ph.DfN * dot3(dS, dF) + lsme + lsce_s * d.Con_EQ[j][k];
+    vreal lsme_t = -(dF[k] * d.Momentum_time[j]);
+    if (j != k) lsme_t = lsme_t - dF[j] * d.Momentum_time[k];
+    lsme_t = b.tlsme * lsme_t * b.WiN;
+    TB[3 + s] = d.Con_EQ_time[j][k] * bifn + lsme_t + lsce_s * d.Con_EQ_time[j][k];
+  }
+  for (int q = 0; q < 6; q++) TB[q] = -TB[q];
+}
+
+static void setGaussPoint(Elem& E, const Tables& T, int g) {
+  E.bfn = T.bfn[g];
+  E.dfdc = T.dfdc[g];
+  E.dfde = T.dfde[g];
+  E.d2fdc2 = T.d2fdc2[g];
+  E.d2fde2 = T.d2fde2[g];
+  E.d2fdce = T.d2fdce[g];
+}
+
+// residualBulkStability (Equations_Stability_mod.F90:61-96); E.TL is the base.
+static void residualBulkStability(const Phys& ph, Elem& E, const vreal TLd[9][6], vreal ResA[9][6], vreal ResB[9][6]) {
+  const Tables& T = tables();
+  vreal I[3][3];
+  identity(I);
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) ResA[n][q] = Z(), ResB[n][q] = Z();
+  for (int g = 0; g < 9; g++) {
+    setGaussPoint(E, T, g);
+    GsPt b;
+    basis2D(E, b);
+    setFlowQuantities(ph, E, b, I);
+    GsPtD d;
+    setFlowQuantitiesD(ph, E, TLd, b, d, I);
+    b.WET = vreal(T.W2[g]) * fabsr(b.detJ);
+    for (int n = 0; n < 9; n++) {
+      vreal TA[6], TB[6];
+      nodeResidualStability(ph, E, b, d, n, TA, TB);
+      for (int q = 0; q < 6; q++) {
+        ResA[n][q] = ResA[n][q] + TA[q] * b.WET;
+        ResB[n][q] = ResB[n][q] + TB[q] * b.WET;
+      }
+    }
+  }
+}
+
+// storeBulkStability (Equations_Stability_mod.F90:135-226) before the
+// scatter: A[(inod,ieq)*54 + (jnod,jeq)] = TEMP_A_f(inod,ieq,jnod,jeq), the
+// quotient (dRes_a - Res_a)/eps_jac; B likewise.  TEMP_TL_d starts at 0.
+static void elementStability(const Phys& ph, Elem& E, vreal* A, vreal* B) {
+  vreal TLd[9][6];
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) TLd[n][q] = Z();
+  vreal ResA[9][6], ResB[9][6], dA[9][6], dB[9][6];
+  residualBulkStability(ph, E, TLd, ResA, ResB);
+  const vreal h = vreal(1.0e-8);
+  for (int inod = 0; inod < 9; inod++)
+    for (int ieq = 0; ieq < 6; ieq++) {
+      TLd[inod][ieq] = TLd[inod][ieq] + h;
+      residualBulkStability(ph, E, TLd, dA, dB);
+      TLd[inod][ieq] = TLd[inod][ieq] - h;
+      int c = inod * 6 + ieq;
+      for (int jnod = 0; jnod < 9; jnod++)
+        for (int jeq = 0; jeq < 6; jeq++) {
+          A[c * 54 + jnod * 6 + jeq] = cdiv(dA[jnod][jeq] - ResA[jnod][jeq], h);
+          B[c * 54 + jnod * 6 + jeq] = cdiv(dB[jnod][jeq] - ResB[jnod][jeq], h);
+        }
+    }
+}
+
+// The base residual's test-node terms with the test-function coefficients
+// (tau's, LSME/LSCE operators) taken from Gauss point c and the strong-form
+// quantities from Gauss point s — the frozen-coefficient form whose
+// derivative in the state is the stability operator A.  Used by the tests
+// only; with c = s it equals nodeResidual.
+static void nodeResidualFrozen(const Phys& ph, const Elem& E, const GsPt& c, const GsPt& s, int n, vreal TERM[6]) {
+  vreal bifn = vreal(E.bfn[n]);
+  vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
+  for (int i = 0; i < 2; i++) {
+    vreal lsme = c.tlsme * ph.ReN * dot3(c.U_f, dF) * s.Momentum[i];
+    vreal lsic = c.tlsic * dF[i] * s.Continuity;
+    vreal lsce[3][3];
+    lsceMomentum(ph, c, i, dF, lsce);
+    vreal Prow[3] = {s.P_tot[i][0], s.P_tot[i][1], s.P_tot[i][2]};
+    TERM[i] = ph.ReN * s.Convection[i] * bifn + dot3(Prow, dF) + lsme + lsic + ddot33(lsce, s.Con_EQ);
+  }
+  TERM[2] = s.Continuity * bifn + c.tlsme * dot3(dF, s.Momentum);
+  vreal lsce_s = c.tlsce * (c.WiN * dot3(c.U_f, dF) + cdiv(c.f_T * bifn, vreal(2.0)));
+  const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  for (int q = 0; q < 3; q++) {
+    int j = comp[q][0], k = comp[q][1];
+    vreal lsme = -(dF[k] * s.Momentum[j]);
+    if (j != k) lsme = lsme - dF[j] * s.Momentum[k];
+    lsme = c.tlsme * lsme * c.WiN;
+    vreal dS[3] = {s.dSdXi[j][k][0], s.dSdXi[j][k][1], s.dSdXi[j][k][2]};
+    TERM[3 + q] = s.Con_EQ[j][k] * bifn + ph.DfN * dot3(dS, dF) + lsme + lsce_s * s.Con_EQ[j][k];
+  }
+}
+
 #ifndef VEF_COUNT_FLOPS
 static void to_double(const vreal* a, double* b, int n) {
   for (int i = 0; i < n; i++) b[i] = val(a[i]);
@@ -1212,6 +1601,174 @@ int vo_nodal_stress(double BvN, double WiN, int n_nodes, const double* x, double
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// f3: stability assembly entry points.
+// ---------------------------------------------------------------------------
+static void load_elem(Elem& E, const double* xy, const double* TL, double EU) {
+  for (int n = 0; n < 9; n++) {
+    for (int q = 0; q < 6; q++) E.TL[n][q] = TL[n * 6 + q];
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0.0;
+  }
+  E.EU = EU;
+}
+
+int vo_stab_plant(int which) {
+  g_stab_plant = which;
+  return 0;
+}
+
+// Element matrices of storeBulkStability: A, B [54 columns (inod,ieq)][54 rows (jnod,jeq)].
+int vo_stab_element(const vo_params* p, const double* xy, const double* TLb, double EU, double* A, double* B) {
+  Phys ph = make_phys(p);
+  if (!ph.dcs_off) return -2;
+  Elem E;
+  load_elem(E, xy, TLb, EU);
+  std::vector<vreal> Av(2916), Bv(2916);
+  elementStability(ph, E, Av.data(), Bv.data());
+  to_double(Av.data(), A, 2916);
+  to_double(Bv.data(), B, 2916);
+  return 0;
+}
+
+// Res_a, Res_b of residualBulkStability for a given disturbance TLd[9][6].
+int vo_stab_residual(const vo_params* p, const double* xy, const double* TLb, double EU, const double* TLd,
+                     double* ResA, double* ResB) {
+  Phys ph = make_phys(p);
+  if (!ph.dcs_off) return -2;
+  Elem E;
+  load_elem(E, xy, TLb, EU);
+  vreal D[9][6], RA[9][6], RB[9][6];
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) D[n][q] = TLd[n * 6 + q];
+  residualBulkStability(ph, E, D, RA, RB);
+  to_double(&RA[0][0], ResA, 54);
+  to_double(&RB[0][0], ResB, 54);
+  return 0;
+}
+
+// At bulk Gauss point gp: stab[9][6] = the stability test-node terms
+// TERM_RES_a for disturbance TLd, and fd[9][6] = the central difference
+// (T(b + h d) - T(b - h d))/(2h) of the base test-node terms with the
+// test-function coefficients frozen at the base state b.
+int vo_stab_linearization(const vo_params* p, const double* xy, const double* TLb, double EU, const double* TLd,
+                          int gp, double h, double* stab, double* fd) {
+  Phys ph = make_phys(p);
+  if (!ph.dcs_off) return -2;
+  const Tables& T = tables();
+  vreal I[3][3];
+  identity(I);
+  Elem E;
+  load_elem(E, xy, TLb, EU);
+  setGaussPoint(E, T, gp);
+  GsPt b;
+  basis2D(E, b);
+  setFlowQuantities(ph, E, b, I);
+  vreal D[9][6];
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) D[n][q] = TLd[n * 6 + q];
+  GsPtD d;
+  setFlowQuantitiesD(ph, E, D, b, d, I);
+  GsPt sp, sm;
+  Elem Ep = E, Em = E;
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) {
+      Ep.TL[n][q] = E.TL[n][q] + vreal(h) * D[n][q];
+      Em.TL[n][q] = E.TL[n][q] - vreal(h) * D[n][q];
+    }
+  basis2D(Ep, sp);
+  setFlowQuantities(ph, Ep, sp, I);
+  basis2D(Em, sm);
+  setFlowQuantities(ph, Em, sm, I);
+  for (int n = 0; n < 9; n++) {
+    vreal TA[6], TB[6], Tp[6], Tm[6];
+    nodeResidualStability(ph, E, b, d, n, TA, TB);
+    nodeResidualFrozen(ph, E, b, sp, n, Tp);
+    nodeResidualFrozen(ph, E, b, sm, n, Tm);
+    for (int q = 0; q < 6; q++) {
+      stab[n * 6 + q] = val(TA[q]);
+      fd[n * 6 + q] = (val(Tp[q]) - val(Tm[q])) / (2.0 * h);
+    }
+  }
+  return 0;
+}
+
+// Pattern of the stability matrices (preAllocatePetsc with Nex = 0): the node
+// blocks only, rows/cols 6*gnod + ieq.
+static void build_pattern_stability(int n_nodes, int n_el, const int32_t* conn, CSR& P) {
+  CSR Q;
+  build_pattern(n_nodes, n_el, conn, Q);  // + the arclength column (last in each row) and row
+  int64_t N = (int64_t)n_nodes * 6;
+  P.n = N;
+  P.rowptr.assign(N + 1, 0);
+  for (int64_t r = 0; r < N; r++) P.rowptr[r + 1] = P.rowptr[r] + (Q.rowptr[r + 1] - Q.rowptr[r] - 1);
+  P.col.resize(P.rowptr[N]);
+  for (int64_t r = 0; r < N; r++)
+    std::memcpy(&P.col[P.rowptr[r]], &Q.col[Q.rowptr[r]], sizeof(int32_t) * (P.rowptr[r + 1] - P.rowptr[r]));
+}
+int64_t vo_stab_pattern_nnz(int n_nodes, int n_el, const int32_t* conn) {
+  CSR P;
+  build_pattern_stability(n_nodes, n_el, conn, P);
+  return P.rowptr[P.n];
+}
+int vo_stab_pattern(int n_nodes, int n_el, const int32_t* conn, int64_t* rowptr, int32_t* col) {
+  CSR P;
+  build_pattern_stability(n_nodes, n_el, conn, P);
+  std::memcpy(rowptr, P.rowptr.data(), sizeof(int64_t) * (P.n + 1));
+  std::memcpy(col, P.col.data(), sizeof(int32_t) * P.rowptr[P.n]);
+  return 0;
+}
+
+// setEquationsBulkStability over all elements (MatZeroEntries first): element
+// matrices on `nthreads` OpenMP threads, the ADD_VALUES scatter serially in
+// element order.  x[6*n_nodes + 1] is the base state (TL node-major, EU last).
+int vo_stab_assemble(const vo_params* p, int n_nodes, int n_el, const double* coords, const int32_t* conn,
+                     const double* x, const int64_t* rowptr, const int32_t* col, double* valsA, double* valsB,
+                     int el_begin, int el_end, int nthreads) {
+  Phys ph = make_phys(p);
+  if (!ph.dcs_off) return -2;
+  const int64_t N = (int64_t)n_nodes * 6;
+  const double EU = x[N];
+  std::fill(valsA, valsA + rowptr[N], 0.0);
+  std::fill(valsB, valsB + rowptr[N], 0.0);
+  if (el_end < 0 || el_end > n_el) el_end = n_el;
+  const int BLK = 256;
+  std::vector<double> Abuf((size_t)BLK * 2916), Bbuf((size_t)BLK * 2916);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  for (int e0 = el_begin; e0 < el_end; e0 += BLK) {
+    int e1 = std::min(el_end, e0 + BLK);
+#pragma omp parallel for schedule(static)
+    for (int e = e0; e < e1; e++) {
+      Elem E;
+      copyToLocal(E, coords, conn, e, x, EU);
+      std::vector<vreal> Av(2916), Bv(2916);
+      elementStability(ph, E, Av.data(), Bv.data());
+      to_double(Av.data(), &Abuf[(size_t)(e - e0) * 2916], 2916);
+      to_double(Bv.data(), &Bbuf[(size_t)(e - e0) * 2916], 2916);
+    }
+    for (int e = e0; e < e1; e++) {
+      const int32_t* nodes = conn + (size_t)e * 9;
+      const double* A = &Abuf[(size_t)(e - e0) * 2916];
+      const double* B = &Bbuf[(size_t)(e - e0) * 2916];
+      for (int jn = 0; jn < 9; jn++)
+        for (int jq = 0; jq < 6; jq++) {
+          int64_t row = (int64_t)nodes[jn] * 6 + jq;
+          for (int in = 0; in < 9; in++)
+            for (int iq = 0; iq < 6; iq++) {
+              int64_t s = find_slot(rowptr, col, row, nodes[in] * 6 + iq);
+              if (s < 0) return -1;
+              valsA[s] += A[(in * 6 + iq) * 54 + jn * 6 + jq];
+              valsB[s] += B[(in * 6 + iq) * 54 + jn * 6 + jq];
+            }
+        }
+    }
+  }
+  return 0;
+}
+
 #else  // VEF_COUNT_FLOPS
 
 // Operation counts of one element-residual evaluation (bulk or edge) at a given
@@ -1235,6 +1792,48 @@ int vo_count_flops(const vo_params* p, const double* xy, const double* TL, doubl
   out[2] = FlopCounter::div;
   out[3] = FlopCounter::sqrt;
   out[4] = FlopCounter::total();
+  return 0;
+}
+
+// Operation counts of the stability element: out[0..4] one residualBulkStability
+// evaluation (add, mul, div, sqrt, total), out[5..9] its base part (basis2D and
+// the base state's setFlowQuantities at the 9 Gauss points).
+int vo_count_flops_stability(const vo_params* p, const double* xy, const double* TLb, double EU, const double* TLd,
+                             uint64_t* out) {
+  Phys ph = make_phys(p);
+  Elem E;
+  for (int n = 0; n < 9; n++) {
+    for (int q = 0; q < 6; q++) E.TL[n][q] = vreal(TLb[n * 6 + q]);
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0.0;
+  }
+  E.EU = vreal(EU);
+  vreal D[9][6], RA[9][6], RB[9][6];
+  for (int n = 0; n < 9; n++)
+    for (int q = 0; q < 6; q++) D[n][q] = vreal(TLd[n * 6 + q]);
+  FlopCounter::reset();
+  residualBulkStability(ph, E, D, RA, RB);
+  out[0] = FlopCounter::add;
+  out[1] = FlopCounter::mul;
+  out[2] = FlopCounter::div;
+  out[3] = FlopCounter::sqrt;
+  out[4] = FlopCounter::total();
+  const Tables& T = tables();
+  vreal I[3][3];
+  identity(I);
+  FlopCounter::reset();
+  for (int g = 0; g < 9; g++) {
+    setGaussPoint(E, T, g);
+    GsPt b;
+    basis2D(E, b);
+    setFlowQuantities(ph, E, b, I);
+  }
+  out[5] = FlopCounter::add;
+  out[6] = FlopCounter::mul;
+  out[7] = FlopCounter::div;
+  out[8] = FlopCounter::sqrt;
+  out[9] = FlopCounter::total();
   return 0;
 }
 

@@ -88,6 +88,28 @@ def test_newton_matches_cpu_iteration(oracle):
     assert abs(Fd - oFd) <= 1e-8 * abs(oFd)
 
 
+def test_single_precision_factorisation_newton():
+    """Inexact Newton: the FP64 Jacobian factorised and solved in FP32, the
+    residual, update and convergence test in FP64.  It converges with the same
+    iteration flags, and the drag differs from the FP64 Newton's by at most
+    1e-8 relative."""
+    mesh = cylinder_channel(8, 4, 6, 8)
+    LN0 = 0.1
+    x0 = initial_state(mesh, LN0)
+    print("[gpu newton] fp32 factorisation, small mesh", flush=True)
+    edges = boundary_edges(mesh, "Cylinder")
+    out = {}
+    for prec in ("f64", "f32"):
+        solver = NewtonSolver(mesh, Params(**OLDROYD_B), linear_precision=prec)
+        x, hist = solver.solve(x0, LN0)
+        assert hist[-1].cor_norm < 1e-8 and hist[-1].res_norm < 1e-8
+        assert solver.lu.host_fallbacks == 0
+        solver.asm.set_drag_edges(edges)
+        out[prec] = ([h.flag for h in hist], solver.asm.drag(x).item())
+    assert out["f32"][0] == out["f64"][0]
+    assert abs(out["f32"][1] - out["f64"][1]) <= 1e-8 * abs(out["f64"][1])
+
+
 def test_arclength_continuation_matches_cpu(oracle):
     """Six continuation steps (two pinned, four pseudo-arclength) on the GPU
     against the same continuation on the CPU: the same step sizes, and EU and

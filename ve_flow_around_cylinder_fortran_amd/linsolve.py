@@ -12,6 +12,11 @@ Every 'NRP' matrix is then factorised on the GPU in that pattern
 GPU (csrrf_solve).  Each factorisation is checked by the residual of its first
 solve; if static pivoting has lost accuracy, that system is solved on the host
 with threshold partial pivoting instead, and the event is counted.
+
+`precision="f32"` factorises and solves in single precision (the FP64 values
+rounded once per factorisation; right-hand sides rounded on the way in, the
+solution widened on the way out): the linear solve of an inexact Newton whose
+residual stays FP64.  Its check tolerance is correspondingly looser.
 """
 from __future__ import annotations
 
@@ -36,10 +41,13 @@ def _libs():
         _rocblas.rocblas_set_stream.argtypes = [vp, vp]
         _rocsolver.rocsolver_create_rfinfo.argtypes = [ctypes.POINTER(vp), vp]
         _rocsolver.rocsolver_destroy_rfinfo.argtypes = [vp]
-        _rocsolver.rocsolver_dcsrrf_analysis.argtypes = [vp, i32, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp,
-                                                         i32, vp]
-        _rocsolver.rocsolver_dcsrrf_refactlu.argtypes = [vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]
-        _rocsolver.rocsolver_dcsrrf_solve.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp]
+        for t in "sd":
+            getattr(_rocsolver, f"rocsolver_{t}csrrf_analysis").argtypes = [vp, i32, i32, i32, vp, vp, vp, i32, vp, vp,
+                                                                            vp, vp, vp, vp, i32, vp]
+            getattr(_rocsolver, f"rocsolver_{t}csrrf_refactlu").argtypes = [vp, i32, i32, vp, vp, vp, i32, vp, vp, vp,
+                                                                            vp, vp, vp]
+            getattr(_rocsolver, f"rocsolver_{t}csrrf_solve").argtypes = [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp,
+                                                                         i32, vp]
     return _rocblas, _rocsolver
 
 
@@ -75,10 +83,18 @@ def _bundle(L, U, n):
 class RfLU:
     """LU of one sparsity pattern on the GPU by re-factorisation."""
 
-    def __init__(self, rowptr: np.ndarray, col: np.ndarray, device="cuda:0", check_tol: float = 1e-10):
+    def __init__(self, rowptr: np.ndarray, col: np.ndarray, device="cuda:0", check_tol: float = None,
+                 precision: str = "f64"):
         import torch
 
+        if precision not in ("f64", "f32"):
+            raise RfLUError(f"precision {precision!r}: expected 'f64' or 'f32'")
         self.torch = torch
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "f64" else torch.float32
+        self._t = "d" if precision == "f64" else "s"
+        if check_tol is None:
+            check_tol = 1e-10 if precision == "f64" else 1e-3
         self.n = len(rowptr) - 1
         if rowptr[-1] >= 2 ** 31:
             raise RfLUError("rocSOLVER's csrrf takes 32-bit indices: pattern too large")
@@ -141,21 +157,22 @@ class RfLU:
             raise RfLUError("L+U too large for 32-bit indices")
         self.ptrT = torch.from_numpy(T.indptr.astype(np.int32)).to(self.device)
         self.indT = torch.from_numpy(T.indices.astype(np.int32)).to(self.device)
-        self.valT = torch.from_numpy(T.data.astype(np.float64)).to(self.device)
+        self.valT = torch.from_numpy(T.data).to(device=self.device, dtype=self.dtype)
         perm = np.argsort(lu.perm_c).astype(np.int32)
         self.pivP = torch.from_numpy(perm).to(self.device)
         self.pivQ = torch.from_numpy(perm.copy()).to(self.device)
         self.nnzT = int(T.nnz)
-        M = torch.from_numpy(vals).to(self.device)
+        M = torch.from_numpy(vals).to(device=self.device, dtype=self.dtype)
         self.rfinfo = ctypes.c_void_p()
         _ok(rs.rocsolver_create_rfinfo(ctypes.byref(self.rfinfo), self.handle), "rocsolver_create_rfinfo")
         self._stream()
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
         # the solve phase needs an analysis with nrhs >= 1 and a right-hand side buffer
-        rhs = torch.zeros(self.n, dtype=torch.float64, device=self.device)
-        _ok(rs.rocsolver_dcsrrf_analysis(self.handle, self.n, 1, int(self.rowptr_h[-1]), p(self.ptrA), p(self.indA),
-                                         p(M), self.nnzT, p(self.ptrT), p(self.indT), p(self.valT), p(self.pivP),
-                                         p(self.pivQ), p(rhs), self.n, self.rfinfo), "rocsolver_dcsrrf_analysis")
+        rhs = torch.zeros(self.n, dtype=self.dtype, device=self.device)
+        analysis = getattr(rs, f"rocsolver_{self._t}csrrf_analysis")
+        _ok(analysis(self.handle, self.n, 1, int(self.rowptr_h[-1]), p(self.ptrA), p(self.indA), p(M), self.nnzT,
+                     p(self.ptrT), p(self.indT), p(self.valT), p(self.pivP), p(self.pivQ), p(rhs), self.n,
+                     self.rfinfo), f"rocsolver_{self._t}csrrf_analysis")
         torch.cuda.synchronize(self.device)
         self.analyses += 1
 
@@ -169,9 +186,11 @@ class RfLU:
         self._host_lu = None
         self._stream()
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-        _ok(rs.rocsolver_dcsrrf_refactlu(self.handle, self.n, int(self.rowptr_h[-1]), p(self.ptrA), p(self.indA),
-                                         p(values), self.nnzT, p(self.ptrT), p(self.indT), p(self.valT), p(self.pivP),
-                                         p(self.pivQ), self.rfinfo), "rocsolver_dcsrrf_refactlu")
+        vals = values if values.dtype == self.dtype else values.to(self.dtype)
+        refactlu = getattr(rs, f"rocsolver_{self._t}csrrf_refactlu")
+        _ok(refactlu(self.handle, self.n, int(self.rowptr_h[-1]), p(self.ptrA), p(self.indA), p(vals), self.nnzT,
+                     p(self.ptrT), p(self.indT), p(self.valT), p(self.pivP), p(self.pivQ), self.rfinfo),
+            f"rocsolver_{self._t}csrrf_refactlu")
         self.refactorizations += 1
 
     def _residual(self, values, x, b):
@@ -191,11 +210,13 @@ class RfLU:
             raise RfLUError("factor() first")
         if self._host_lu is not None:
             return torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
-        x = b.clone()
+        x = b.to(self.dtype, copy=True)
         self._stream()
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-        _ok(rs.rocsolver_dcsrrf_solve(self.handle, self.n, 1, self.nnzT, p(self.ptrT), p(self.indT), p(self.valT),
-                                      p(self.pivP), p(self.pivQ), p(x), self.n, self.rfinfo), "rocsolver_dcsrrf_solve")
+        solve = getattr(rs, f"rocsolver_{self._t}csrrf_solve")
+        _ok(solve(self.handle, self.n, 1, self.nnzT, p(self.ptrT), p(self.indT), p(self.valT), p(self.pivP),
+                  p(self.pivQ), p(x), self.n, self.rfinfo), f"rocsolver_{self._t}csrrf_solve")
+        x = x.to(b.dtype)
         if not self._checked:
             self._checked = True
             rn, bn = self._residual(self._A, x, b)

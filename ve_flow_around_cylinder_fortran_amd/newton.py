@@ -105,7 +105,7 @@ class NewtonError(RuntimeError):
 class NewtonSolver:
     """Fixed-Wi Newton solve (arclength unknown pinned at LN0) on the GPU."""
 
-    def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0"):
+    def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64"):
         import torch
 
         from .linsolve import RfLU
@@ -118,7 +118,7 @@ class NewtonSolver:
         self.device = device
         self.asm = Assembler(mesh, params)
         rowptr, col = self.asm.pattern()
-        self.lu = RfLU(rowptr, col, device=device)
+        self.lu = RfLU(rowptr, col, device=device, precision=linear_precision)
         self.N = self.asm.N
         self.b = torch.empty(self.N, dtype=torch.float64, device=device)
         self.values = torch.empty(self.asm.nnz, dtype=torch.float64, device=device)
