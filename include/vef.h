@@ -153,6 +153,36 @@ int vef_drag(vef_handle* h, const double* x, double* Fd, double* per_edge, void*
  * caller.  T: device double[n_nodes][4] = (Txx, Txy, Tyy, Tzz). */
 int vef_nodal_stress(vef_handle* h, const double* x, double WiN, double* T, void* hip_stream);
 
+/* Linear stability assembly of Cylinder_Stability_2D (6 unknowns per node,
+ * no arclength unknown) for the unchanged SLEPc shift-invert solve.
+ * Replaces assembleSlepcObjects (src/Stability/Stability_mod.F90:97-199) for
+ * this problem: MatZeroEntries of A_f and B_f, setEquationsBulkStability
+ * (src/Stability/Equations_Stability_mod.F90:20-57: element residuals
+ * residualBulkStability and the forward-difference matrices of
+ * storeBulkStability, :61-226, in the disturbance around the base state, with
+ * eps_jac = 1e-8, and their ADD_VALUES scatter), and applyDirichletBCStability
+ * (src/Dirichlet_mod.F90:287-316).  No edge equations apply to the names of
+ * this problem (setBoundaryEQsStability, src/Boundary_mod.F90:319-439), and
+ * no periodic boundaries are configured (src/Parameters_mod.F90:152).
+ *   - The base state is x[N] as for vef_assemble (TL node-major, EU last;
+ *     WiN = WiN0*EU).  The reference's values are complex with zero imaginary
+ *     parts on this path; A and B hold the real parts (FP64).
+ *   - Pattern: the node blocks of vef_get_pattern without the arclength
+ *     column and row: rowptr[6*n_nodes + 1], colind[vef_stability_nnz].
+ *   - Requires DfN > 1e-12 (the discontinuity-capturing coefficient is then
+ *     zero); otherwise VEF_ERR_UNSUPPORTED.
+ *   - vef_set_stability_dirichlet stores the rows (host int64[n], 6*gnod +
+ *     ieq) of the Inlet and Outlet (all unknowns) and Wall and Cylinder
+ *     (velocity) boundaries; vef_stability_assemble then sets them to the rows
+ *     of the identity in A and to zero in B (MatZeroRows with diag 1 and 0).
+ *   - A, B: device double[vef_stability_nnz], fully overwritten.
+ * While profiling is enabled (vef_profile), the element kernel counts as
+ * bulk_ms and the ordered sums and Dirichlet rows as scatter_ms. */
+int64_t vef_stability_nnz(const vef_handle* h);
+int vef_get_stability_pattern(const vef_handle* h, int64_t* rowptr, int32_t* colind);
+int vef_set_stability_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows);
+int vef_stability_assemble(vef_handle* h, const double* x, double* A, double* B, void* hip_stream);
+
 /* The same with host arrays (copies in, assembles, copies out). */
 int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values);
 

@@ -1,0 +1,38 @@
+// Host build of the stability kernel's per-Gauss-point arithmetic
+// (ve_flow_around_cylinder_fortran_amd/csrc/vef_stability_math.h), for a CPU
+// test that compares it bit for bit with the oracle.  The caller supplies the
+// element's Gauss-point geometry and basis tables; this file repeats the
+// kernel's per-element loop: base point per Gauss point, disturbance per
+// perturbation, test-node terms, ordered sum over the Gauss points, quotient.
+#include "vef_stability_math.h"
+
+extern "C" int stab_host_element(const double* prm5, int model, const double* u54, double EU, const double* bf,
+                                 const double* dFx, const double* dFy, const double* d2x, const double* d2y,
+                                 const double* H2, const double* WET, double* A, double* B) {
+  vefstab::Params prm{prm5[0], prm5[1], prm5[2], prm5[3], prm5[4], 1.0 - prm5[1], model};
+  vefstab::Base base[9];
+  for (int g = 0; g < 9; g++)
+    vefstab::base_point(u54, bf + 9 * g, dFx + 9 * g, dFy + 9 * g, H2[g], EU, prm, base[g]);
+  for (int p = 0; p < 54; p++) {
+    double ra[9][6] = {}, rb[9][6] = {};
+    for (int g = 0; g < 9; g++) {
+      vefstab::Dist d;
+      vefstab::disturbance(base[g], p / 6, p % 6, bf + 9 * g, dFx + 9 * g, dFy + 9 * g, d2x + 9 * g, d2y + 9 * g,
+                           prm, d);
+      for (int n = 0; n < 9; n++) {
+        double a[6], b[6];
+        vefstab::node_terms(base[g], d, bf[9 * g + n], dFx[9 * g + n], dFy[9 * g + n], WET[g], prm, a, b);
+        for (int c = 0; c < 6; c++) {
+          ra[n][c] = g == 0 ? a[c] : ra[n][c] + a[c];
+          rb[n][c] = g == 0 ? b[c] : rb[n][c] + b[c];
+        }
+      }
+    }
+    for (int n = 0; n < 9; n++)
+      for (int c = 0; c < 6; c++) {
+        A[p * 54 + n * 6 + c] = 0.0 + vefstab::fd_quotient(ra[n][c]);
+        B[p * 54 + n * 6 + c] = 0.0 + vefstab::fd_quotient(rb[n][c]);
+      }
+  }
+  return 0;
+}

@@ -115,6 +115,14 @@ def load_library():
     lib.vef_nodal_stress.argtypes = [c_vp, c_vp, ctypes.c_double, c_vp, c_vp]
     lib.vef_nodal_stress.restype = ctypes.c_int
     lib.vef_assemble_host.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp]
+    lib.vef_stability_nnz.argtypes = [c_vp]
+    lib.vef_stability_nnz.restype = c_i64
+    lib.vef_get_stability_pattern.argtypes = [c_vp, c_vp, c_vp]
+    lib.vef_get_stability_pattern.restype = ctypes.c_int
+    lib.vef_set_stability_dirichlet.argtypes = [c_vp, c_i64, c_vp]
+    lib.vef_set_stability_dirichlet.restype = ctypes.c_int
+    lib.vef_stability_assemble.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp]
+    lib.vef_stability_assemble.restype = ctypes.c_int
     lib.vef_assemble_host.restype = ctypes.c_int
     lib.vef_profile.argtypes = [c_vp, c_i32]
     lib.vef_profile.restype = ctypes.c_int
@@ -289,6 +297,40 @@ class Assembler:
                                        ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
         self._check(rc, "vef_nodal_stress")
         return T
+
+    # -- linear stability (Cylinder_Stability_2D) ---------------------------
+    def stability_pattern(self) -> Tuple[np.ndarray, np.ndarray]:
+        """CSR pattern of A and B (the node blocks; rows/cols 6*g + q)."""
+        nnz = int(self.lib.vef_stability_nnz(self._h))
+        rowptr = np.empty(self.N, dtype=np.int64)  # 6*n_nodes + 1
+        col = np.empty(nnz, dtype=np.int32)
+        self._check(self.lib.vef_get_stability_pattern(self._h, _ptr(rowptr), _ptr(col)), "vef_get_stability_pattern")
+        return rowptr, col
+
+    def set_stability_dirichlet(self, rows: np.ndarray):
+        """Rows (6*g + q) of applyDirichletBCStability."""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        self._check(self.lib.vef_set_stability_dirichlet(self._h, rows.size, _ptr(rows)),
+                    "vef_set_stability_dirichlet")
+
+    def stability(self, x, A=None, B=None, stream=None):
+        """A and B values (device float64 tensors, stability pattern) for the
+        base state x (device tensor, length N: TL node-major, EU last)."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        nnz = int(self.lib.vef_stability_nnz(self._h))
+        if A is None:
+            A = torch.empty(nnz, dtype=torch.float64, device=x.device)
+        if B is None:
+            B = torch.empty(nnz, dtype=torch.float64, device=x.device)
+        if x.numel() != self.N or A.numel() != nnz or B.numel() != nnz:
+            raise VefError("x must have length N, A and B the stability nnz")
+        rc = self.lib.vef_stability_assemble(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(A.data_ptr()),
+                                             ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
+        self._check(rc, "vef_stability_assemble")
+        return A, B
 
     def assemble_host(self, x: np.ndarray, flag_nr: str = "NRP"):
         """The same with numpy arrays (copies through the host)."""

@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "vef.h"
+#include "vef_stability_math.h"
 
 namespace {
 
@@ -1126,6 +1127,114 @@ vef_dirichlet_kernel(int64_t n_rows, const int64_t* __restrict__ rows, const dou
 }
 
 // ----------------------------------------------------------------------------
+// Linear stability assembly (Cylinder_Stability_2D): A and B of the
+// generalised eigenproblem by forward differences in the disturbance
+// (storeBulkStability, src/Stability/Equations_Stability_mod.F90:135-226).
+// Eight single-wavefront workgroups per element, ids 64m + 8j + x for element
+// 8m + x and group j (one XCD per element); lane (g, s) = Gauss point g of 9,
+// slot s of 7, perturbation p = 7j + s < 54 of unknown (p/6, p%6).  Nine lanes
+// first compute the Gauss-point geometry and base quantities into LDS; each
+// lane then forms its disturbance quantities and, per test node, the 6 A and
+// 6 B terms; an ordered reduction over the Gauss points gives
+// Res(eps e_p) and the quotient Res/eps (the zero disturbance has zero
+// residuals), stored like the base Jacobian: straight into the values of node
+// pairs with one contributing element, else into the element's bank rows for
+// vef_shared_kernel.  The stability pattern is the base one without the
+// arclength column (the last of each row) and row, so the codes of the base
+// scatter map apply with the stability row pointers.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+vef_stab_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
+                int64_t eu_index, int32_t n_el, vefstab::Params prm, const int32_t* __restrict__ code,
+                const int64_t* __restrict__ rowptr_s, double* __restrict__ A, double* __restrict__ B,
+                double* __restrict__ cbank_a, double* __restrict__ cbank_b) {
+  __shared__ double sU[54];
+  __shared__ double sX[18];
+  __shared__ Geo sG[9];
+  __shared__ vefstab::Base sB[9];
+  __shared__ double sRed[9][7][12];
+  __shared__ int32_t sCode[81];
+  __shared__ int64_t sRb[9];
+  __shared__ int32_t sRl[9];
+  const int tid = threadIdx.x;
+  const int64_t blk = blockIdx.x;
+  const int64_t e = (blk >> 6) * 8 + (blk & 7);
+  const int kg = (int)((blk >> 3) & 7);
+  if (e >= n_el) return;  // uniform over the workgroup
+  const double EU = x[eu_index];
+  if (tid < 54) sU[tid] = x[(int64_t)conn[e * 9 + tid / 6] * 6 + tid % 6];
+  if (tid < 18) sX[tid] = coords[(int64_t)conn[e * 9 + tid / 2] * 2 + (tid & 1)];
+  for (int t = tid; t < 81; t += 64) sCode[t] = code[e * 81 + t];
+  if (tid < 9) {
+    const int64_t r6 = (int64_t)conn[e * 9 + tid] * 6;
+    const int64_t rb = rowptr_s[r6];
+    sRb[tid] = rb;
+    sRl[tid] = (int32_t)(rowptr_s[r6 + 1] - rb);
+  }
+  __syncthreads();
+  if (tid < 9) {
+    Geo& G = sG[tid];
+    geometry(sX, c_tab.dc[tid], c_tab.de[tid], c_tab.dcc[tid], c_tab.dee[tid], c_tab.dce[tid], c_tab.W2[tid], G);
+    vefstab::base_point(sU, c_tab.bf[tid], G.dFx, G.dFy, G.H2, EU, prm, sB[tid]);
+  }
+  __syncthreads();
+  const int g = tid / 7, s = tid % 7;
+  const int p = kg * 7 + s;
+  const bool valid = tid < 63 && p < 54;
+  vefstab::Dist D;
+  if (valid)
+    vefstab::disturbance(sB[g], p / 6, p % 6, c_tab.bf[g], sG[g].dFx, sG[g].dFy, sG[g].d2x, sG[g].d2y, prm, D);
+  for (int n = 0; n < 9; n++) {
+    __syncthreads();  // the previous sums have been read
+    if (valid) {
+      double a[6], bb[6];
+      vefstab::node_terms(sB[g], D, c_tab.bf[g][n], sG[g].dFx[n], sG[g].dFy[n], sG[g].WET, prm, a, bb);
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        sRed[g][s][c] = a[c];
+        sRed[g][s][6 + c] = bb[c];
+      }
+    }
+    __syncthreads();
+    // Res = Res + TERM*WET over the Gauss points in order; c < 6 rows of A, else of B.
+    for (int t = tid; t < 84; t += 64) {
+      const int c = t / 7, ss = t % 7, pp = kg * 7 + ss;
+      if (pp < 54) {
+        double acc = sRed[0][ss][c];
+#pragma unroll
+        for (int gg = 1; gg < 9; gg++) acc = acc + sRed[gg][ss][c];
+        const double q = vefstab::fd_quotient(acc);
+        const int cc = c % 6, in = pp / 6, iq = pp % 6;
+        const int32_t cd = sCode[n * 9 + in];
+        if ((cd & 1) == 0)
+          (c < 6 ? A : B)[sRb[n] + (int64_t)cc * sRl[n] + (int64_t)(cd >> 1) * 6 + iq] = 0.0 + q;
+        else
+          (c < 6 ? cbank_a : cbank_b)[(int64_t)(cd >> 1) * 36 + cc * 6 + iq] = q;
+      }
+    }
+  }
+}
+
+// applyDirichletBCStability (src/Dirichlet_mod.F90:287-316): MatZeroRows of A
+// with diagonal 1 and of B with diagonal 0, a wavefront per row.  Columns are
+// read from the base pattern, whose row r holds the stability row's columns
+// followed by the arclength column.
+__global__ void __launch_bounds__(256)
+vef_stab_dirichlet_kernel(int64_t n_rows, const int64_t* __restrict__ rows, const int64_t* __restrict__ rowptr_s,
+                          const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col, double* __restrict__ A,
+                          double* __restrict__ B) {
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (i >= n_rows) return;
+  const int64_t r = rows[i];
+  const int64_t ps = rowptr_s[r], len = rowptr_s[r + 1] - ps, pb = rowptr[r];
+  for (int64_t k = lane; k < len; k += 64) {
+    A[ps + k] = (col[pb + k] == r) ? 1.0 : 0.0;
+    B[ps + k] = 0.0;
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Arclength row (arclength / pseudoArclength, src/Extra_Equations_mod.F90:698-828)
 // ----------------------------------------------------------------------------
 // Increment <= 2: MatZeroRowsColumns(N-1) with diagonal 1.  Column N-1 is the
@@ -1234,6 +1343,18 @@ struct vef_handle {
   int64_t* d_bc_rows = nullptr;     // Dirichlet rows, distinct
   double* d_bc_values = nullptr;    // their target values
   int64_t n_bc = 0;
+  // linear stability assembly (allocated on first use)
+  std::vector<int64_t> h_sbase_s;   // shared pairs: first slot in the stability pattern
+  std::vector<int32_t> h_slen_s;    //   its row length there
+  bool stab_ready = false;
+  int64_t nnz_s = 0;
+  int64_t* d_rowptr_s = nullptr;
+  int64_t* d_sbase_s = nullptr;
+  int32_t* d_slen_s = nullptr;
+  double* d_cbank_a = nullptr;
+  double* d_cbank_b = nullptr;
+  int64_t* d_sbc_rows = nullptr;    // stability Dirichlet rows, distinct
+  int64_t n_sbc = 0;
   int32_t* d_drag_edges = nullptr;  // boundary edges of the drag integral, [n][2]
   double* d_drag_terms = nullptr;   // term_el per edge
   int32_t n_drag_edges = 0;
@@ -1362,7 +1483,8 @@ void vef_destroy(vef_handle* h) {
                   h->d_slen,   h->d_ustart,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
                   h->d_drag_edges, h->d_drag_terms, h->d_bc_rows, h->d_bc_values,
-                  h->d_tmp_b,  h->d_tmp_v};
+                  h->d_tmp_b,  h->d_tmp_v, h->d_rowptr_s, h->d_sbase_s, h->d_slen_s,
+                  h->d_cbank_a, h->d_cbank_b, h->d_sbc_rows};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -1521,6 +1643,8 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
     } else {
       sbase.push_back(rb + pos * 6);
       slen.push_back(rl);
+      h->h_sbase_s.push_back(rb - (int64_t)r * 6 + pos * 6);  // rows before r lose their arclength entry
+      h->h_slen_s.push_back(rl - 1);
       for (int32_t c = c0; c < c1; c++) {
         const int32_t w = contrib[c];
         code[(size_t)(w / 81) * 81 + w % 81] = (int32_t)((n_cidx << 1) | 1);
@@ -1810,6 +1934,106 @@ int vef_nodal_stress(vef_handle* h, const double* x, double WiN, double* T, void
     hipLaunchKernelGGL(vef_nodal_stress_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, n, x, h->dprm.omb, WiN, T);
     VEF_HIP(h, hipGetLastError());
+  }
+  return VEF_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Linear stability assembly
+// ----------------------------------------------------------------------------
+int64_t vef_stability_nnz(const vef_handle* h) {
+  if (!h) return -1;
+  return h->h_rowptr[h->N - 1] - (h->N - 1);
+}
+
+int vef_get_stability_pattern(const vef_handle* h, int64_t* rowptr, int32_t* colind) {
+  if (!h || !rowptr || !colind) return VEF_ERR_ARG;
+  const int64_t Ns = h->N - 1;
+  for (int64_t r = 0; r <= Ns; r++) rowptr[r] = h->h_rowptr[r] - r;
+  for (int64_t r = 0; r < Ns; r++)
+    std::memcpy(colind + rowptr[r], h->h_col.data() + h->h_rowptr[r], sizeof(int32_t) * (rowptr[r + 1] - rowptr[r]));
+  return VEF_OK;
+}
+
+int vef_set_stability_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (n_rows < 0 || (n_rows > 0 && !rows)) return set_err(h, VEF_ERR_ARG, "bad row list");
+  const int64_t Ns = h->N - 1;
+  std::vector<int64_t> r(rows, rows + n_rows);
+  for (int64_t v : r)
+    if (v < 0 || v >= Ns) return set_err(h, VEF_ERR_ARG, "stability Dirichlet row out of range");
+  std::sort(r.begin(), r.end());
+  r.erase(std::unique(r.begin(), r.end()), r.end());  // MatZeroRows is idempotent
+  if (h->d_sbc_rows) {
+    VEF_HIP(h, hipFree(h->d_sbc_rows));
+    h->d_sbc_rows = nullptr;
+  }
+  h->n_sbc = (int64_t)r.size();
+  if (h->n_sbc > 0) {
+    int rc = dev_upload(h, &h->d_sbc_rows, r.data(), r.size());
+    if (rc) return rc;
+  }
+  return VEF_OK;
+}
+
+static int ensure_stability(vef_handle* h) {
+  if (h->stab_ready) return VEF_OK;
+  const int64_t Ns = h->N - 1;
+  std::vector<int64_t> rp(Ns + 1);
+  for (int64_t r = 0; r <= Ns; r++) rp[r] = h->h_rowptr[r] - r;
+  h->nnz_s = rp[Ns];
+  int rc = dev_upload(h, &h->d_rowptr_s, rp.data(), rp.size());
+  if (!rc) rc = dev_upload(h, &h->d_sbase_s, h->h_sbase_s.data(), h->h_sbase_s.size());
+  if (!rc) rc = dev_upload(h, &h->d_slen_s, h->h_slen_s.data(), h->h_slen_s.size());
+  if (!rc) rc = dev_upload(h, &h->d_cbank_a, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
+  if (!rc) rc = dev_upload(h, &h->d_cbank_b, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
+  if (rc) return rc;
+  h->stab_ready = true;
+  return VEF_OK;
+}
+
+int vef_stability_assemble(vef_handle* h, const double* x, double* A, double* B, void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (!x || !A || !B) return set_err(h, VEF_ERR_ARG, "x, A and B are required");
+  if (!(h->params.DfN > 1.0e-12))
+    return set_err(h, VEF_ERR_UNSUPPORTED,
+                   "the stability assembly needs DfN > 1e-12 (discontinuity capturing off)");
+  int rc = ensure_stability(h);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  vefstab::Params prm{h->params.ReN, h->params.BvN, h->params.VeN, h->params.DfN, h->params.WiN0,
+                      1.0 - h->params.BvN, h->params.model};
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  if (h->profiling) {
+    for (int i = 0; i < 3; i++) VEF_HIP(h, hipEventCreate(&ev[i]));
+    VEF_HIP(h, hipEventRecord(ev[0], st));
+  }
+  hipLaunchKernelGGL(vef_stab_kernel, dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_coords,
+                     h->d_conn, x, h->N - 1, h->n_el, prm, h->d_code, h->d_rowptr_s, A, B, h->d_cbank_a,
+                     h->d_cbank_b);
+  VEF_HIP(h, hipGetLastError());
+  if (h->profiling) VEF_HIP(h, hipEventRecord(ev[1], st));
+  if (h->n_shared > 0) {
+    const unsigned blocks = (unsigned)((h->n_shared * 36 + 255) / 256);
+    hipLaunchKernelGGL(vef_shared_kernel, dim3(blocks), dim3(256), 0, st, h->n_shared, h->d_sbase_s, h->d_slen_s,
+                       h->d_ustart, h->d_cbank_a, A);
+    VEF_HIP(h, hipGetLastError());
+    hipLaunchKernelGGL(vef_shared_kernel, dim3(blocks), dim3(256), 0, st, h->n_shared, h->d_sbase_s, h->d_slen_s,
+                       h->d_ustart, h->d_cbank_b, B);
+    VEF_HIP(h, hipGetLastError());
+  }
+  if (h->n_sbc > 0) {
+    hipLaunchKernelGGL(vef_stab_dirichlet_kernel, dim3((unsigned)((h->n_sbc + 3) / 4)), dim3(256), 0, st, h->n_sbc,
+                       h->d_sbc_rows, h->d_rowptr_s, h->d_rowptr, h->d_col, A, B);
+    VEF_HIP(h, hipGetLastError());
+  }
+  if (h->profiling) {
+    VEF_HIP(h, hipEventRecord(ev[2], st));
+    h->ev_bulk.push_back(ev[0]);
+    h->ev_bulk.push_back(ev[1]);
+    h->ev_scatter.push_back(ev[1]);
+    h->ev_scatter.push_back(ev[2]);
+    h->ev_other.push_back(ev[0]);  // release_events destroys ev_other and ev_scatter: each event once
   }
   return VEF_OK;
 }

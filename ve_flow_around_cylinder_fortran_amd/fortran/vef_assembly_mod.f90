@@ -64,6 +64,34 @@ module vef_assembly_mod
             integer(c_int32_t), value :: flag
         end function vef_assemble
 
+        ! Linear stability (replaces assembleSlepcObjects for Cylinder_Stability_2D):
+        ! A and B in the node-block pattern, device arrays; x the base state.
+        integer(c_int64_t) function vef_stability_nnz(handle) bind(C, name='vef_stability_nnz')
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: handle
+        end function vef_stability_nnz
+
+        integer(c_int) function vef_get_stability_pattern(handle, rowptr, colind) &
+            bind(C, name='vef_get_stability_pattern')
+            import :: c_int, c_int32_t, c_int64_t, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int64_t), intent(out) :: rowptr(*)
+            integer(c_int32_t), intent(out) :: colind(*)
+        end function vef_get_stability_pattern
+
+        integer(c_int) function vef_set_stability_dirichlet(handle, n_rows, rows) &
+            bind(C, name='vef_set_stability_dirichlet')
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int64_t), value :: n_rows
+            integer(c_int64_t), intent(in) :: rows(*)
+        end function vef_set_stability_dirichlet
+
+        integer(c_int) function vef_stability_assemble(handle, x, A, B, stream) bind(C, name='vef_stability_assemble')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle, x, A, B, stream
+        end function vef_stability_assemble
+
         integer(c_int) function vef_assemble_host(handle, x, flag, b, values) bind(C, name='vef_assemble_host')
             import :: c_int, c_int32_t, c_double, c_ptr
             type(c_ptr), value :: handle
