@@ -33,6 +33,14 @@ FLOPS_PER_RESIDUAL_EVAL = 24426
 EVALS_PER_ELEMENT = 56            # base + 54 nodal + 1 arclength perturbation
 QUOTIENT_FLOPS = 54 * 55 * 3      # ((dR - R)*h)/(h*h): sub, mul, div per entry
 SCATTER_FLOPS = 2916 + 2 * 54     # one add per ADD_VALUES contribution
+# Linear stability assembly (A and B): one residualBulkStability evaluation
+# counts STAB_FLOPS_PER_EVAL operations, STAB_FLOPS_BASE of them for the base
+# state's Gauss-point quantities, which the GPU computes once per element;
+# each of the 54 forward differences repeats the rest.  Quotients: sub, mul,
+# div per entry of A and B.  Pinned by tests/test_flop_count.py.
+STAB_FLOPS_PER_EVAL = 35289
+STAB_FLOPS_BASE = 9261
+STAB_QUOTIENT_FLOPS = 2 * 2916 * 3
 PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector = matrix), FMA counted as 2
 PEAK_HBM_GBS = 8000.0
 N_ELEMENTS = 100000
@@ -50,6 +58,9 @@ def parse():
                     help="measure seconds per converged arclength step (fixed-pattern GPU Newton + continuation on "
                          "the 1,600-element mesh) instead of the assembly throughput")
     ap.add_argument("--arc-steps", type=int, default=10, help="continuation steps for --newton (2 pinned + the rest)")
+    ap.add_argument("--stability", action="store_true",
+                    help="linear stability assembly (A and B) of the 40,000-element mesh instead of the Newton "
+                         "assembly")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference launch for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -165,6 +176,125 @@ def newton_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def stability_flops_per_element():
+    return STAB_FLOPS_BASE + 54 * (STAB_FLOPS_PER_EVAL - STAB_FLOPS_BASE) + STAB_QUOTIENT_FLOPS
+
+
+def stability_bench(args):
+    """A and B of the linear stability problem (Cylinder_Stability_2D) around a
+    base state at Wi = 0.6 on the 40,000-element mesh; a step is one
+    vef_stability_assemble (element kernel, ordered sums of shared node pairs,
+    Dirichlet rows).  The reference's A and B do not depend on the spanwise
+    wavenumber (its spanwise terms are commented out,
+    src/Stability/Element_Calculations_Stability_mod.F90:51-81), so one
+    assembly serves every wavenumber of a batch."""
+    import numpy as np  # noqa: F401
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd import Params, smooth_state, workload_mesh
+    from ve_flow_around_cylinder_fortran_amd.stability import StabilityAssembler
+
+    n_el = 40000 if args.elements == N_ELEMENTS else args.elements
+    pset = dict(ReN=0.0, BvN=0.59, VeN=0.25, DfN=1.0e-3, WiN0=1.0, model="L-PTT")
+    mesh = workload_mesh(n_el)
+    x = smooth_state(mesh, EU=0.6)
+    sa = StabilityAssembler(mesh, Params(**pset))
+    xd = torch.from_numpy(x).to("cuda:0")
+    nnz = len(sa.col)
+    A = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
+    B = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
+    for _ in range(args.warmup):
+        sa.assemble(xd, A, B)
+    torch.cuda.synchronize()
+    sa.asm.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sa.assemble(xd, A, B)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kern_ms, rest_ms, launches = sa.asm.profile_read()
+    sa.asm.profile(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    ne = mesh.n_elements
+    kern_ms_launch = kern_ms / max(launches, 1)
+    flops = ne * stability_flops_per_element()
+    achieved = flops / (kern_ms_launch * 1e-3) / 1e12
+    out = {
+        "metric": "elements/s stability A+B assembly",
+        "value": ne / (ms_step * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cylinder-in-channel Q2 mesh, {ne} elements, linear stability A and B "
+                        "(Cylinder_Stability_2D) around a smooth base state at Wi=0.6, L-PTT epsilon=0.25 beta=0.59 "
+                        "Re=0 DfN=1e-3, Dirichlet rows applied",
+            "elements": ne, "unknowns": 6 * mesh.n_nodes, "nnz": nnz,
+            "wavenumbers": "A and B do not depend on the spanwise wavenumber in the reference (spanwise terms "
+                           "commented out); one assembly serves a batch of 16",
+            "element_kernel_ms": kern_ms_launch,
+            "scatter_ms": rest_ms / max(launches, 1),
+            "flops_per_element": stability_flops_per_element(),
+        },
+        "roofline": {
+            "kernel": "vef_stab_kernel (54 forward differences per element, A and B)",
+            "bound": "mfma",
+            "ceiling": "FP64 compute (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; no FMA "
+                       "contraction, so 0.5 of peak is its ceiling)",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": stability_traffic_from_profiles(ne),
+            "flops_per_launch": flops,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = stability_cpu_baseline(mesh, x, pset, sa.rowptr, sa.col, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def stability_cpu_baseline(mesh, x, pset, rowptr, col, target_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    prm = oracle.params(**pset)
+    probe = min(1000, mesh.n_elements)
+    t0 = time.perf_counter()
+    oracle.stab_assemble(prm, mesh, x, rowptr, col, el_range=(0, probe), nthreads=threads)
+    rate = probe / (time.perf_counter() - t0)
+    n = int(min(mesh.n_elements, max(probe, rate * target_s)))
+    t0 = time.perf_counter()
+    oracle.stab_assemble(prm, mesh, x, rowptr, col, el_range=(0, n), nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "elements/s", "cores": threads, "kind": "port",
+            "sample": f"elements 0..{n} of the same mesh and base state ({n} of {mesh.n_elements}), A and B by "
+                      f"forward differences + serial ADD_VALUES scatter, {dt:.1f} s"}
+
+
+def stability_traffic_from_profiles(n_elements):
+    path = os.path.join(ROOT, "profiles", "vef_stab_kernel_pmc_40k.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("elements") != n_elements:
+        return None
+    return d.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(mesh, x, target_s):
     """The CPU restatement (oracle) on the host cores, reference order: element
     work on OpenMP threads, the ADD_VALUES scatter serial."""
@@ -211,6 +341,9 @@ def main():
         return
     if args.newton:
         newton_bench(args)
+        return
+    if args.stability:
+        stability_bench(args)
         return
 
     import numpy as np  # noqa: F401

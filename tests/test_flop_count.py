@@ -40,3 +40,24 @@ def test_counter_sees_model_terms(oracle):
     inert = oracle.count_flops(oracle.params(ReN=1.0), xy, TL, 0.5)["total"]
     dcs = oracle.count_flops(oracle.params(DfN=0.0), xy, TL, 0.5)["total"]
     assert gies > base and inert > base and dcs > base
+
+
+def test_stability_flop_constants_match_oracle_count(oracle):
+    """bench.py's per-evaluation and base-state counts of the stability
+    assembly, for the --stability workload (L-PTT epsilon 0.25, Re = 0,
+    DfN = 1e-3), at every perturbed unknown of an element."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    rng = np.random.default_rng(3)
+    prm = oracle.params(ReN=0.0, BvN=0.59, VeN=0.25, DfN=1e-3, WiN0=1.0, model="L-PTT")
+    seen = set()
+    for p in (0, 7, 20, 33, 53):
+        xy = curved_xy(rng)
+        TLd = np.zeros((9, 6))
+        TLd.flat[p] = 1e-8
+        full, base = oracle.count_flops_stability(prm, xy, random_state(rng, xy), 0.6, TLd)
+        seen.add((full["total"], base["total"]))
+    assert seen == {(mod.STAB_FLOPS_PER_EVAL, mod.STAB_FLOPS_BASE)}
