@@ -85,13 +85,20 @@ def newton_bench(args):
     """Arclength continuation in Wi on the 1,600-element mesh: two pinned steps
     (Wi = 0.1, 0.12), then pseudo-arclength steps; value = mean wall time of a
     converged pseudo-arclength step (assembly, arclength and Dirichlet rows,
-    GPU re-factorisation and solves, tangent, predictor), the state resident on
-    the GPU.  The CPU baseline runs the same continuation with the oracle's
+    GPU factorisation and solves, tangent, predictor) of the reference's
+    Newton policy in FP64 (a fresh Jacobian at the start of every step), the
+    state resident on the GPU.  Reported beside it: the same policy with the
+    scalar-order factorisation, and the inexact Newton (the last factorisation
+    carried across steps) alone, with the learned predictor, and with the
+    learned predictor and an FP32 factorisation; each with its Wi gained per
+    second and the relative drag difference to an FP64 Newton solve at its
+    final Wi.  The CPU baseline runs the reference policy with the oracle's
     assembly and SuperLU on the host cores."""
     import numpy as np
     import torch
 
     from ve_flow_around_cylinder_fortran_amd import Params, workload_mesh
+    from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
     from ve_flow_around_cylinder_fortran_amd.mesh import boundary_edges
     from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation, NewtonSolver, initial_state
 
@@ -99,48 +106,56 @@ def newton_bench(args):
     LN0, dArN = 0.1, 0.02
     mesh = workload_mesh(1600)
     x0 = initial_state(mesh, LN0)
-    solver = NewtonSolver(mesh, Params(**pset))
-    cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN)
-    cont.start(x0)
-    times, iters, EU = [], [], []
-    for _ in range(args.arc_steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        x, hist = cont.step()
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-        iters.append("".join(h.flag[0] for h in hist))
-        EU.append(float(x[-1]))
-    solver.asm.set_drag_edges(boundary_edges(mesh, "Cylinder"))
-    Fd = float(solver.asm.drag(x).item())
-    arc = times[2:]
-    # the same continuation with the learned predictor (per-node MLP on the matrix cores, trained online)
-    from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
+    edges = boundary_edges(mesh, "Cylinder")
 
-    lcont = ArclengthContinuation(NewtonSolver(mesh, Params(**pset)), LN0=LN0, dArN=dArN,
-                                  predictor=LearnedPredictor(mesh, seed=0))
-    lcont.start(x0)
-    ltimes, liters, lEU = [], [], []
-    for _ in range(args.arc_steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xl, hist = lcont.step()
-        torch.cuda.synchronize()
-        ltimes.append(time.perf_counter() - t0)
-        liters.append("".join(h.flag[0] for h in hist))
-        lEU.append(float(xl[-1]))
-    learned = {"step_seconds": ltimes, "newton_flags_per_step": liters, "EU_per_step": lEU,
-               "choices": lcont.choices,
-               "Wi_gain_per_second": (lEU[-1] - lEU[1]) * pset["WiN0"] / sum(ltimes[2:]),
-               "tangent_Wi_gain_per_second": (EU[-1] - EU[1]) * pset["WiN0"] / sum(arc)}
+    def run(factorization="hybrid", reuse=False, learned=False, precision="f64"):
+        solver = NewtonSolver(mesh, Params(**pset), factorization=factorization, reuse_factors=reuse,
+                              linear_precision=precision)
+        cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN,
+                                     predictor=LearnedPredictor(mesh, seed=0, epochs=50) if learned else None)
+        cont.start(x0)
+        times, flags, EU = [], [], []
+        for _ in range(args.arc_steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, hist = cont.step()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            flags.append("".join(h.flag[0] for h in hist))
+            EU.append(float(x[-1]))
+        solver.asm.set_drag_edges(edges)
+        Fd = float(solver.asm.drag(x).item())
+        # FP64 Newton (reference policy) at the same Wi, started from this state
+        ref = NewtonSolver(mesh, Params(**pset))
+        xr, _ = ref.solve(x, float(x[-1]))
+        ref.asm.set_drag_edges(edges)
+        Fr = float(ref.asm.drag(xr).item())
+        arc = times[2:]
+        return {"s_per_step": float(np.mean(arc)), "step_seconds": times, "newton_flags_per_step": flags,
+                "EU_per_step": EU, "Wi_gain_per_second": (EU[-1] - EU[1]) * pset["WiN0"] / sum(arc),
+                "factorizations": solver.lu.refactorizations, "host_fallbacks": solver.lu.host_fallbacks,
+                "Fd_last_step": Fd, "drag_rel_diff_vs_fp64_newton": abs(Fd - Fr) / abs(Fr),
+                "choices": cont.choices if learned else None, "factor_pattern_nnz":
+                    getattr(solver.lu, "nnzS", None) or getattr(solver.lu, "nnzT", None)}
+
+    plain = run()
+    modes = {
+        "reference_policy_scalar_order_lu": run(factorization="rf"),
+        "inexact_newton": run(reuse=True),
+        "inexact_newton_learned": run(reuse=True, learned=True),
+        "inexact_newton_learned_fp32_factorization": run(reuse=True, learned=True, precision="f32"),
+    }
+    for m in modes.values():
+        m["speedup_per_step_vs_reference_policy"] = plain["s_per_step"] / m["s_per_step"]
+        m["Wi_per_second_vs_reference_policy"] = m["Wi_gain_per_second"] / plain["Wi_gain_per_second"]
     out = {
         "metric": "s per converged arclength step",
-        "value": float(np.mean(arc)),
+        "value": plain["s_per_step"],
         "unit": "s",
         "n_gpus": 1,
-        "steps": len(arc),
+        "steps": args.arc_steps - 2,
         "warmup": 2,
-        "ms_per_step": float(np.mean(arc)) * 1e3,
+        "ms_per_step": plain["s_per_step"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -149,12 +164,11 @@ def newton_bench(args):
         "config": {
             "workload": "cylinder-in-channel Q2 mesh, 1600 elements, Oldroyd-B beta=0.59 Re=0 DfN=1e-3, arclength "
                         f"continuation in Wi from {LN0} (pinned steps {LN0}, {LN0 + dArN}), Newton tolerance 1e-8 on "
-                        "||b|| and ||s||",
+                        "||b|| and ||s||; reference Newton policy, FP64, scalar re-factorisation in node-block "
+                        "order with block triangular solves",
             "elements": mesh.n_elements, "unknowns": mesh.n_unknowns,
-            "step_seconds": times, "newton_flags_per_step": iters, "EU_per_step": EU,
-            "factor_pattern_nnz": solver.lu.nnzT, "host_fallbacks": solver.lu.host_fallbacks,
-            "Fd_last_step": Fd,
-            "learned_predictor": learned,
+            "reference_policy": plain,
+            "modes": modes,
         },
     }
     if not args.no_cpu_baseline:
