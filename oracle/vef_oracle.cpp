@@ -853,11 +853,11 @@ static int scatter_element(const int32_t* nodes, const double* Res54, const doub
 //   momentumStabilitySQRT, continuityStability, stressStabilitySQRT
 //                             src/Stability/Residuals_Stability_mod.F90:125-153,209-313
 // The base state b is TEMP_TL_b = TL (real), the disturbance d is TEMP_TL_d
-// (complex; zero in storeBulkStability's base evaluation, Stability_mod.F90:58-62).
+// (complex; zero in storeBulkStability's base evaluation, Stability_mod.F90:54-58).
 // Every disturbance value has a zero imaginary part on this path (the
 // spanwise terms are commented out, Element_Calculations_Stability_mod.F90:51-81,
 // and eps_jac is the real part of perturbVariable's 1e-8+1e-8i, assigned to a
-// PetscReal, Equations_Stability_mod.F90:155,178), so the real restatement
+// PetscReal, Equations_Stability_mod.F90:155,179), so the real restatement
 // with the division rule cdiv() reproduces the reference's values.
 // A (Res_a) and B (Res_b, negated) are forward differences in the
 // disturbance.  The discontinuity-capturing terms multiply tdcs = 0 (DfN >
@@ -1102,7 +1102,7 @@ static void lsceMomentum(const Phys& ph, const GsPt& G, int i, const vreal dF[3]
 
 // momentumStabilitySQRT, continuityStability, stressStabilitySQRT
 // (Residuals_Stability_mod.F90:125-153,209-313) at test node n; TB already
-// negated (equationsResidualsBulkStability, Equations_Stability_mod.F90:123).
+// negated (equationsResidualsBulkStability, Equations_Stability_mod.F90:124).
 static void nodeResidualStability(const Phys& ph, const Elem& E, const GsPt& b, const GsPtD& d, int n, vreal TA[6],
                                   vreal TB[6]) {
   vreal bifn = vreal(E.bfn[n]);

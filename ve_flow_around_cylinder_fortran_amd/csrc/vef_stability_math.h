@@ -13,7 +13,7 @@
 // On this path every imaginary part is zero: the spanwise terms are commented
 // out (Element_Calculations_Stability_mod.F90:51-81) and eps_jac is the real
 // part of perturbVariable's value, assigned to a PetscReal
-// (Equations_Stability_mod.F90:155,178).  Divisions with a complex operand
+// (Equations_Stability_mod.F90:155,179).  Divisions with a complex operand
 // round as (a*c)/(c*c), real ones as a/c, every sum is the reference's
 // left-to-right sum, and the code is compiled with -ffp-contract=off.
 //
