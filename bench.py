@@ -89,8 +89,9 @@ def newton_bench(args):
     Newton policy in FP64 (a fresh Jacobian at the start of every step), the
     state resident on the GPU.  Reported beside it: the same policy with the
     scalar-order factorisation, and the inexact Newton (the last factorisation
-    carried across steps) alone, with the learned predictor, and with the
-    learned predictor and an FP32 factorisation; each with its Wi gained per
+    carried across steps) alone, with the learned predictor, with the learned
+    predictor and an FP32 factorisation, with the FP32 analytic Jacobian, and
+    with all three; each with its Wi gained per
     second and the relative drag difference to an FP64 Newton solve at its
     final Wi.  The CPU baseline runs the reference policy with the oracle's
     assembly and SuperLU on the host cores."""
@@ -108,9 +109,9 @@ def newton_bench(args):
     x0 = initial_state(mesh, LN0)
     edges = boundary_edges(mesh, "Cylinder")
 
-    def run(factorization="hybrid", reuse=False, learned=False, precision="f64"):
+    def run(factorization="hybrid", reuse=False, learned=False, precision="f64", jacobian="fd64"):
         solver = NewtonSolver(mesh, Params(**pset), factorization=factorization, reuse_factors=reuse,
-                              linear_precision=precision)
+                              linear_precision=precision, jacobian=jacobian)
         cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN,
                                      predictor=LearnedPredictor(mesh, seed=0, epochs=50) if learned else None)
         cont.start(x0)
@@ -144,6 +145,9 @@ def newton_bench(args):
         "inexact_newton": run(reuse=True),
         "inexact_newton_learned": run(reuse=True, learned=True),
         "inexact_newton_learned_fp32_factorization": run(reuse=True, learned=True, precision="f32"),
+        "inexact_newton_fp32_jacobian": run(reuse=True, jacobian="ad32"),
+        "inexact_newton_learned_fp32_jacobian_fp32_factorization": run(reuse=True, learned=True, precision="f32",
+                                                                        jacobian="ad32"),
     }
     for m in modes.values():
         m["speedup_per_step_vs_reference_policy"] = plain["s_per_step"] / m["s_per_step"]
@@ -394,6 +398,22 @@ def main():
     torch.cuda.synchronize()
     t3 = time.perf_counter()
 
+    # the same assembly with the FP32 analytic Jacobian (VEF_RES_JAC_FP32), reported beside
+    v32 = torch.empty_like(v)
+    for _ in range(args.warmup):
+        asm.assemble(xd, "NRP32", b=b, values=v32)
+    asm.profile(True)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    for _ in range(args.steps):
+        asm.assemble(xd, "NRP32", b=b, values=v32)
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    jac32_ms, scatter32_ms, launches32 = asm.profile_read()
+    asm.profile(False)
+    rel32 = float(torch.linalg.vector_norm(v32 - v) / torch.linalg.vector_norm(v))
+    del v32
+
     ms_step = (t1 - t0) * 1e3 / args.steps
     ne = mesh.n_elements
     bulk_ms_launch = bulk_ms / max(launches, 1)
@@ -418,6 +438,15 @@ def main():
                         "Oldroyd-B Wi=0.5 beta=0.59 Re=0 DfN=1e-3, fixed CSR pattern",
             "elements": ne, "unknowns": asm.N, "nnz": asm.nnz,
             "residual_only_ms": (t3 - t2) * 1e3 / args.steps,
+            "fp32_jacobian": {
+                "ms_per_step": (t5 - t4) * 1e3 / args.steps,
+                "elements_per_s": ne / ((t5 - t4) / args.steps),
+                "jacobian_kernel_ms": jac32_ms / max(launches32, 1),
+                "scatter_ms": scatter32_ms / max(launches32, 1),
+                "relative_difference_vs_fp64_fd": rel32,
+                "note": "VEF_RES_JAC_FP32: the same FP64 residual, the Jacobian from forward-mode "
+                        "differentiation of the Gauss-point terms in FP32 contracted with the basis functions",
+            },
             "element_kernel_ms": bulk_ms_launch,
             "scatter_ms": scatter_ms / max(launches, 1),
             "total_tflops": total_flops / (ms_step * 1e-3) / 1e12,

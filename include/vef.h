@@ -43,8 +43,14 @@ extern "C" {
 typedef struct vef_handle vef_handle;
 
 enum { VEF_MODEL_LPTT = 0, VEF_MODEL_GIESEKUS = 1 };
-/* Flag_NR: 'MNR' residual only, 'NRP' residual and Jacobian (src/Equations_mod.F90:264) */
-enum { VEF_RES_ONLY = 0, VEF_RES_JAC = 1 };
+/* Flag_NR: 'MNR' residual only, 'NRP' residual and Jacobian (src/Equations_mod.F90:264).
+ * VEF_RES_JAC_FP32 has no counterpart in the reference: the residual as with
+ * VEF_RES_JAC (FP64, the reference's arithmetic) and an analytic Jacobian of
+ * the element residual computed in FP32 (forward-mode differentiation of the
+ * Gauss-point terms, contraction with the basis functions) in place of the
+ * forward differences, for an inexact Newton; values are stored as FP64.
+ * It needs DfN > 1e-12. */
+enum { VEF_RES_ONLY = 0, VEF_RES_JAC = 1, VEF_RES_JAC_FP32 = 2 };
 enum {
   VEF_OK = 0,
   VEF_ERR_ARG = 1,
@@ -96,16 +102,18 @@ const int32_t* vef_device_colind(const vef_handle* h);
  *   x[N]       state (TL node-major, EU last)
  *   b[N]       residual: zeroed, then element and outlet-edge contributions
  *              added in the reference's ADD_VALUES order
- *   values[nnz] matrix values (only written when flag == VEF_RES_JAC):
- *              zeroed, then the forward-difference element Jacobians added in
- *              the same order; column N-1 holds dR/dEU.  Row N-1 stays zero
- *              (the caller's arclength row goes there).
+ *   values[nnz] matrix values (written with VEF_RES_JAC or VEF_RES_JAC_FP32,
+ *              may be NULL with VEF_RES_ONLY): zeroed, then the element
+ *              Jacobians (forward differences, or the FP32 analytic ones)
+ *              added in the same order; column N-1 holds dR/dEU.  Row N-1
+ *              stays zero (the caller's arclength row goes there).
  * Replaces initializeLSObjects + setEquationsBulk + setEquationsEdge for the
  * Cylinder_2D outlet (src/Newton_Raphson_mod.F90:101,332-357). */
 int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* hip_stream);
 
 /* The arclength row N-1, applied after vef_assemble (arclength and
  * pseudoArclength, src/Extra_Equations_mod.F90:698-828).
+ *   flag: VEF_RES_JAC and VEF_RES_JAC_FP32 write the row of values alike.
  *   VEF_ARC_PIN (continuation steps 1-2): b[N-1] = x[N-1] - (LN0 + (increment-1)*dArN);
  *     with VEF_RES_JAC, row and column N-1 are zeroed and the diagonal set to 1
  *     (MatZeroRowsColumns).  x_old and tangent may be NULL.
@@ -125,9 +133,9 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
  * vef_set_dirichlet stores rows (host int64[n]: global rows 6*gnod + ieq) with
  * their prescribed values (host double[n]); a row listed more than once keeps
  * its last value, as the reference's boundary-by-boundary INSERT_VALUES does.
- * vef_apply_dirichlet sets b[row] = x[row] - value and, with VEF_RES_JAC,
- * zeroes the row's stored values and puts 1 on its diagonal (MatZeroRows with
- * diag = 1). */
+ * vef_apply_dirichlet sets b[row] = x[row] - value and, with VEF_RES_JAC (or
+ * VEF_RES_JAC_FP32), zeroes the row's stored values and puts 1 on its diagonal
+ * (MatZeroRows with diag = 1). */
 int vef_set_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows, const double* values);
 int vef_apply_dirichlet(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* hip_stream);
 

@@ -1,0 +1,57 @@
+// Host build of the analytic Jacobian's Gauss-point function
+// (ve_flow_around_cylinder_fortran_amd/csrc/vef_ad_math.h) for a CPU test:
+// the element residual it implies and its forward-mode Jacobian in double,
+// from caller-supplied Gauss-point geometry and basis tables.
+#include "vef_ad_math.h"
+
+namespace {
+using vefad::Dual;
+using vefad::kNumFields;
+
+void fields(const double* u, const double* bf, const double* dFx, const double* dFy, const double* d2x,
+            const double* d2y, double EU, double* F) {
+  const double* basis[5] = {bf, dFx, dFy, d2x, d2y};
+  for (int f = 0; f < vefad::kNumInterp; f++) {
+    const int q = vefad::field_unknown(f), k = vefad::field_basis(f);
+    double s = 0.0;
+    for (int n = 0; n < 9; n++) s += u[n * 6 + q] * basis[k][n];
+    F[f] = s;
+  }
+  F[vefad::kEU] = EU;
+}
+}  // namespace
+
+extern "C" int ad_host_element(const double* prm5, int model, const double* u54, double EU, const double* bf,
+                               const double* dFx, const double* dFy, const double* d2x, const double* d2y,
+                               const double* H2, const double* WET, double* R, double* J, double* Jeu) {
+  vefad::Params<double> prm{prm5[0], prm5[1], prm5[2], prm5[3], prm5[4], 1.0 - prm5[1], model};
+  for (int i = 0; i < 54; i++) R[i] = 0.0, Jeu[i] = 0.0;
+  for (int i = 0; i < 54 * 54; i++) J[i] = 0.0;
+  for (int g = 0; g < 9; g++) {
+    const double *b = bf + 9 * g, *fx = dFx + 9 * g, *fy = dFy + 9 * g, *gx = d2x + 9 * g, *gy = d2y + 9 * g;
+    double F[kNumFields];
+    fields(u54, b, fx, fy, gx, gy, EU, F);
+    double T[3][6];
+    vefad::gauss_terms(F, fx, fy, H2[g], prm, T);
+    for (int n = 0; n < 9; n++)
+      for (int q = 0; q < 6; q++) R[n * 6 + q] += WET[g] * (b[n] * T[0][q] + fx[n] * T[1][q] + fy[n] * T[2][q]);
+    const double* basis[5] = {b, fx, fy, gx, gy};
+    for (int f = 0; f < kNumFields; f++) {
+      Dual<double> Fd[kNumFields];
+      for (int i = 0; i < kNumFields; i++) Fd[i] = Dual<double>(F[i], i == f ? 1.0 : 0.0);
+      Dual<double> Td[3][6];
+      vefad::gauss_terms(Fd, fx, fy, H2[g], prm, Td);
+      for (int n = 0; n < 9; n++)
+        for (int q = 0; q < 6; q++) {
+          const double dr = WET[g] * (b[n] * Td[0][q].d + fx[n] * Td[1][q].d + fy[n] * Td[2][q].d);
+          if (f == vefad::kEU) {
+            Jeu[n * 6 + q] += dr;
+          } else {
+            const int p = vefad::field_unknown(f), k = vefad::field_basis(f);
+            for (int m = 0; m < 9; m++) J[(m * 6 + p) * 54 + n * 6 + q] += dr * basis[k][m];
+          }
+        }
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,92 @@
+"""The analytic Jacobian's Gauss-point function (csrc/vef_ad_math.h, the
+coefficients T of the test functions as functions of the interpolated
+fields) built for the host in double (tests/jacobian/ad_host_check.hip):
+
+* the element residual it implies equals the oracle's residualBulk to 1e-12
+  of the residual's scale (pins T against the reference restatement: every
+  term, model and inertia set);
+* its forward-mode Jacobian equals the oracle's forward-difference Jacobian
+  (storeBulk, eps_jac = 1e-8) and dR/dEU to the difference-quotient error
+  (1e-6 of the matrix scale, norm-wise), and dropping a single derivative
+  (one column) is caught at that tolerance."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_oracle_element import PARAM_SETS as ELEMENT_SETS
+from test_oracle_element import curved_xy, random_state
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(HERE), "ve_flow_around_cylinder_fortran_amd", "csrc")
+PSETS = [p for p in ELEMENT_SETS if p["DfN"] > 1e-12]
+
+
+@pytest.fixture(scope="module")
+def ad_lib(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path_factory.mktemp("ad") / "libad_host.so")
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-I", CSRC, "-o", out,
+                    os.path.join(HERE, "jacobian", "ad_host_check.hip")], check=True, capture_output=True)
+    L = ctypes.CDLL(out)
+    L.ad_host_element.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double] + \
+        [ctypes.c_void_p] * 10
+    return L
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def ad_element(L, oracle, pset, xy, TL, EU):
+    tab = oracle.tables()
+    bf = np.ascontiguousarray(tab["bfn"])
+    dFx, dFy, d2x, d2y = (np.zeros((9, 9)) for _ in range(4))
+    H2, WET = np.zeros(9), np.zeros(9)
+    for g in range(9):
+        geo = oracle.basis2d(xy, g)
+        dFx[g], dFy[g] = geo["dFdXi"]
+        d2x[g], d2y[g] = geo["d2FdXi2"]
+        H2[g] = 1.0 / np.sum(dFx[g] ** 2 + dFy[g] ** 2)
+        WET[g] = geo["WET"]
+    prm5 = np.array([pset["ReN"], pset["BvN"], pset["VeN"], pset["DfN"], pset["WiN0"]])
+    R, J, Jeu = np.zeros(54), np.zeros((54, 54)), np.zeros(54)
+    u = np.ascontiguousarray(TL, dtype=np.float64)
+    L.ad_host_element(_p(prm5), 0 if pset["model"] == "L-PTT" else 1, _p(u), float(EU), _p(bf), _p(dFx), _p(dFy),
+                      _p(d2x), _p(d2y), _p(H2), _p(WET), _p(R), _p(J), _p(Jeu))
+    return R, J, Jeu
+
+
+@pytest.mark.parametrize("pset", PSETS)
+def test_residual_and_jacobian_match_oracle(ad_lib, oracle, pset):
+    prm = oracle.params(**pset)
+    rng = np.random.default_rng(31)
+    for trial in range(3):
+        xy = curved_xy(rng)
+        TL = random_state(rng, xy)
+        EU = 0.6 + 0.2 * trial
+        Res, A, Ac = oracle.element(prm, xy, TL, EU)
+        R, J, Jeu = ad_element(ad_lib, oracle, pset, xy, TL, EU)
+        assert np.abs(R - Res.reshape(-1)).max() <= 1e-12 * np.abs(Res).max()
+        assert np.linalg.norm(J - A) <= 1e-6 * np.linalg.norm(A), np.linalg.norm(J - A) / np.linalg.norm(A)
+        assert np.linalg.norm(Jeu - Ac) <= 1e-6 * np.linalg.norm(Ac)
+
+
+def test_fd_tolerance_catches_a_dropped_derivative(ad_lib, oracle):
+    """Dropping a single derivative (the S11 column of the first node) fails
+    the tolerance above by more than a factor of ten, so it is not vacuous."""
+    pset = PSETS[0]
+    prm = oracle.params(**pset)
+    rng = np.random.default_rng(32)
+    xy = curved_xy(rng)
+    TL = random_state(rng, xy)
+    _, A, _ = oracle.element(prm, xy, TL, 0.7)
+    _, J, _ = ad_element(ad_lib, oracle, pset, xy, TL, 0.7)
+    J2 = J.copy()
+    J2[:, 3] = 0.0
+    assert np.linalg.norm(J2 - A) > 1e-5 * np.linalg.norm(A)

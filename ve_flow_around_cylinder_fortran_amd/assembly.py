@@ -35,6 +35,7 @@ VEF_MODEL_LPTT = 0
 VEF_MODEL_GIESEKUS = 1
 VEF_RES_ONLY = 0
 VEF_RES_JAC = 1
+VEF_RES_JAC_FP32 = 2
 MODELS = {"L-PTT": VEF_MODEL_LPTT, "GIESEKUS": VEF_MODEL_GIESEKUS}
 
 
@@ -213,7 +214,7 @@ class Assembler:
         dev = x.device
         if b is None:
             b = torch.empty(self.N, dtype=torch.float64, device=dev)
-        if flag == VEF_RES_JAC and values is None:
+        if flag != VEF_RES_ONLY and values is None:
             values = torch.empty(self.nnz, dtype=torch.float64, device=dev)
         if stream is None:
             stream = torch.cuda.current_stream(dev)
@@ -221,7 +222,7 @@ class Assembler:
                                    ctypes.c_void_p(values.data_ptr()) if values is not None else None,
                                    ctypes.c_void_p(stream.cuda_stream))
         self._check(rc, "vef_assemble")
-        return b, (values if flag == VEF_RES_JAC else None)
+        return b, (values if flag != VEF_RES_ONLY else None)
 
     def arclength(self, x, b, values=None, *, increment: int, LN0: float = 0.0, dArN: float = 0.0, x_old=None,
                   tangent=None, dS: float = 0.0, flag_nr: str = "NRP", stream=None):
@@ -229,7 +230,7 @@ class Assembler:
         for increments 1-2, pseudo-arclength row from increment 3 on."""
         import torch
 
-        flag = self._flag(flag_nr)
+        flag = self._flag(flag_nr)  # the row is the same for either Jacobian
         mode = 0 if increment <= 2 else 1
         if stream is None:
             stream = torch.cuda.current_stream(x.device)
@@ -249,7 +250,8 @@ class Assembler:
         self._check(self.lib.vef_set_dirichlet(self._h, len(r), _ptr(r), _ptr(v)), "vef_set_dirichlet")
 
     def apply_dirichlet(self, x, b, values=None, flag_nr: str = "NRP", stream=None):
-        """b[row] = x[row] - value; with 'NRP' the rows of values become unit rows."""
+        """b[row] = x[row] - value; with 'NRP' (or 'NRP32') the rows of values
+        become unit rows."""
         import torch
 
         flag = self._flag(flag_nr)
@@ -339,7 +341,7 @@ class Assembler:
         if x.shape != (self.N,):
             raise VefError("x must have length N")
         b = np.empty(self.N)
-        values = np.empty(self.nnz) if flag == VEF_RES_JAC else None
+        values = np.empty(self.nnz) if flag != VEF_RES_ONLY else None
         rc = self.lib.vef_assemble_host(self._h, _ptr(x), flag, _ptr(b), _ptr(values) if values is not None else None)
         self._check(rc, "vef_assemble_host")
         return b, values
@@ -360,4 +362,6 @@ class Assembler:
             return VEF_RES_JAC
         if flag_nr == "MNR":
             return VEF_RES_ONLY
-        raise VefError(f"Flag_NR must be 'NRP' or 'MNR', got {flag_nr!r}")
+        if flag_nr == "NRP32":  # FP32 analytic Jacobian (no counterpart in the reference)
+            return VEF_RES_JAC_FP32
+        raise VefError(f"Flag_NR must be 'NRP', 'MNR' or 'NRP32', got {flag_nr!r}")

@@ -50,6 +50,7 @@
 
 #include "vef.h"
 #include "vef_stability_math.h"
+#include "vef_ad_math.h"
 
 namespace {
 
@@ -777,6 +778,146 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     return dc.failed();
   };
   if (__ballot(evaluate(std::false_type{})) != 0) evaluate(std::true_type{});
+}
+
+// ----------------------------------------------------------------------------
+// Analytic FP32 Jacobian (VEF_RES_JAC_FP32): one 256-thread workgroup per
+// element.  (1) The 22 fields at the 9 Gauss points in FP32; (2) 207 lanes =
+// 9 Gauss points x 23 directions (the fields and EU) evaluate the Gauss-point
+// coefficients T(F) of the test functions (vef_ad_math.h) with one-component
+// dual numbers, giving D = WET * dT/dF in LDS; (3) every matrix entry is the
+// contraction sum_g sum_k phi^n_k(g) sum_{f in p} D[g][f][k,q] beta^m_f(g)
+// (FP32 accumulation), stored through the same scatter map as the
+// forward-difference pass; the EU direction gives dR/dEU into acbank.
+// ----------------------------------------------------------------------------
+// Field index of basis kind j (0 value, 1 d/dx, 2 d/dy, 3 d2/dx2, 4 d2/dy2) of
+// unknown p (-1 where the unknown has no such field).
+__host__ __device__ constexpr int field_of(int p, int j) {
+  return p == 0 ? (j == 0 ? vefad::kUx : j == 1 ? vefad::kGU00 : j == 2 ? vefad::kGU10 : j == 3 ? vefad::kG2x0 : vefad::kG2y0)
+       : p == 1 ? (j == 0 ? vefad::kUy : j == 1 ? vefad::kGU01 : j == 2 ? vefad::kGU11 : j == 3 ? vefad::kG2x1 : vefad::kG2y1)
+       : j > 2  ? -1
+       : p == 2 ? (j == 0 ? vefad::kP : j == 1 ? vefad::kPx : vefad::kPy)
+       : p == 3 ? (j == 0 ? vefad::kS00 : j == 1 ? vefad::kS00x : vefad::kS00y)
+       : p == 4 ? (j == 0 ? vefad::kS01 : j == 1 ? vefad::kS01x : vefad::kS01y)
+                : (j == 0 ? vefad::kS11 : j == 1 ? vefad::kS11x : vefad::kS11y);
+}
+
+__global__ void __launch_bounds__(256)
+vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
+                 int64_t eu_index, int32_t n_el, vefad::Params<float> prm, FdOut fo) {
+  constexpr int NF = vefad::kNumFields;
+  constexpr int NOUT = 2916 + 54;          // matrix entries and the dR/dEU column
+  constexpr int PER = (NOUT + 255) / 256;  // entries per thread
+  __shared__ double sU[54];
+  __shared__ double sX[18];
+  __shared__ Geo sG[9];
+  __shared__ float sBas[9][5][9];  // [g][basis kind][node]
+  __shared__ float sWET[9], sH2[9];
+  __shared__ float sF[9][NF];
+  __shared__ float sD[9][NF][18];  // WET * dT[k][q]/dF_f
+  __shared__ float sM[18 * 54];    // one Gauss point's D.B: [k*6+q][m*6+p]
+  __shared__ int32_t sCode[81];
+  __shared__ int64_t sRb[9];
+  __shared__ int32_t sRl[9];
+  const int tid = threadIdx.x;
+  const int64_t e = blockIdx.x;
+  if (e >= n_el) return;  // uniform over the workgroup
+  const double EU = x[eu_index];
+  if (tid < 54) sU[tid] = x[(int64_t)conn[e * 9 + tid / 6] * 6 + tid % 6];
+  if (tid < 18) sX[tid] = coords[(int64_t)conn[e * 9 + tid / 2] * 2 + (tid & 1)];
+  if (tid < 81) sCode[tid] = fo.code[e * 81 + tid];
+  if (tid < 9) {
+    const int64_t r6 = (int64_t)conn[e * 9 + tid] * 6;
+    const int64_t rb = fo.rowptr[r6];
+    sRb[tid] = rb;
+    sRl[tid] = (int32_t)(fo.rowptr[r6 + 1] - rb);
+  }
+  __syncthreads();
+  if (tid < 9) {
+    Geo& G = sG[tid];
+    geometry(sX, c_tab.dc[tid], c_tab.de[tid], c_tab.dcc[tid], c_tab.dee[tid], c_tab.dce[tid], c_tab.W2[tid], G);
+    for (int n = 0; n < 9; n++) {
+      sBas[tid][0][n] = (float)c_tab.bf[tid][n];
+      sBas[tid][1][n] = (float)G.dFx[n];
+      sBas[tid][2][n] = (float)G.dFy[n];
+      sBas[tid][3][n] = (float)G.d2x[n];
+      sBas[tid][4][n] = (float)G.d2y[n];
+    }
+    sWET[tid] = (float)G.WET;
+    sH2[tid] = (float)G.H2;
+  }
+  __syncthreads();
+  // (1) the fields at the Gauss points
+  if (tid < 9 * vefad::kNumInterp) {
+    const int g = tid / vefad::kNumInterp, f = tid % vefad::kNumInterp;
+    const int q = vefad::field_unknown(f), k = vefad::field_basis(f);
+    float acc = 0.0f;
+    for (int n = 0; n < 9; n++) acc += (float)sU[n * 6 + q] * sBas[g][k][n];
+    sF[g][f] = acc;
+  } else if (tid < 9 * vefad::kNumInterp + 9) {
+    sF[tid - 9 * vefad::kNumInterp][vefad::kEU] = (float)EU;
+  }
+  __syncthreads();
+  // (2) D = WET * dT/dF, one direction per lane
+  if (tid < 9 * NF) {
+    const int g = tid / NF, f = tid % NF;
+    vefad::Dual<float> F[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) F[i] = vefad::Dual<float>(sF[g][i], i == f ? 1.0f : 0.0f);
+    vefad::Dual<float> T[3][6];
+    vefad::gauss_terms(F, sBas[g][1], sBas[g][2], sH2[g], prm, T);
+    const float w = sWET[g];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int q = 0; q < 6; q++) sD[g][f][k * 6 + q] = w * T[k][q].d;
+  }
+  // (3) J = sum_g Phi_g^T (D_g B_g): per Gauss point, M = D.B into LDS, then
+  //     each thread adds Phi^T M to its PER entries (FP32 accumulation)
+  float acc[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) acc[r] = 0.0f;
+  for (int g = 0; g < 9; g++) {
+    __syncthreads();  // D is complete / the previous M has been read
+    for (int t = tid; t < 18 * 54; t += 256) {
+      const int kq = t / 54, m = (t % 54) / 6, p = t % 6;
+      float v = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        const int f = field_of(p, j);
+        if (f >= 0) v += sD[g][f][kq] * sBas[g][j][m];
+      }
+      sM[t] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int o = tid + 256 * r;
+      if (o < 2916) {
+        const int n = o / 324, q = (o / 54) % 6, mp = o % 54;
+        acc[r] += sBas[g][0][n] * sM[q * 54 + mp] + sBas[g][1][n] * sM[(6 + q) * 54 + mp] +
+                  sBas[g][2][n] * sM[(12 + q) * 54 + mp];
+      } else if (o < NOUT) {
+        const int n = (o - 2916) / 6, q = (o - 2916) % 6;
+        const float* d = sD[g][vefad::kEU];
+        acc[r] += sBas[g][0][n] * d[q] + sBas[g][1][n] * d[6 + q] + sBas[g][2][n] * d[12 + q];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int o = tid + 256 * r;
+    if (o < 2916) {
+      const int n = o / 324, q = (o / 54) % 6, m = (o % 54) / 6, p = o % 6;  // row (n,q), column (m,p)
+      const int32_t cd = sCode[n * 9 + m];
+      if ((cd & 1) == 0)
+        fo.values[sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 1) * 6 + p] = 0.0 + (double)acc[r];
+      else
+        fo.cbank[(int64_t)(cd >> 1) * 36 + q * 6 + p] = (double)acc[r];
+    } else if (o < NOUT) {
+      fo.acbank[e * 54 + (o - 2916)] = (double)acc[r];
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -1708,11 +1849,14 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
 int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* stream) {
   if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
   if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
-  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "flag must be VEF_RES_ONLY or VEF_RES_JAC");
-  if (flag == VEF_RES_JAC && !values) return set_err(h, VEF_ERR_ARG, "values is required for VEF_RES_JAC");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC && flag != VEF_RES_JAC_FP32)
+    return set_err(h, VEF_ERR_ARG, "flag must be VEF_RES_ONLY, VEF_RES_JAC or VEF_RES_JAC_FP32");
+  if (flag != VEF_RES_ONLY && !values) return set_err(h, VEF_ERR_ARG, "values is required with a Jacobian flag");
+  if (flag == VEF_RES_JAC_FP32 && h->dprm.dcs_on)
+    return set_err(h, VEF_ERR_UNSUPPORTED, "the FP32 analytic Jacobian needs DfN > 1e-12 (discontinuity capturing off)");
   hipStream_t st = (hipStream_t)stream;
   const int64_t eu = h->N - 1;
-  const int jac = flag == VEF_RES_JAC;
+  const int jac = flag != VEF_RES_ONLY;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   if (h->profiling) {
     for (int i = 0; i < 4; i++) VEF_HIP(h, hipEventCreate(&ev[i]));
@@ -1728,9 +1872,17 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
     VEF_HIP(h, hipGetLastError());
   }
   if (h->profiling) VEF_HIP(h, hipEventRecord(ev[1], st));
-  // 3. FD element Jacobians: single-contributor entries straight into values
-  if (jac) {
+  // 3. element Jacobians (forward differences, or the FP32 analytic one):
+  //    single-contributor entries straight into values
+  if (flag == VEF_RES_JAC) {
     launch_bulk(h, 1, values, x, eu, st);
+    VEF_HIP(h, hipGetLastError());
+  } else if (flag == VEF_RES_JAC_FP32) {
+    FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank};
+    vefad::Params<float> prm{(float)h->params.ReN, (float)h->params.BvN, (float)h->params.VeN, (float)h->params.DfN,
+                             (float)h->params.WiN0, (float)(1.0 - h->params.BvN), h->params.model};
+    hipLaunchKernelGGL(vef_jac32_kernel, dim3((unsigned)h->n_el), dim3(256), 0, st, h->d_coords, h->d_conn, x, eu,
+                       h->n_el, prm, fo);
     VEF_HIP(h, hipGetLastError());
   }
   if (h->profiling) VEF_HIP(h, hipEventRecord(ev[2], st));
@@ -1819,8 +1971,10 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
                   void* stream) {
   if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
   if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
-  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "bad flag");
-  if (flag == VEF_RES_JAC && !values) return set_err(h, VEF_ERR_ARG, "values is required for VEF_RES_JAC");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC && flag != VEF_RES_JAC_FP32)
+    return set_err(h, VEF_ERR_ARG, "bad flag");
+  if (flag != VEF_RES_ONLY && !values) return set_err(h, VEF_ERR_ARG, "values is required with a Jacobian flag");
+  if (flag == VEF_RES_JAC_FP32) flag = VEF_RES_JAC;  // the row itself is the same for both Jacobians
   hipStream_t st = (hipStream_t)stream;
   const int64_t N = h->N;
   if (mode == VEF_ARC_PIN) {
@@ -1879,8 +2033,10 @@ int vef_set_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows, const 
 int vef_apply_dirichlet(vef_handle* h, const double* x, int32_t flag, double* b, double* values, void* stream) {
   if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
   if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
-  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC) return set_err(h, VEF_ERR_ARG, "bad flag");
-  if (flag == VEF_RES_JAC && !values) return set_err(h, VEF_ERR_ARG, "values is required for VEF_RES_JAC");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC && flag != VEF_RES_JAC_FP32)
+    return set_err(h, VEF_ERR_ARG, "bad flag");
+  if (flag != VEF_RES_ONLY && !values) return set_err(h, VEF_ERR_ARG, "values is required with a Jacobian flag");
+  if (flag == VEF_RES_JAC_FP32) flag = VEF_RES_JAC;
   if (h->n_bc == 0) return VEF_OK;
   const int64_t blocks = (h->n_bc + 3) / 4;
   hipLaunchKernelGGL(vef_dirichlet_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, h->n_bc,
@@ -2040,18 +2196,21 @@ int vef_stability_assemble(vef_handle* h, const double* x, double* A, double* B,
 
 int vef_assemble_host(vef_handle* h, const double* x, int32_t flag, double* b, double* values) {
   if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
-  if (!x || !b || (flag == VEF_RES_JAC && !values)) return set_err(h, VEF_ERR_ARG, "missing array");
+  if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC && flag != VEF_RES_JAC_FP32)
+    return set_err(h, VEF_ERR_ARG, "bad flag");
+  const bool jac = flag != VEF_RES_ONLY;
+  if (!x || !b || (jac && !values)) return set_err(h, VEF_ERR_ARG, "missing array");
   if (!h->d_tmp_x) {
     VEF_HIP(h, hipMalloc((void**)&h->d_tmp_x, sizeof(double) * h->N));
     VEF_HIP(h, hipMalloc((void**)&h->d_tmp_b, sizeof(double) * h->N));
   }
-  if (flag == VEF_RES_JAC && !h->d_tmp_v) VEF_HIP(h, hipMalloc((void**)&h->d_tmp_v, sizeof(double) * h->nnz));
+  if (jac && !h->d_tmp_v) VEF_HIP(h, hipMalloc((void**)&h->d_tmp_v, sizeof(double) * h->nnz));
   VEF_HIP(h, hipMemcpy(h->d_tmp_x, x, sizeof(double) * h->N, hipMemcpyHostToDevice));
-  int rc = vef_assemble(h, h->d_tmp_x, flag, h->d_tmp_b, flag == VEF_RES_JAC ? h->d_tmp_v : nullptr, nullptr);
+  int rc = vef_assemble(h, h->d_tmp_x, flag, h->d_tmp_b, jac ? h->d_tmp_v : nullptr, nullptr);
   if (rc) return rc;
   VEF_HIP(h, hipDeviceSynchronize());
   VEF_HIP(h, hipMemcpy(b, h->d_tmp_b, sizeof(double) * h->N, hipMemcpyDeviceToHost));
-  if (flag == VEF_RES_JAC) VEF_HIP(h, hipMemcpy(values, h->d_tmp_v, sizeof(double) * h->nnz, hipMemcpyDeviceToHost));
+  if (jac) VEF_HIP(h, hipMemcpy(values, h->d_tmp_v, sizeof(double) * h->nnz, hipMemcpyDeviceToHost));
   return VEF_OK;
 }
 

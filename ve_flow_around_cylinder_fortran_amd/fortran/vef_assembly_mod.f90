@@ -15,7 +15,7 @@ module vef_assembly_mod
     implicit none
 
     integer(c_int32_t), parameter :: VEF_MODEL_LPTT = 0, VEF_MODEL_GIESEKUS = 1
-    integer(c_int32_t), parameter :: VEF_RES_ONLY = 0, VEF_RES_JAC = 1
+    integer(c_int32_t), parameter :: VEF_RES_ONLY = 0, VEF_RES_JAC = 1, VEF_RES_JAC_FP32 = 2
 
     type, bind(C) :: vef_params
         real(c_double) :: ReN, BvN, VeN, DfN, WiN0
@@ -117,7 +117,8 @@ module vef_assembly_mod
 
     ! Sol%TL(Nnodes, Neq) and Sol%EU(Nex) (src/Global_Variables_mod.F90:28-40)
     ! -> the global vector x(6*(g-1)+q) = TL(g,q), x(N) = EU(Nex), then one
-    ! assembly.  Flag_NR is 'NRP' (residual and Jacobian) or 'MNR'.
+    ! assembly.  Flag_NR is 'NRP' (residual and Jacobian), 'MNR', or 'NRP32'
+    ! (the FP64 residual with the FP32 analytic Jacobian, for an inexact Newton).
     subroutine vef_assemble_tl(handle, TL, EU, Flag_NR, b, values, ierr)
 
         type(c_ptr), intent(in) :: handle
@@ -146,6 +147,8 @@ module vef_assembly_mod
             flag = VEF_RES_JAC
         case ('MNR')
             flag = VEF_RES_ONLY
+        case ('NRP32')
+            flag = VEF_RES_JAC_FP32
         case default
             ierr = -1
             return

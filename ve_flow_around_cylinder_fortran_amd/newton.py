@@ -121,7 +121,8 @@ class NewtonSolver:
     (the Newton iteration absorbs a less accurate solve)."""
 
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
-                 reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = "hybrid"):
+                 reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = "hybrid",
+                 jacobian: str = "fd64"):
         import torch
 
         from .linsolve import BlockLU, HybridLU, RfLU
@@ -144,6 +145,9 @@ class NewtonSolver:
         self.N = self.asm.N
         self.b = torch.empty(self.N, dtype=torch.float64, device=device)
         self.values = torch.empty(self.asm.nnz, dtype=torch.float64, device=device)
+        if jacobian not in ("fd64", "ad32"):
+            raise NewtonError("jacobian must be 'fd64' (the reference's forward differences) or 'ad32' (analytic, FP32)")
+        self.jac_flag = "NRP" if jacobian == "fd64" else "NRP32"
         self.reuse_factors = reuse_factors
         self.reuse_max_iters = reuse_max_iters
         self.factored = False
@@ -176,7 +180,7 @@ class NewtonSolver:
                 rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * EU)
                 self.asm.set_dirichlet(rows, vals)
                 last_EU = EU
-            self.asm.assemble(x, flag, b=self.b, values=self.values if jac else None)
+            self.asm.assemble(x, self.jac_flag if jac else flag, b=self.b, values=self.values if jac else None)
             self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
                                x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
             self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
