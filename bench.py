@@ -36,10 +36,14 @@ SCATTER_FLOPS = 2916 + 2 * 54     # one add per ADD_VALUES contribution
 # Linear stability assembly (A and B): one residualBulkStability evaluation
 # counts STAB_FLOPS_PER_EVAL operations, STAB_FLOPS_BASE of them for the base
 # state's Gauss-point quantities, which the GPU computes once per element;
-# each of the 54 forward differences repeats the rest.  Quotients: sub, mul,
-# div per entry of A and B.  Pinned by tests/test_flop_count.py.
+# each of the 54 forward differences repeats the rest except the test-node
+# coefficients that depend on the base state only (STAB_COEF_FLOPS per Gauss
+# point and test node, computed once per element).  Quotients: sub, mul, div
+# per entry of A and B.  Pinned by tests/test_flop_count.py and
+# tests/test_stability_host_math.py.
 STAB_FLOPS_PER_EVAL = 35289
 STAB_FLOPS_BASE = 9261
+STAB_COEF_FLOPS = 79
 STAB_QUOTIENT_FLOPS = 2 * 2916 * 3
 PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector = matrix), FMA counted as 2
 PEAK_HBM_GBS = 8000.0
@@ -195,7 +199,8 @@ def newton_bench(args):
 
 
 def stability_flops_per_element():
-    return STAB_FLOPS_BASE + 54 * (STAB_FLOPS_PER_EVAL - STAB_FLOPS_BASE) + STAB_QUOTIENT_FLOPS
+    coef = 81 * STAB_COEF_FLOPS
+    return STAB_FLOPS_BASE + coef + 54 * (STAB_FLOPS_PER_EVAL - STAB_FLOPS_BASE - coef) + STAB_QUOTIENT_FLOPS
 
 
 def stability_bench(args):
@@ -259,11 +264,13 @@ def stability_bench(args):
             "wavenumbers": "A and B do not depend on the spanwise wavenumber in the reference (spanwise terms "
                            "commented out); one assembly serves a batch of 16",
             "element_kernel_ms": kern_ms_launch,
+            "element_kernels": "vef_stab_prep_kernel + vef_stab_kernel",
             "scatter_ms": rest_ms / max(launches, 1),
             "flops_per_element": stability_flops_per_element(),
         },
         "roofline": {
-            "kernel": "vef_stab_kernel (54 forward differences per element, A and B)",
+            "kernel": "vef_stab_prep_kernel + vef_stab_kernel (base state and test-node coefficients once per "
+                      "element, then 54 forward differences per element, A and B)",
             "bound": "mfma",
             "ceiling": "FP64 compute (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; no FMA "
                        "contraction, so 0.5 of peak is its ceiling)",

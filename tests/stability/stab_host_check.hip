@@ -6,6 +6,8 @@
 // perturbation, test-node terms, ordered sum over the Gauss points, quotient.
 #include "vef_stability_math.h"
 
+#include "flop_real.hpp"
+
 extern "C" int stab_host_element(const double* prm5, int model, const double* u54, double EU, const double* bf,
                                  const double* dFx, const double* dFy, const double* d2x, const double* d2y,
                                  const double* H2, const double* WET, double* A, double* B) {
@@ -34,5 +36,23 @@ extern "C" int stab_host_element(const double* prm5, int model, const double* u5
         B[p * 54 + n * 6 + c] = 0.0 + vefstab::fd_quotient(rb[n][c]);
       }
   }
+  return 0;
+}
+
+// Operations of node_coef for one (Gauss point, test node), counted with the
+// oracle's instrumented scalar (every +, -, *, / counts one; negation none):
+// out = add, mul, div, total.
+extern "C" int stab_count_node_coef(const double* prm5, const double* base10, double bifn, double fx, double fy,
+                                    unsigned long long* out) {
+  vefstab::Params prm{prm5[0], prm5[1], prm5[2], prm5[3], prm5[4], 1.0 - prm5[1], 0};
+  FlopReal v[10];
+  for (int i = 0; i < 10; i++) v[i] = FlopReal(base10[i]);
+  vefstab::NodeCoefT<FlopReal> c;
+  FlopCounter::reset();
+  vefstab::node_coef<FlopReal>(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], bifn, fx, fy, prm, c);
+  out[0] = FlopCounter::add;
+  out[1] = FlopCounter::mul;
+  out[2] = FlopCounter::div;
+  out[3] = FlopCounter::total();
   return 0;
 }

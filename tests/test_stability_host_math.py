@@ -17,6 +17,7 @@ from test_stability_oracle import PARAM_SETS, affine_xy, div_free_state
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "ve_flow_around_cylinder_fortran_amd", "csrc")
+ORACLE = os.path.join(os.path.dirname(HERE), "oracle")
 
 
 @pytest.fixture(scope="module")
@@ -26,11 +27,14 @@ def host_lib(tmp_path_factory):
         pytest.skip("hipcc not available")
     out = str(tmp_path_factory.mktemp("stab") / "libstab_host.so")
     subprocess.run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-ffp-contract=off",
-                    "-fno-fast-math", "-I", CSRC, "-o", out, os.path.join(HERE, "stability", "stab_host_check.hip")],
+                    "-fno-fast-math", "-I", CSRC, "-I", ORACLE, "-o", out,
+                    os.path.join(HERE, "stability", "stab_host_check.hip")],
                    check=True, capture_output=True)
     L = ctypes.CDLL(out)
     L.stab_host_element.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double] + \
         [ctypes.c_void_p] * 9
+    L.stab_count_node_coef.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_void_p]
     return L
 
 
@@ -86,3 +90,22 @@ def test_host_math_matches_oracle_bitwise(host_lib, oracle, pset):
         na = int(np.count_nonzero(canon(Ao) != canon(Ah)))
         nb = int(np.count_nonzero(canon(Bo) != canon(Bh)))
         assert na == 0 and nb == 0, (trial, na, nb, np.abs(Ao - Ah).max(), np.abs(Bo - Bh).max())
+
+
+def test_node_coefficient_operation_count(host_lib):
+    """The per-(Gauss point, test node) coefficients computed once per element
+    instead of once per disturbance: their operation count (the oracle's
+    counting rules) is the constant the stability roofline subtracts for the
+    53 disturbances that reuse them."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(os.path.dirname(HERE), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    prm5 = np.array([0.3, 0.59, 0.25, 1e-3, 1.0])
+    base10 = np.array([0.7, -0.2, 1.1, 0.05, 0.9, 0.6, 0.3, 0.02, 0.4, 1.2])
+    out = np.zeros(4, dtype=np.uint64)
+    host_lib.stab_count_node_coef(_p(prm5), _p(base10), 0.25, 1.5, -0.75, _p(out))
+    add, mul, div, total = map(int, out)
+    assert div == 11 and total == add + mul + div
+    assert total == bench.STAB_COEF_FLOPS

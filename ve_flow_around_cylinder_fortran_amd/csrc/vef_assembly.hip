@@ -1271,29 +1271,121 @@ vef_dirichlet_kernel(int64_t n_rows, const int64_t* __restrict__ rows, const dou
 // Linear stability assembly (Cylinder_Stability_2D): A and B of the
 // generalised eigenproblem by forward differences in the disturbance
 // (storeBulkStability, src/Stability/Equations_Stability_mod.F90:135-226).
-// Eight single-wavefront workgroups per element, ids 64m + 8j + x for element
-// 8m + x and group j (one XCD per element); lane (g, s) = Gauss point g of 9,
-// slot s of 7, perturbation p = 7j + s < 54 of unknown (p/6, p%6).  Nine lanes
-// first compute the Gauss-point geometry and base quantities into LDS; each
-// lane then forms its disturbance quantities and, per test node, the 6 A and
-// 6 B terms; an ordered reduction over the Gauss points gives
-// Res(eps e_p) and the quotient Res/eps (the zero disturbance has zero
+//
+// Everything that depends on the base state alone is computed once per
+// element by vef_stab_prep_kernel: the Gauss-point geometry, the base
+// quantities (base_point) and, per Gauss point and test node, the test-node
+// coefficients (node_coef).  The reference recomputes them for each of the 54
+// disturbances; they do not depend on the disturbance, so the values are the
+// same bits.  Per element the prep block holds, in doubles:
+//   [0, 288)      Base of Gauss points 0..8 (32 each)
+//   [288, 648)    per Gauss point: dFx[9], dFy[9], d2x[9], d2y[9], WET, 3 pad
+//   [648, 1944)   NodeCoef of (test node n, Gauss point g) at 648 + 16*(9n + g)
+// ----------------------------------------------------------------------------
+constexpr int kPrepBase = 0, kPrepGeo = 9 * 32, kPrepCoef = kPrepGeo + 9 * 40;
+constexpr int kPrepPerElement = kPrepCoef + 81 * 16;
+static_assert(sizeof(vefstab::Base) == 32 * sizeof(double), "Base is 32 doubles");
+static_assert(sizeof(vefstab::NodeCoef) == 16 * sizeof(double), "NodeCoef is 16 doubles");
+
+// Seven elements per wavefront, lane (element el, Gauss point g).
+__global__ void __launch_bounds__(64)
+vef_stab_prep_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn,
+                     const double* __restrict__ x, int64_t eu_index, int32_t n_el, vefstab::Params prm,
+                     double* __restrict__ prep) {
+  __shared__ double sU[7][54];
+  __shared__ double sX[7][18];
+  __shared__ Geo sG[63];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * 7;
+  for (int t = tid; t < 7 * 54; t += 64) {
+    const int el = t / 54, i = t % 54;
+    if (e0 + el < n_el) sU[el][i] = x[(int64_t)conn[(e0 + el) * 9 + i / 6] * 6 + i % 6];
+  }
+  for (int t = tid; t < 7 * 18; t += 64) {
+    const int el = t / 18, i = t % 18;
+    if (e0 + el < n_el) sX[el][i] = coords[(int64_t)conn[(e0 + el) * 9 + i / 2] * 2 + (i & 1)];
+  }
+  __syncthreads();
+  const int el = tid / 9, g = tid % 9;
+  if (tid >= 63 || e0 + el >= n_el) return;  // no barrier follows
+  const double EU = x[eu_index];
+  Geo& G = sG[tid];
+  geometry(sX[el], c_tab.dc[g], c_tab.de[g], c_tab.dcc[g], c_tab.dee[g], c_tab.dce[g], c_tab.W2[g], G);
+  vefstab::Base b;
+  vefstab::base_point(sU[el], c_tab.bf[g], G.dFx, G.dFy, G.H2, EU, prm, b);
+  double* dst = prep + (e0 + el) * kPrepPerElement;
+  const double* bs = reinterpret_cast<const double*>(&b);
+#pragma unroll
+  for (int i = 0; i < 32; i += 2)
+    *reinterpret_cast<double2*>(dst + kPrepBase + g * 32 + i) = make_double2(bs[i], bs[i + 1]);
+  double* geo = dst + kPrepGeo + g * 40;
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    geo[n] = G.dFx[n];
+    geo[9 + n] = G.dFy[n];
+    geo[18 + n] = G.d2x[n];
+    geo[27 + n] = G.d2y[n];
+  }
+  geo[36] = G.WET;
+  geo[37] = 0.0;
+  geo[38] = 0.0;
+  geo[39] = 0.0;
+  for (int n = 0; n < 9; n++) {
+    vefstab::NodeCoef c;
+    vefstab::node_coef(b, c_tab.bf[g][n], G.dFx[n], G.dFy[n], prm, c);
+    const double* cs = reinterpret_cast<const double*>(&c);
+    double* cd = dst + kPrepCoef + (n * 9 + g) * 16;
+#pragma unroll
+    for (int i = 0; i < 16; i += 2) *reinterpret_cast<double2*>(cd + i) = make_double2(cs[i], cs[i + 1]);
+  }
+}
+
+// (dRes - Res)/eps_jac = ((r*h)/(h*h)) with Res = 0: the two-FMA division by
+// the shared reciprocal of h*h (correctly rounded while |q0| lies in
+// [2^-901, 2^900) or r*h = +-0; h*h = 1e-16 is inside [2^-60, 2^40]), the IEEE
+// division for any other lane.
+__device__ __forceinline__ double stab_quotient(double r) {
+  constexpr double hh = vefstab::kEpsJac * vefstab::kEpsJac;
+  constexpr double y = 1.0 / hh;
+  const double a = r * vefstab::kEpsJac;
+  const double q0 = a * y;
+  const double m = fabs(q0);
+  if (!((m >= 0x1p-901 && m < 0x1p900) || a == 0.0)) return a / hh;
+  const double e = fma(hh, q0, -a);
+  return fma(-e, y, q0);
+}
+
+// The forward differences: eight single-wavefront workgroups per element, ids
+// 64m + 8j + x for element 8m + x and group j (one XCD per element); slot s of
+// 7 is perturbation p = 7j + s < 54 of unknown (p/6, p%6).  The base and
+// geometry part of the element's prep block is copied to LDS; lane (g, s)
+// forms the disturbance quantities of Gauss point g (Dist) into LDS; then
+// lane (n, s) forms the 6 A and 6 B terms of test node n at each Gauss point
+// from the coefficients, read from the prep block in global memory (seven
+// lanes read each record, and the element's eight workgroups share it through
+// the caches; staged in LDS instead, the nine records a load instruction
+// touches sit 128 bytes apart, in the same banks), and sums them over the
+// Gauss points in order (Res = Res + TERM*WET, ig = 1..9) in registers,
+// giving Res(eps e_p) and the quotient Res/eps (the zero disturbance has zero
 // residuals), stored like the base Jacobian: straight into the values of node
 // pairs with one contributing element, else into the element's bank rows for
 // vef_shared_kernel.  The stability pattern is the base one without the
 // arclength column (the last of each row) and row, so the codes of the base
 // scatter map apply with the stability row pointers.
-// ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
-vef_stab_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
-                int64_t eu_index, int32_t n_el, vefstab::Params prm, const int32_t* __restrict__ code,
-                const int64_t* __restrict__ rowptr_s, double* __restrict__ A, double* __restrict__ B,
-                double* __restrict__ cbank_a, double* __restrict__ cbank_b) {
-  __shared__ double sU[54];
-  __shared__ double sX[18];
-  __shared__ Geo sG[9];
-  __shared__ vefstab::Base sB[9];
-  __shared__ double sRed[9][7][12];
+vef_stab_kernel(const double* __restrict__ prep, const int32_t* __restrict__ conn, int32_t n_el, vefstab::Params prm,
+                const int32_t* __restrict__ code, const int64_t* __restrict__ rowptr_s, double* __restrict__ A,
+                double* __restrict__ B, double* __restrict__ cbank_a, double* __restrict__ cbank_b) {
+  static_assert(sizeof(vefstab::Dist) == 24 * sizeof(double), "Dist is 24 doubles");
+  // Base and geometry while the disturbances are formed, then the disturbances
+  // of (g, s) at stride kDist (26 doubles: the seven records one load
+  // instruction reads start in distinct banks).
+  constexpr int kDist = 26;
+  __shared__ __align__(16) union {
+    double pre[kPrepCoef];  // Base[9] then geometry[9][40]
+    double dist[63 * kDist];
+  } sW;
+  __shared__ double sT[9][3];  // tlsme, WiN, WET per Gauss point
   __shared__ int32_t sCode[81];
   __shared__ int64_t sRb[9];
   __shared__ int32_t sRl[9];
@@ -1302,9 +1394,9 @@ vef_stab_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   const int64_t e = (blk >> 6) * 8 + (blk & 7);
   const int kg = (int)((blk >> 3) & 7);
   if (e >= n_el) return;  // uniform over the workgroup
-  const double EU = x[eu_index];
-  if (tid < 54) sU[tid] = x[(int64_t)conn[e * 9 + tid / 6] * 6 + tid % 6];
-  if (tid < 18) sX[tid] = coords[(int64_t)conn[e * 9 + tid / 2] * 2 + (tid & 1)];
+  const double2* src = reinterpret_cast<const double2*>(prep + e * kPrepPerElement);
+  for (int t = tid; t < kPrepCoef / 2; t += 64) reinterpret_cast<double2*>(sW.pre)[t] = src[t];
+  const vefstab::NodeCoef* gK = reinterpret_cast<const vefstab::NodeCoef*>(prep + e * kPrepPerElement + kPrepCoef);
   for (int t = tid; t < 81; t += 64) sCode[t] = code[e * 81 + t];
   if (tid < 9) {
     const int64_t r6 = (int64_t)conn[e * 9 + tid] * 6;
@@ -1313,45 +1405,67 @@ vef_stab_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     sRl[tid] = (int32_t)(rowptr_s[r6 + 1] - rb);
   }
   __syncthreads();
-  if (tid < 9) {
-    Geo& G = sG[tid];
-    geometry(sX, c_tab.dc[tid], c_tab.de[tid], c_tab.dcc[tid], c_tab.dee[tid], c_tab.dce[tid], c_tab.W2[tid], G);
-    vefstab::base_point(sU, c_tab.bf[tid], G.dFx, G.dFy, G.H2, EU, prm, sB[tid]);
-  }
-  __syncthreads();
-  const int g = tid / 7, s = tid % 7;
+  const int hi = tid / 7, s = tid % 7;  // (g, s) in the first phase, (n, s) in the second
   const int p = kg * 7 + s;
   const bool valid = tid < 63 && p < 54;
+  // The first two coefficient records of test node hi, loaded before the
+  // disturbance is formed; each later one is loaded a Gauss point ahead.
+  vefstab::NodeCoef kc, kn;
+  if (valid) {
+    kc = gK[hi * 9];
+    kn = gK[hi * 9 + 1];
+  }
   vefstab::Dist D;
-  if (valid)
-    vefstab::disturbance(sB[g], p / 6, p % 6, c_tab.bf[g], sG[g].dFx, sG[g].dFy, sG[g].d2x, sG[g].d2y, prm, D);
-  for (int n = 0; n < 9; n++) {
-    __syncthreads();  // the previous sums have been read
-    if (valid) {
-      double a[6], bb[6];
-      vefstab::node_terms(sB[g], D, c_tab.bf[g][n], sG[g].dFx[n], sG[g].dFy[n], sG[g].WET, prm, a, bb);
+  if (valid) {
+    const vefstab::Base& b = reinterpret_cast<const vefstab::Base*>(sW.pre + kPrepBase)[hi];
+    const double* geo = sW.pre + kPrepGeo + hi * 40;
+    vefstab::disturbance(b, p / 6, p % 6, c_tab.bf[hi], geo, geo + 9, geo + 18, geo + 27, prm, D);
+  }
+  if (tid < 9) {
+    sT[tid][0] = reinterpret_cast<const vefstab::Base*>(sW.pre + kPrepBase)[tid].tlsme;
+    sT[tid][1] = reinterpret_cast<const vefstab::Base*>(sW.pre + kPrepBase)[tid].WiN;
+    sT[tid][2] = sW.pre[kPrepGeo + tid * 40 + 36];
+  }
+  __syncthreads();  // the base and geometry have been read
+  if (valid) *reinterpret_cast<vefstab::Dist*>(sW.dist + (hi * 7 + s) * kDist) = D;
+  __syncthreads();
+  if (!valid) return;  // no barrier follows
+  const int n = hi;
+  double ra[6], rb[6];
+  {
+    double a[6], bb[6];
+    auto dist = [&](int g) -> const vefstab::Dist& {
+      return *reinterpret_cast<const vefstab::Dist*>(sW.dist + (g * 7 + s) * kDist);
+    };
+    vefstab::node_terms_c(kc, dist(0), sT[0][0], sT[0][1], sT[0][2], prm, ra, rb);
+#pragma unroll
+    for (int g = 1; g < 9; g++) {
+      kc = kn;
+      if (g < 8) kn = gK[n * 9 + g + 1];
+      vefstab::node_terms_c(kc, dist(g), sT[g][0], sT[g][1], sT[g][2], prm, a, bb);
 #pragma unroll
       for (int c = 0; c < 6; c++) {
-        sRed[g][s][c] = a[c];
-        sRed[g][s][6 + c] = bb[c];
+        ra[c] = ra[c] + a[c];
+        rb[c] = rb[c] + bb[c];
       }
     }
-    __syncthreads();
-    // Res = Res + TERM*WET over the Gauss points in order; c < 6 rows of A, else of B.
-    for (int t = tid; t < 84; t += 64) {
-      const int c = t / 7, ss = t % 7, pp = kg * 7 + ss;
-      if (pp < 54) {
-        double acc = sRed[0][ss][c];
+  }
+  const int in = p / 6, iq = p % 6;
+  const int32_t cd = sCode[n * 9 + in];
+  if ((cd & 1) == 0) {
+    const int64_t o = sRb[n] + (int64_t)(cd >> 1) * 6 + iq;
+    const int64_t rl = sRl[n];
 #pragma unroll
-        for (int gg = 1; gg < 9; gg++) acc = acc + sRed[gg][ss][c];
-        const double q = vefstab::fd_quotient(acc);
-        const int cc = c % 6, in = pp / 6, iq = pp % 6;
-        const int32_t cd = sCode[n * 9 + in];
-        if ((cd & 1) == 0)
-          (c < 6 ? A : B)[sRb[n] + (int64_t)cc * sRl[n] + (int64_t)(cd >> 1) * 6 + iq] = 0.0 + q;
-        else
-          (c < 6 ? cbank_a : cbank_b)[(int64_t)(cd >> 1) * 36 + cc * 6 + iq] = q;
-      }
+    for (int c = 0; c < 6; c++) {
+      A[o + c * rl] = 0.0 + stab_quotient(ra[c]);
+      B[o + c * rl] = 0.0 + stab_quotient(rb[c]);
+    }
+  } else {
+    const int64_t o = (int64_t)(cd >> 1) * 36 + iq;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      cbank_a[o + c * 6] = stab_quotient(ra[c]);
+      cbank_b[o + c * 6] = stab_quotient(rb[c]);
     }
   }
 }
@@ -1494,6 +1608,7 @@ struct vef_handle {
   int32_t* d_slen_s = nullptr;
   double* d_cbank_a = nullptr;
   double* d_cbank_b = nullptr;
+  double* d_stab_prep = nullptr;  // per element: base, geometry and test-node coefficients
   int64_t* d_sbc_rows = nullptr;    // stability Dirichlet rows, distinct
   int64_t n_sbc = 0;
   int32_t* d_drag_edges = nullptr;  // boundary edges of the drag integral, [n][2]
@@ -1625,7 +1740,7 @@ void vef_destroy(vef_handle* h) {
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
                   h->d_drag_edges, h->d_drag_terms, h->d_bc_rows, h->d_bc_values,
                   h->d_tmp_b,  h->d_tmp_v, h->d_rowptr_s, h->d_sbase_s, h->d_slen_s,
-                  h->d_cbank_a, h->d_cbank_b, h->d_sbc_rows};
+                  h->d_cbank_a, h->d_cbank_b, h->d_sbc_rows, h->d_stab_prep};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -2143,6 +2258,8 @@ static int ensure_stability(vef_handle* h) {
   if (!rc) rc = dev_upload(h, &h->d_slen_s, h->h_slen_s.data(), h->h_slen_s.size());
   if (!rc) rc = dev_upload(h, &h->d_cbank_a, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
   if (!rc) rc = dev_upload(h, &h->d_cbank_b, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
+  if (!rc)
+    rc = dev_upload(h, &h->d_stab_prep, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_el, 1) * kPrepPerElement);
   if (rc) return rc;
   h->stab_ready = true;
   return VEF_OK;
@@ -2164,9 +2281,11 @@ int vef_stability_assemble(vef_handle* h, const double* x, double* A, double* B,
     for (int i = 0; i < 3; i++) VEF_HIP(h, hipEventCreate(&ev[i]));
     VEF_HIP(h, hipEventRecord(ev[0], st));
   }
-  hipLaunchKernelGGL(vef_stab_kernel, dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_coords,
-                     h->d_conn, x, h->N - 1, h->n_el, prm, h->d_code, h->d_rowptr_s, A, B, h->d_cbank_a,
-                     h->d_cbank_b);
+  hipLaunchKernelGGL(vef_stab_prep_kernel, dim3((unsigned)((h->n_el + 6) / 7)), dim3(64), 0, st, h->d_coords,
+                     h->d_conn, x, h->N - 1, h->n_el, prm, h->d_stab_prep);
+  VEF_HIP(h, hipGetLastError());
+  hipLaunchKernelGGL(vef_stab_kernel, dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_stab_prep,
+                     h->d_conn, h->n_el, prm, h->d_code, h->d_rowptr_s, A, B, h->d_cbank_a, h->d_cbank_b);
   VEF_HIP(h, hipGetLastError());
   if (h->profiling) VEF_HIP(h, hipEventRecord(ev[1], st));
   if (h->n_shared > 0) {

@@ -307,64 +307,114 @@ __host__ __device__ inline void disturbance(const Base& b, int inod, int ieq, co
   o.dS11y = dS11y;
 }
 
-// Test-node rows of one Gauss point (momentumStabilitySQRT, continuityStability,
-// stressStabilitySQRT), times WET: a[6] for A, bb[6] for B (negated, as
-// equationsResidualsBulkStability does).
-__host__ __device__ inline void node_terms(const Base& b, const Dist& d, double bifn, double fx, double fy,
-                                           double WET, const Params& prm, double* a, double* bb) {
-  const double Udf = b.U0 * fx + b.U1 * fy;
-  const double lsme_m = (b.tlsme * prm.ReN) * Udf;  // LSMEMomentum(dFdXi, GsPt_b)
-  const double bS10 = b.S01;
-  const double rW = b.S00 + b.S11, WW = b.WiN * b.WiN;
-  double TA0, TA1, TB0, TB1;
+// Test-node coefficients that depend only on the base state, the test node
+// and the parameters, not on the disturbance: the LSME momentum factor, the
+// LSIC factors, the LSCE momentum operators of GUT_lsce rows 1 and 2 and the
+// LSCE stress factor (src/Stability/Residuals_Stability_mod.F90:209-313).
+// Every one of them is a leading subexpression of the reference's term, so
+// computing it once per (Gauss point, test node) and reusing it for all 54
+// disturbances gives the same bits.  R is double, or a counting type for the
+// operation count.
+template <class R>
+struct NodeCoefT {
+  R lsme_m;          // (tlsme*ReN)*(U.grad phi)
+  R licx, licy;      // tlsic*fx, tlsic*fy
+  R l1[4], l2[4];    // LSCEMomentumSQRT rows 1 and 2: l00, l01, l10, l11
+  R lsce_s;          // tlsce*(WiN*(U.grad phi) + f_T*phi/2)
+  R fx, fy, bifn;    // the test function and its gradient
+  R pad;
+};
+using NodeCoef = NodeCoefT<double>;
+
+template <class R, class B>
+__host__ __device__ inline R cdiv_t(const R& a, const B& c) { return (a * c) / (c * c); }
+
+template <class R>
+__host__ __device__ inline void node_coef(const R& U0, const R& U1, const R& S00, const R& S01, const R& S11,
+                                          const R& WiN, const R& tlsme, const R& tlsic, const R& tlsce,
+                                          const R& f_T, double bifn, double fx, double fy, const Params& prm,
+                                          NodeCoefT<R>& c) {
+  const R Udf = U0 * fx + U1 * fy;
+  c.lsme_m = (tlsme * prm.ReN) * Udf;  // LSMEMomentum(dFdXi, GsPt_b)
+  const R bS10 = S01;
+  const R rW = S00 + S11, WW = WiN * WiN;
   {
     // LSCEMomentumSQRT(1, dFdXi, GsPt_b): GUT_lsce row 1 = grad(phi)
-    const double w = (b.S01 * fx + b.S11 * fy) / rW;
-    const double m00 = b.S00 * fx + b.S01 * fy, m10 = bS10 * fx + b.S11 * fy;
-    const double u00 = -m00 - w * bS10, u01 = -(w * b.S11);
-    const double u10 = -m10 - (-w) * b.S00, u11 = -((-w) * b.S01);
-    const double l00 = ((b.tlsce * u00) * prm.omb * b.WiN) / WW, l01 = ((b.tlsce * u01) * prm.omb * b.WiN) / WW;
-    const double l10 = ((b.tlsce * u10) * prm.omb * b.WiN) / WW, l11 = ((b.tlsce * u11) * prm.omb * b.WiN) / WW;
-    const double lsic = (b.tlsic * fx) * d.Cont;
-    const double lce = l00 * d.CE00 + l01 * d.CE10 + l10 * d.CE01 + l11 * d.CE11;
-    TA0 = (((d.RC0 * bifn + (d.P00 * fx + d.P01 * fy)) + lsme_m * d.Mom0) + lsic) + lce;
-    const double lct = l00 * d.CT00 + l01 * d.CT01 + l10 * d.CT01 + l11 * d.CT11;
-    TB0 = (d.Mt0 * bifn + lsme_m * d.Mt0) + lct;
+    const R w = (S01 * fx + S11 * fy) / rW;
+    const R m00 = S00 * fx + S01 * fy, m10 = bS10 * fx + S11 * fy;
+    const R u00 = -m00 - w * bS10, u01 = -(w * S11);
+    const R u10 = -m10 - (-w) * S00, u11 = -((-w) * S01);
+    c.l1[0] = ((tlsce * u00) * prm.omb * WiN) / WW;
+    c.l1[1] = ((tlsce * u01) * prm.omb * WiN) / WW;
+    c.l1[2] = ((tlsce * u10) * prm.omb * WiN) / WW;
+    c.l1[3] = ((tlsce * u11) * prm.omb * WiN) / WW;
+    c.licx = tlsic * fx;
   }
   {
     // LSCEMomentumSQRT(2, dFdXi, GsPt_b): GUT_lsce row 2 = grad(phi)
-    const double w = (-(b.S00 * fx) + b.S01 * (-fy)) / rW;
-    const double m01 = b.S00 * fx + b.S01 * fy, m11 = bS10 * fx + b.S11 * fy;
-    const double u00 = -(w * bS10), u01 = -m01 - w * b.S11;
-    const double u10 = -((-w) * b.S00), u11 = -m11 - (-w) * b.S01;
-    const double l00 = ((b.tlsce * u00) * prm.omb * b.WiN) / WW, l01 = ((b.tlsce * u01) * prm.omb * b.WiN) / WW;
-    const double l10 = ((b.tlsce * u10) * prm.omb * b.WiN) / WW, l11 = ((b.tlsce * u11) * prm.omb * b.WiN) / WW;
-    const double lsic = (b.tlsic * fy) * d.Cont;
-    const double lce = l00 * d.CE00 + l01 * d.CE10 + l10 * d.CE01 + l11 * d.CE11;
-    TA1 = (((d.RC1 * bifn + (d.P10 * fx + d.P11 * fy)) + lsme_m * d.Mom1) + lsic) + lce;
-    const double lct = l00 * d.CT00 + l01 * d.CT01 + l10 * d.CT01 + l11 * d.CT11;
-    TB1 = (d.Mt1 * bifn + lsme_m * d.Mt1) + lct;
+    const R w = (-(S00 * fx) + S01 * (-fy)) / rW;
+    const R m01 = S00 * fx + S01 * fy, m11 = bS10 * fx + S11 * fy;
+    const R u00 = -(w * bS10), u01 = -m01 - w * S11;
+    const R u10 = -((-w) * S00), u11 = -m11 - (-w) * S01;
+    c.l2[0] = ((tlsce * u00) * prm.omb * WiN) / WW;
+    c.l2[1] = ((tlsce * u01) * prm.omb * WiN) / WW;
+    c.l2[2] = ((tlsce * u10) * prm.omb * WiN) / WW;
+    c.l2[3] = ((tlsce * u11) * prm.omb * WiN) / WW;
+    c.licy = tlsic * fy;
   }
-  const double TA2 = d.Cont * bifn + b.tlsme * (fx * d.Mom0 + fy * d.Mom1);
-  const double TB2 = b.tlsme * (fx * d.Mt0 + fy * d.Mt1);
   // LSCEStressSQRT(bifn, dFdXi, GsPt_b)
-  const double lsce_s = b.tlsce * (b.WiN * Udf + cdiv(b.f_T * bifn, 2.0));
+  c.lsce_s = tlsce * (WiN * Udf + cdiv_t(f_T * bifn, 2.0));
+  c.fx = fx;
+  c.fy = fy;
+  c.bifn = bifn;
+  c.pad = 0.0;
+}
+
+__host__ __device__ inline void node_coef(const Base& b, double bifn, double fx, double fy, const Params& prm,
+                                          NodeCoef& c) {
+  node_coef<double>(b.U0, b.U1, b.S00, b.S01, b.S11, b.WiN, b.tlsme, b.tlsic, b.tlsce, b.f_T, bifn, fx, fy, prm, c);
+}
+
+// Test-node rows of one Gauss point (momentumStabilitySQRT, continuityStability,
+// stressStabilitySQRT), times WET, from the coefficients above and the
+// disturbance: a[6] for A, bb[6] for B (negated, as
+// equationsResidualsBulkStability does).
+__host__ __device__ inline void node_terms_c(const NodeCoef& k, const Dist& d, double tlsme, double WiN, double WET,
+                                             const Params& prm, double* a, double* bb) {
+  const double bifn = k.bifn, fx = k.fx, fy = k.fy;
+  double TA0, TA1, TB0, TB1;
+  {
+    const double lsic = k.licx * d.Cont;
+    const double lce = k.l1[0] * d.CE00 + k.l1[1] * d.CE10 + k.l1[2] * d.CE01 + k.l1[3] * d.CE11;
+    TA0 = (((d.RC0 * bifn + (d.P00 * fx + d.P01 * fy)) + k.lsme_m * d.Mom0) + lsic) + lce;
+    const double lct = k.l1[0] * d.CT00 + k.l1[1] * d.CT01 + k.l1[2] * d.CT01 + k.l1[3] * d.CT11;
+    TB0 = (d.Mt0 * bifn + k.lsme_m * d.Mt0) + lct;
+  }
+  {
+    const double lsic = k.licy * d.Cont;
+    const double lce = k.l2[0] * d.CE00 + k.l2[1] * d.CE10 + k.l2[2] * d.CE01 + k.l2[3] * d.CE11;
+    TA1 = (((d.RC1 * bifn + (d.P10 * fx + d.P11 * fy)) + k.lsme_m * d.Mom1) + lsic) + lce;
+    const double lct = k.l2[0] * d.CT00 + k.l2[1] * d.CT01 + k.l2[2] * d.CT01 + k.l2[3] * d.CT11;
+    TB1 = (d.Mt1 * bifn + k.lsme_m * d.Mt1) + lct;
+  }
+  const double TA2 = d.Cont * bifn + tlsme * (fx * d.Mom0 + fy * d.Mom1);
+  const double TB2 = tlsme * (fx * d.Mt0 + fy * d.Mt1);
   const double d00 = d.dS00x * fx + d.dS00y * fy;
   const double d01 = d.dS01x * fx + d.dS01y * fy;
   const double d11 = d.dS11x * fx + d.dS11y * fy;
   // LSMEStressSQRT with Momentum_d and with Momentum_time_d
-  const double l3 = (b.tlsme * (-(fx * d.Mom0))) * b.WiN;
-  const double l4 = (b.tlsme * (-(fy * d.Mom0) - fx * d.Mom1)) * b.WiN;
-  const double l5 = (b.tlsme * (-(fy * d.Mom1))) * b.WiN;
-  const double k3 = (b.tlsme * (-(fx * d.Mt0))) * b.WiN;
-  const double k4 = (b.tlsme * (-(fy * d.Mt0) - fx * d.Mt1)) * b.WiN;
-  const double k5 = (b.tlsme * (-(fy * d.Mt1))) * b.WiN;
-  const double TA3 = ((d.CE00 * bifn + prm.DfN * d00) + l3) + lsce_s * d.CE00;
-  const double TA4 = ((d.CE01 * bifn + prm.DfN * d01) + l4) + lsce_s * d.CE01;
-  const double TA5 = ((d.CE11 * bifn + prm.DfN * d11) + l5) + lsce_s * d.CE11;
-  const double TB3 = (d.CT00 * bifn + k3) + lsce_s * d.CT00;
-  const double TB4 = (d.CT01 * bifn + k4) + lsce_s * d.CT01;
-  const double TB5 = (d.CT11 * bifn + k5) + lsce_s * d.CT11;
+  const double l3 = (tlsme * (-(fx * d.Mom0))) * WiN;
+  const double l4 = (tlsme * (-(fy * d.Mom0) - fx * d.Mom1)) * WiN;
+  const double l5 = (tlsme * (-(fy * d.Mom1))) * WiN;
+  const double k3 = (tlsme * (-(fx * d.Mt0))) * WiN;
+  const double k4 = (tlsme * (-(fy * d.Mt0) - fx * d.Mt1)) * WiN;
+  const double k5 = (tlsme * (-(fy * d.Mt1))) * WiN;
+  const double TA3 = ((d.CE00 * bifn + prm.DfN * d00) + l3) + k.lsce_s * d.CE00;
+  const double TA4 = ((d.CE01 * bifn + prm.DfN * d01) + l4) + k.lsce_s * d.CE01;
+  const double TA5 = ((d.CE11 * bifn + prm.DfN * d11) + l5) + k.lsce_s * d.CE11;
+  const double TB3 = (d.CT00 * bifn + k3) + k.lsce_s * d.CT00;
+  const double TB4 = (d.CT01 * bifn + k4) + k.lsce_s * d.CT01;
+  const double TB5 = (d.CT11 * bifn + k5) + k.lsce_s * d.CT11;
   a[0] = TA0 * WET;
   a[1] = TA1 * WET;
   a[2] = TA2 * WET;
@@ -377,6 +427,14 @@ __host__ __device__ inline void node_terms(const Base& b, const Dist& d, double 
   bb[3] = (-TB3) * WET;
   bb[4] = (-TB4) * WET;
   bb[5] = (-TB5) * WET;
+}
+
+// Both parts for one (Gauss point, test node, disturbance).
+__host__ __device__ inline void node_terms(const Base& b, const Dist& d, double bifn, double fx, double fy,
+                                           double WET, const Params& prm, double* a, double* bb) {
+  NodeCoef k;
+  node_coef(b, bifn, fx, fy, prm, k);
+  node_terms_c(k, d, b.tlsme, b.WiN, WET, prm, a, bb);
 }
 
 // (dRes - Res)/eps_jac with Res = 0 (the zero disturbance gives zero residuals).
