@@ -44,6 +44,7 @@ def lib(flops: bool = False):
         if flops:
             L.vo_count_flops.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, vp]
             L.vo_count_flops_stability.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, vp, vp]
+            L.vo_count_flops_geometry.argtypes = [ctypes.POINTER(OracleParams), vp, vp]
         else:
             L.vo_tables.argtypes = [vp]
             L.vo_basis2d.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -148,6 +149,15 @@ def count_flops(prm, xy, TL, EU, ied=0):
     xy = np.ascontiguousarray(xy, dtype=np.float64)
     TL = np.ascontiguousarray(TL, dtype=np.float64)
     lib(flops=True).vo_count_flops(ctypes.byref(prm), _p(xy), _p(TL), float(EU), ied, _p(out))
+    return dict(add=int(out[0]), mul=int(out[1]), div=int(out[2]), sqrt=int(out[3]), total=int(out[4]))
+
+
+def count_flops_geometry(prm, xy):
+    """Operations of the geometry-only part of one residualBulk evaluation (all
+    9 Gauss points: basis2D, Helem**2, BvN/H2, DfN/H2)."""
+    out = np.zeros(5, dtype=np.uint64)
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    lib(flops=True).vo_count_flops_geometry(ctypes.byref(prm), _p(xy), _p(out))
     return dict(add=int(out[0]), mul=int(out[1]), div=int(out[2]), sqrt=int(out[3]), total=int(out[4]))
 
 

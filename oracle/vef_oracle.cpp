@@ -500,17 +500,24 @@ static void strongFormMomentum(const Phys& ph, GsPt& G) {
 // ----------------------------------------------------------------------------
 // a7: setStabilizationTerms, Element_Calculations_mod.F90:701-773
 // ----------------------------------------------------------------------------
-static void setStabilizationTerms(const Phys& ph, const Elem& E, GsPt& G) {
+// Helem**2 of setStabilizationTerms (Element_Calculations_mod.F90:718-727):
+// geometry only.
+static vreal elementH2(const Elem& E) {
   vreal Helem = Z();
-  vreal tsugn = Z();
-  for (int n = 0; n < 9; n++) {
+  for (int n = 0; n < 9; n++)
     Helem = Helem + E.dFdXi[0][n] * E.dFdXi[0][n] + E.dFdXi[1][n] * E.dFdXi[1][n] +
             E.dFdXi[2][n] * E.dFdXi[2][n];
+  Helem = vreal(1.0) / vsqrt(Helem);
+  return Helem * Helem;
+}
+
+static void setStabilizationTerms(const Phys& ph, const Elem& E, GsPt& G) {
+  vreal tsugn = Z();
+  for (int n = 0; n < 9; n++) {
     vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
     tsugn = tsugn + fabsr(dot3(G.U_f, dF));
   }
-  Helem = vreal(1.0) / vsqrt(Helem);
-  vreal H2 = Helem * Helem;
+  vreal H2 = elementH2(E);
 
   vreal ddT = ddot33(G.T_ve, G.T_ve);
   vreal ddG = ddot33(G.G_dot_tr, G.G_dot_tr);
@@ -1787,6 +1794,39 @@ int vo_count_flops(const vo_params* p, const double* xy, const double* TL, doubl
   FlopCounter::reset();
   if (ied == 0) residualBulk(ph, E, R);
   else residualEdge(ph, E, ied, R);
+  out[0] = FlopCounter::add;
+  out[1] = FlopCounter::mul;
+  out[2] = FlopCounter::div;
+  out[3] = FlopCounter::sqrt;
+  out[4] = FlopCounter::total();
+  return 0;
+}
+
+// Operation counts of the geometry-only part of one residualBulk evaluation,
+// all 9 Gauss points: basis2D, Helem**2 and the two parameter-over-Helem**2
+// quotients of setStabilizationTerms (BvN/H2, DfN/H2).  out = add, mul, div,
+// sqrt, total.
+int vo_count_flops_geometry(const vo_params* p, const double* xy, uint64_t* out) {
+  Phys ph = make_phys(p);
+  Elem E;
+  for (int n = 0; n < 9; n++) {
+    for (int q = 0; q < 6; q++) E.TL[n][q] = vreal(0.0);
+    E.X[n][0] = xy[n * 2];
+    E.X[n][1] = xy[n * 2 + 1];
+    E.X[n][2] = 0.0;
+  }
+  const Tables& T = tables();
+  FlopCounter::reset();
+  for (int g = 0; g < 9; g++) {
+    setGaussPoint(E, T, g);
+    GsPt b;
+    basis2D(E, b);
+    vreal H2 = elementH2(E);
+    vreal r3 = ph.BvN / H2;
+    vreal t7 = ph.DfN / H2;
+    (void)r3;
+    (void)t7;
+  }
   out[0] = FlopCounter::add;
   out[1] = FlopCounter::mul;
   out[2] = FlopCounter::div;

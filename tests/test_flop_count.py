@@ -61,3 +61,22 @@ def test_stability_flop_constants_match_oracle_count(oracle):
         full, base = oracle.count_flops_stability(prm, xy, random_state(rng, xy), 0.6, TLd)
         seen.add((full["total"], base["total"]))
     assert seen == {(mod.STAB_FLOPS_PER_EVAL, mod.STAB_FLOPS_BASE)}
+
+
+def test_geometry_flop_constant_matches_oracle_count(oracle):
+    """The geometry-only part of an evaluation (basis2D, Helem**2, BvN/H2,
+    DfN/H2 at the 9 Gauss points) that the Jacobian pass takes from the
+    residual pass: bench.py's constant, the same for any element shape, and
+    the derived kernel and step counts."""
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    rng = np.random.default_rng(4)
+    prm = oracle.params(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1e-3, WiN0=1.0, model="L-PTT")
+    counts = {oracle.count_flops_geometry(prm, curved_xy(rng))["total"] for _ in range(3)}
+    assert counts == {mod.FLOPS_GEOMETRY_PER_EVAL}
+    g = oracle.count_flops_geometry(prm, curved_xy(rng))
+    assert g["sqrt"] == 9 and g["div"] == 9 * 7  # 1/sqrt(Helem), 4 in basis2D, BvN/H2, DfN/H2 per point
+    E, G = mod.FLOPS_PER_RESIDUAL_EVAL, mod.FLOPS_GEOMETRY_PER_EVAL
+    assert mod.JACOBIAN_KERNEL_FLOPS == 55 * (E - G) + 54 * 55 * 3
+    assert mod.STEP_FLOPS == G + 56 * (E - G) + 54 * 55 * 3 + 2916 + 108

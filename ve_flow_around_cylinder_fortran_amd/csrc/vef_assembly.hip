@@ -627,7 +627,11 @@ struct FdOut {
   double* values;
   double* cbank;
   double* acbank;
+  double* geo;           // [n_el][9] Geo records: written by the residual pass when a
+                         // Jacobian pass follows (else NULL), read by the Jacobian pass
 };
+constexpr int kGeoWords = (int)(sizeof(Geo) / sizeof(double));
+static_assert(sizeof(Geo) % sizeof(double) == 0, "Geo is a whole number of 8-byte words");
 
 template <int EPB, int KPB, int NEVAL, int MODEL, bool DCS, bool INERTIA>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
@@ -704,12 +708,31 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     }
   }
   __syncthreads();
-  for (int t = tid; t < EPB * 9; t += 64) {
-    const int el = t / 9, g = t % 9;
-    Geo& G = sG[el][g];
-    geometry(sX[el], c_tab.dc[g], c_tab.de[g], c_tab.dcc[g], c_tab.dee[g], c_tab.dce[g], c_tab.W2[g], G);
-    G.r3 = prm.BvN / G.H2;
-    G.t7 = prm.DfN / G.H2;
+  // Gauss-point geometry (basis2D, Helem**2 and the two quotients by it).  It
+  // does not depend on the state, so the residual pass computes it once per
+  // element, 63 lanes at a time, and hands it to the Jacobian pass through
+  // fo.geo (the same bits: same code, same coordinates).
+  if constexpr (FD) {
+    if (e0 < n_el) {
+      const double* src = fo.geo + e0 * 9 * kGeoWords;
+      double* dst = reinterpret_cast<double*>(&sG[0][0]);
+      for (int t = tid; t < 9 * kGeoWords; t += 64) dst[t] = src[t];
+    }
+  } else {
+    for (int t = tid; t < EPB * 9; t += 64) {
+      const int el = t / 9, g = t % 9;
+      Geo& G = sG[el][g];
+      geometry(sX[el], c_tab.dc[g], c_tab.de[g], c_tab.dcc[g], c_tab.dee[g], c_tab.dce[g], c_tab.W2[g], G);
+      G.r3 = prm.BvN / G.H2;
+      G.t7 = prm.DfN / G.H2;
+    }
+    if (fo.geo) {
+      __syncthreads();
+      const int64_t nv = (e0 + EPB <= n_el ? EPB : n_el - e0) * 9 * kGeoWords;
+      const double* src = reinterpret_cast<const double*>(&sG[0][0]);
+      double* dst = fo.geo + e0 * 9 * kGeoWords;
+      for (int t = tid; t < nv; t += 64) dst[t] = src[t];
+    }
   }
   __syncthreads();
 
@@ -832,10 +855,14 @@ vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ 
     sRb[tid] = rb;
     sRl[tid] = (int32_t)(fo.rowptr[r6 + 1] - rb);
   }
+  {
+    const double* src = fo.geo + e * 9 * kGeoWords;  // written by the residual pass
+    double* dst = reinterpret_cast<double*>(&sG[0]);
+    for (int t = tid; t < 9 * kGeoWords; t += 256) dst[t] = src[t];
+  }
   __syncthreads();
   if (tid < 9) {
-    Geo& G = sG[tid];
-    geometry(sX, c_tab.dc[tid], c_tab.de[tid], c_tab.dcc[tid], c_tab.dee[tid], c_tab.dce[tid], c_tab.W2[tid], G);
+    const Geo& G = sG[tid];
     for (int n = 0; n < 9; n++) {
       sBas[tid][0][n] = (float)c_tab.bf[tid][n];
       sBas[tid][1][n] = (float)G.dFx[n];
@@ -1587,6 +1614,7 @@ struct vef_handle {
   int32_t* d_elen = nullptr;
   int64_t n_epairs = 0;
   double* d_r0bank = nullptr;    // [n_el][54] base residual rows
+  double* d_geob = nullptr;      // [n_el][9] Geo records of the residual pass, for the Jacobian pass
   double* d_acbank = nullptr;    // [n_el][54] dR/dEU quotients
   double* d_cbank = nullptr;     // [n_cidx][36] quotients of shared entries
   int32_t* d_cstart = nullptr;
@@ -1685,8 +1713,8 @@ int upload_tables(vef_handle* h) {
 
 namespace {
 template <int MODEL, bool DCS, bool INERTIA>
-void launch_bulk_t(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
-  FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank};
+void launch_bulk_t(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st, double* geo) {
+  FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank, geo};
   if (jac)  // 8 workgroups per element, elements padded to a multiple of 8 (XCD-aware ids)
     hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>),
                        dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_coords, h->d_conn, x, eu,
@@ -1697,20 +1725,23 @@ void launch_bulk_t(vef_handle* h, int jac, double* values, const double* x, int6
 }
 
 template <int MODEL, bool DCS>
-void launch_bulk_i(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
-  if (h->dprm.ReN != 0.0) launch_bulk_t<MODEL, DCS, true>(h, jac, values, x, eu, st);
-  else launch_bulk_t<MODEL, DCS, false>(h, jac, values, x, eu, st);
+void launch_bulk_i(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st, double* geo) {
+  if (h->dprm.ReN != 0.0) launch_bulk_t<MODEL, DCS, true>(h, jac, values, x, eu, st, geo);
+  else launch_bulk_t<MODEL, DCS, false>(h, jac, values, x, eu, st, geo);
 }
 
 template <int MODEL>
-void launch_bulk_d(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
-  if (h->dprm.dcs_on) launch_bulk_i<MODEL, true>(h, jac, values, x, eu, st);
-  else launch_bulk_i<MODEL, false>(h, jac, values, x, eu, st);
+void launch_bulk_d(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st, double* geo) {
+  if (h->dprm.dcs_on) launch_bulk_i<MODEL, true>(h, jac, values, x, eu, st, geo);
+  else launch_bulk_i<MODEL, false>(h, jac, values, x, eu, st, geo);
 }
 
-void launch_bulk(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st) {
-  if (h->dprm.model == 0) launch_bulk_d<0>(h, jac, values, x, eu, st);
-  else launch_bulk_d<1>(h, jac, values, x, eu, st);
+// jac = 0: the residual pass, which writes the geometry bank when geo is not
+// NULL; jac = 1: the Jacobian pass, which reads it (geo must be the bank the
+// residual pass of the same assembly wrote).
+void launch_bulk(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st, double* geo) {
+  if (h->dprm.model == 0) launch_bulk_d<0>(h, jac, values, x, eu, st, geo);
+  else launch_bulk_d<1>(h, jac, values, x, eu, st, geo);
 }
 }  // namespace
 
@@ -1740,7 +1771,7 @@ void vef_destroy(vef_handle* h) {
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
                   h->d_drag_edges, h->d_drag_terms, h->d_bc_rows, h->d_bc_values,
                   h->d_tmp_b,  h->d_tmp_v, h->d_rowptr_s, h->d_sbase_s, h->d_slen_s,
-                  h->d_cbank_a, h->d_cbank_b, h->d_sbc_rows, h->d_stab_prep};
+                  h->d_cbank_a, h->d_cbank_b, h->d_sbc_rows, h->d_stab_prep, h->d_geob};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -1946,6 +1977,7 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   UP(h->d_ebase, ebase.data(), ebase.size());
   UP(h->d_elen, elen.data(), elen.size());
   UP(h->d_r0bank, (const double*)nullptr, (size_t)n_el * 54);
+  UP(h->d_geob, (const double*)nullptr, (size_t)n_el * 9 * kGeoWords);
   UP(h->d_acbank, (const double*)nullptr, (size_t)n_el * 54);
   UP(h->d_cbank, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
   UP(h->d_ebank, (const double*)nullptr, (size_t)std::max(n_edges, 1) * 18 * kNeval);
@@ -1978,7 +2010,7 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
     VEF_HIP(h, hipEventRecord(ev[0], st));
   }
   // 1. base residual rows of every element
-  launch_bulk(h, 0, values, x, eu, st);
+  launch_bulk(h, 0, values, x, eu, st, jac ? h->d_geob : nullptr);
   VEF_HIP(h, hipGetLastError());
   // 2. outlet edges (residual and FD bank)
   if (h->n_edges > 0) {
@@ -1990,10 +2022,10 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
   // 3. element Jacobians (forward differences, or the FP32 analytic one):
   //    single-contributor entries straight into values
   if (flag == VEF_RES_JAC) {
-    launch_bulk(h, 1, values, x, eu, st);
+    launch_bulk(h, 1, values, x, eu, st, h->d_geob);
     VEF_HIP(h, hipGetLastError());
   } else if (flag == VEF_RES_JAC_FP32) {
-    FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank};
+    FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank, h->d_geob};
     vefad::Params<float> prm{(float)h->params.ReN, (float)h->params.BvN, (float)h->params.VeN, (float)h->params.DfN,
                              (float)h->params.WiN0, (float)(1.0 - h->params.BvN), h->params.model};
     hipLaunchKernelGGL(vef_jac32_kernel, dim3((unsigned)h->n_el), dim3(256), 0, st, h->d_coords, h->d_conn, x, eu,
