@@ -101,12 +101,12 @@ def newton_bench(args):
     (Wi = 0.1, 0.12), then pseudo-arclength steps; value = mean wall time of a
     converged pseudo-arclength step (assembly, arclength and Dirichlet rows,
     GPU factorisation and solves, tangent, predictor) of the reference's
-    Newton policy in FP64 (a fresh Jacobian at the start of every step), the
-    state resident on the GPU.  Reported beside it: the same policy with the
-    scalar-order factorisation, and the inexact Newton (the last factorisation
-    carried across steps) alone, with the learned predictor, with the learned
-    predictor and an FP32 factorisation, with the FP32 analytic Jacobian, and
-    with all three; each with its Wi gained per
+    Newton policy in FP64 (a fresh Jacobian at the start of every step) with
+    the banded LU, the state resident on the GPU.  Reported beside it: the same
+    policy with the two sparse factorisations, and the inexact Newton (the
+    last factorisation carried across steps) alone, with the learned
+    predictor, with the FP32 analytic Jacobian, with both, and with both and
+    an FP32 (sparse) factorisation; each with its Wi gained per
     second and the relative drag difference to an FP64 Newton solve at its
     final Wi.  The CPU baseline runs the reference policy with the oracle's
     assembly and SuperLU on the host cores."""
@@ -124,7 +124,7 @@ def newton_bench(args):
     x0 = initial_state(mesh, LN0)
     edges = boundary_edges(mesh, "Cylinder")
 
-    def run(factorization="hybrid", reuse=False, learned=False, precision="f64", jacobian="fd64"):
+    def run(factorization="band", reuse=False, learned=False, precision="f64", jacobian="fd64"):
         solver = NewtonSolver(mesh, Params(**pset), factorization=factorization, reuse_factors=reuse,
                               linear_precision=precision, jacobian=jacobian)
         cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN,
@@ -150,19 +150,21 @@ def newton_bench(args):
         return {"s_per_step": float(np.mean(arc)), "step_seconds": times, "newton_flags_per_step": flags,
                 "EU_per_step": EU, "Wi_gain_per_second": (EU[-1] - EU[1]) * pset["WiN0"] / sum(arc),
                 "factorizations": solver.lu.refactorizations, "host_fallbacks": solver.lu.host_fallbacks,
+                "solver": type(solver.lu).__name__,
                 "Fd_last_step": Fd, "drag_rel_diff_vs_fp64_newton": abs(Fd - Fr) / abs(Fr),
                 "choices": cont.choices if learned else None, "factor_pattern_nnz":
                     getattr(solver.lu, "nnzS", None) or getattr(solver.lu, "nnzT", None)}
 
     plain = run()
     modes = {
+        "reference_policy_hybrid_lu": run(factorization="hybrid"),
         "reference_policy_scalar_order_lu": run(factorization="rf"),
         "inexact_newton": run(reuse=True),
         "inexact_newton_learned": run(reuse=True, learned=True),
-        "inexact_newton_learned_fp32_factorization": run(reuse=True, learned=True, precision="f32"),
         "inexact_newton_fp32_jacobian": run(reuse=True, jacobian="ad32"),
-        "inexact_newton_learned_fp32_jacobian_fp32_factorization": run(reuse=True, learned=True, precision="f32",
-                                                                        jacobian="ad32"),
+        "inexact_newton_learned_fp32_jacobian": run(reuse=True, learned=True, jacobian="ad32"),
+        "inexact_newton_learned_fp32_jacobian_fp32_factorization_hybrid_lu": run(
+            factorization="hybrid", reuse=True, learned=True, precision="f32", jacobian="ad32"),
     }
     for m in modes.values():
         m["speedup_per_step_vs_reference_policy"] = plain["s_per_step"] / m["s_per_step"]
@@ -183,8 +185,8 @@ def newton_bench(args):
         "config": {
             "workload": "cylinder-in-channel Q2 mesh, 1600 elements, Oldroyd-B beta=0.59 Re=0 DfN=1e-3, arclength "
                         f"continuation in Wi from {LN0} (pinned steps {LN0}, {LN0 + dArN}), Newton tolerance 1e-8 on "
-                        "||b|| and ||s||; reference Newton policy, FP64, scalar re-factorisation in node-block "
-                        "order with block triangular solves",
+                        "||b|| and ||s||; reference Newton policy, FP64, banded LU in reverse Cuthill-McKee node "
+                        "order (libvef_band.so)",
             "elements": mesh.n_elements, "unknowns": mesh.n_unknowns,
             "reference_policy": plain,
             "modes": modes,

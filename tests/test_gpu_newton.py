@@ -63,23 +63,24 @@ def test_refactorization_lu_matches_scipy(oracle):
     assert lu.analyses == 1 and lu.refactorizations == 2 and lu.host_fallbacks == 0
 
 
-@pytest.mark.parametrize("kind", ["block", "hybrid"])
+@pytest.mark.parametrize("kind", ["block", "hybrid", "band"])
 def test_block_factorisations_match_scipy(kind):
-    """BlockLU (6x6 node blocks) and HybridLU (scalar re-factorisation in the
-    node-block order, block solves) on Newton matrices with the arclength and
-    Dirichlet rows: solutions within 1e-9 of scipy's, for two matrices in a
-    row (the second factorisation reuses the analysis)."""
+    """BlockLU (6x6 node blocks), HybridLU (scalar re-factorisation in the
+    node-block order, block solves) and BandLU (banded LU in reverse
+    Cuthill-McKee node order, arclength border) on Newton matrices with the
+    arclength and Dirichlet rows: solutions within 1e-9 of scipy's, for two
+    matrices in a row (the second factorisation reuses the analysis)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
 
-    from ve_flow_around_cylinder_fortran_amd.linsolve import BlockLU, HybridLU
+    from ve_flow_around_cylinder_fortran_amd.linsolve import BandLU, BlockLU, HybridLU
 
     mesh = cylinder_channel(8, 4, 6, 8)
     asm = Assembler(mesh, Params(**OLDROYD_B))
     rows, vals = cylinder_dirichlet(mesh, 0.1)
     asm.set_dirichlet(rows, vals)
     rowptr, col = asm.pattern()
-    lu = (BlockLU if kind == "block" else HybridLU)(rowptr, col)
+    lu = {"block": BlockLU, "hybrid": HybridLU, "band": BandLU}[kind](rowptr, col)
     for k, x in enumerate([initial_state(mesh, 0.1), smooth_state(mesh, EU=0.1)]):
         print(f"[gpu newton] {kind} LU matrix {k}", flush=True)
         xd = torch.from_numpy(x).cuda()
@@ -231,3 +232,39 @@ def test_learned_predictor_gains_over_tangent():
           f"{iters_l} iterations; choices {cont.choices}", flush=True)
     assert "learned" in cont.choices
     assert steps_l < steps_t and iters_l < iters_t
+
+
+def test_band_lu_workload_mesh_and_partial_last_block():
+    """BandLU on the 1,600-element Newton matrix (39,792 node unknowns: the
+    last 64-column block is partial) against scipy, and a Newton solve with it
+    against the default solver's (same flags, states within the Newton
+    tolerance, drag within 1e-10)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from ve_flow_around_cylinder_fortran_amd.linsolve import BandLU
+    from ve_flow_around_cylinder_fortran_amd.mesh import boundary_edges, workload_mesh
+
+    mesh = workload_mesh(1600)
+    x0 = initial_state(mesh, 0.1)
+    print("[gpu newton] band LU, 1600 elements", flush=True)
+    sb = NewtonSolver(mesh, Params(**OLDROYD_B), factorization="band")
+    assert isinstance(sb.lu, BandLU) and (sb.N - 1) % 64 != 0
+    xb, hb = sb.solve(x0, 0.1)
+    sh = NewtonSolver(mesh, Params(**OLDROYD_B), factorization="hybrid")
+    xh, hh = sh.solve(x0, 0.1)
+    assert [h.flag for h in hb] == [h.flag for h in hh] and sb.lu.host_fallbacks == 0
+    assert float((xb - xh).abs().max()) < 1e-8
+    edges = boundary_edges(mesh, "Cylinder")
+    sb.asm.set_drag_edges(edges)
+    sh.asm.set_drag_edges(edges)
+    Fb, Fh = sb.asm.drag(xb).item(), sh.asm.drag(xh).item()
+    assert abs(Fb - Fh) <= 1e-10 * abs(Fh)
+    v = sb.values
+    rowptr, col = sb.asm.pattern()
+    A = sp.csr_matrix((v.cpu().numpy(), col, rowptr), shape=(sb.N, sb.N))
+    rhs = np.random.default_rng(5).standard_normal(sb.N)
+    ref = sla.spsolve(A.tocsc(), rhs)
+    sb.lu.factor(v)
+    s = sb.lu.solve(torch.from_numpy(rhs).cuda()).cpu().numpy()
+    assert np.abs(s - ref).max() / np.abs(ref).max() < 1e-9

@@ -38,8 +38,32 @@ def test_mlp_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
 
 
+def test_band_library_exports_every_declared_symbol():
+    from ve_flow_around_cylinder_fortran_amd import linsolve
+
+    lib = linsolve._band_lib()
+    names = declared_functions("vef_band.h")
+    assert len(names) == 7
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_band_library_rejects_bad_arguments_without_a_gpu():
+    """Argument checks come before any device call (block size other than 64,
+    empty systems, NULL handles)."""
+    from ve_flow_around_cylinder_fortran_amd import linsolve
+
+    lib = linsolve._band_lib()
+    h = ctypes.c_void_p()
+    assert lib.vef_band_create(100, 10, 32, ctypes.byref(h)) != 0 and not h.value
+    assert lib.vef_band_create(0, 10, 64, ctypes.byref(h)) != 0 and not h.value
+    assert lib.vef_band_factor(None, None) != 0
+    assert lib.vef_band_solve(None, None, None, None) != 0
+    assert lib.vef_band_set(None, 0, None, None, None, None) != 0
+
+
 def test_every_header_is_checked():
-    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["vef.h", "vef_mlp.h"]
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["vef.h", "vef_band.h", "vef_mlp.h"]
 
 
 def test_fortran_binding_names_match_header():

@@ -1,7 +1,8 @@
 """Build steps of the package (run by __graft_entry__.build()).
 
-* libvef.so: the HIP assembler, and libvef_mlp.so: the per-node MLP of the
-  learned predictor; hipcc for gfx950, in-tree.  FP64 results must
+* libvef.so: the HIP assembler, libvef_mlp.so: the per-node MLP of the
+  learned predictor, and libvef_band.so: the banded LU of the Newton solve
+  (links rocBLAS for its trailing updates); hipcc for gfx950, in-tree.  FP64 results must
   follow the source's operation order, so contraction into FMA is off
   (-ffp-contract=off) and no fast-math is used; packed FP32 instructions are
   disabled for all HIP code.
@@ -22,6 +23,8 @@ HIP_SRC = os.path.join(PKG, "csrc", "vef_assembly.hip")
 LIB = os.path.join(PKG, "libvef.so")
 MLP_SRC = os.path.join(PKG, "csrc", "vef_mlp.hip")
 MLP_LIB = os.path.join(PKG, "libvef_mlp.so")
+BAND_SRC = os.path.join(PKG, "csrc", "vef_band.hip")
+BAND_LIB = os.path.join(PKG, "libvef_band.so")
 FORTRAN_DIR = os.path.join(PKG, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
 
@@ -46,9 +49,9 @@ def hipcc() -> str:
 
 
 def build_hip(verbose: bool = False) -> str:
-    for src, lib in ((HIP_SRC, LIB), (MLP_SRC, MLP_LIB)):
+    for src, lib, extra in ((HIP_SRC, LIB, []), (MLP_SRC, MLP_LIB, []), (BAND_SRC, BAND_LIB, ["-ffp-contract=fast", "-lrocblas"])):
         tmp = lib + ".tmp"
-        cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, src]
+        cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, src, *extra]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

@@ -667,58 +667,105 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     kg = blockIdx.x % BPE;
   }
   const int64_t e0 = egroup * EPB;
+  if (FD && e0 >= n_el) return;  // padding workgroups of the XCD-aware numbering (uniform)
   const double EU = x[eu_index];
+  const int64_t elast = (int64_t)n_el - 1;
 
   // Gather element unknowns (copyToLocal) and build each slot's state: for
   // evaluation k of an element, unknown j is u_j for j > p, fl(u_p+h) at p and
   // fl(fl(u_j+h)-h) for j < p, p = k-1 — the in-place perturb/restore sequence
   // of storeBulk (Equations_mod.F90:267-306) as perturbation p sees it.
-  for (int t = tid; t < NSLOT * 54; t += 64) {
-    const int ss = t / 54, j = t % 54;
-    const int el = ss / KPB;
-    const int k = kg * KPB + ss % KPB;
-    const int64_t e = e0 + el;
-    double u = 0.0;
-    if (e < n_el) {
-      const int32_t gn = conn[e * 9 + j / 6];
-      u = x[(int64_t)gn * 6 + (j % 6)];
+  // Every gather here issues all of a lane's loads before the first is used
+  // (indices clamped into range, only the LDS writes predicated), so their
+  // latencies overlap instead of adding up.
+  {
+    constexpr int NG = NSLOT * 54, NI = (NG + 63) / 64;
+    int32_t gn[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int t = min(tid + 64 * i, NG - 1);
+      const int64_t e = e0 + (t / 54) / KPB;
+      gn[i] = conn[(e < elast ? e : elast) * 9 + (t % 54) / 6];
     }
-    if (NEVAL > 1) {
-      const int p = k - 1;
-      const double up = u + kEpsJac;
-      u = (j < p) ? up - kEpsJac : ((j == p) ? up : u);
+    double xv[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int t = min(tid + 64 * i, NG - 1);
+      xv[i] = x[(int64_t)gn[i] * 6 + (t % 54) % 6];
     }
-    sS[ss][j] = u;
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int t = tid + 64 * i;
+      if (t < NG) {
+        const int ss = t / 54, j = t % 54;
+        const int el = ss / KPB;
+        const int k = kg * KPB + ss % KPB;
+        double u = (e0 + el < n_el) ? xv[i] : 0.0;
+        if (NEVAL > 1) {
+          const int p = k - 1;
+          const double up = u + kEpsJac;
+          u = (j < p) ? up - kEpsJac : ((j == p) ? up : u);
+        }
+        sS[ss][j] = u;
+      }
+    }
   }
-  for (int t = tid; t < EPB * 18; t += 64) {
-    const int el = t / 18, c = t % 18;
-    const int64_t e = e0 + el;
-    double v = (c & 1) ? 1.0 : 0.0;
-    if (e < n_el) v = coords[(int64_t)conn[e * 9 + c / 2] * 2 + (c & 1)];
-    sX[el][c] = v;
+  {
+    constexpr int NG = EPB * 18, NI = (NG + 63) / 64;
+    int32_t gn[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int t = min(tid + 64 * i, NG - 1);
+      const int64_t e = e0 + t / 18;
+      gn[i] = conn[(e < elast ? e : elast) * 9 + (t % 18) / 2];
+    }
+    double cv[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int t = min(tid + 64 * i, NG - 1);
+      cv[i] = coords[(int64_t)gn[i] * 2 + (t & 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int t = tid + 64 * i;
+      if (t < NG) {
+        const int el = t / 18, c = t % 18;
+        sX[el][c] = (e0 + el < n_el) ? cv[i] : ((c & 1) ? 1.0 : 0.0);
+      }
+    }
   }
-  if (FD && e0 < n_el) {
-    for (int t = tid; t < 81; t += 64) sCode[t] = fo.code[e0 * 81 + t];
-    if (tid < 54) sR0[tid] = fo.r0[e0 * 54 + tid];
+  if constexpr (FD) {
+    // The element's codes, base residual rows, row offsets and Gauss-point
+    // geometry (basis2D, Helem**2 and the two quotients by it, which do not
+    // depend on the state: the residual pass computed them once per element
+    // and wrote them to fo.geo — the same bits: same code, same coordinates).
+    // e0 < n_el here.
+    constexpr int NW = 9 * kGeoWords, NI = (NW + 63) / 64;
+    const int32_t c0 = fo.code[e0 * 81 + tid], c1 = fo.code[e0 * 81 + min(tid + 64, 80)];
+    const double r0v = fo.r0[e0 * 54 + min(tid, 53)];
+    const int64_t r6 = (int64_t)conn[e0 * 9 + min(tid, 8)] * 6;
+    const double* gsrc = fo.geo + e0 * 9 * kGeoWords;
+    double gv[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) gv[i] = gsrc[min(tid + 64 * i, NW - 1)];
+    const int64_t rb = fo.rowptr[r6], re = fo.rowptr[r6 + 1];
+    sCode[tid] = c0;
+    if (tid + 64 < 81) sCode[tid + 64] = c1;
+    if (tid < 54) sR0[tid] = r0v;
     if (tid < 9) {
-      const int64_t r6 = (int64_t)conn[e0 * 9 + tid] * 6;
-      const int64_t rb = fo.rowptr[r6];
       sRb[tid] = rb;
-      sRl[tid] = (int32_t)(fo.rowptr[r6 + 1] - rb);
+      sRl[tid] = (int32_t)(re - rb);
     }
+    double* gdst = reinterpret_cast<double*>(&sG[0][0]);
+#pragma unroll
+    for (int i = 0; i < NI; i++)
+      if (tid + 64 * i < NW) gdst[tid + 64 * i] = gv[i];
   }
   __syncthreads();
-  // Gauss-point geometry (basis2D, Helem**2 and the two quotients by it).  It
-  // does not depend on the state, so the residual pass computes it once per
-  // element, 63 lanes at a time, and hands it to the Jacobian pass through
-  // fo.geo (the same bits: same code, same coordinates).
-  if constexpr (FD) {
-    if (e0 < n_el) {
-      const double* src = fo.geo + e0 * 9 * kGeoWords;
-      double* dst = reinterpret_cast<double*>(&sG[0][0]);
-      for (int t = tid; t < 9 * kGeoWords; t += 64) dst[t] = src[t];
-    }
-  } else {
+  // Gauss-point geometry (basis2D, Helem**2 and the two quotients by it): the
+  // residual pass computes it, 63 lanes at a time, and hands it to the
+  // Jacobian pass through fo.geo when one follows.
+  if constexpr (!FD) {
     for (int t = tid; t < EPB * 9; t += 64) {
       const int el = t / 9, g = t % 9;
       Geo& G = sG[el][g];
@@ -733,8 +780,8 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
       double* dst = fo.geo + e0 * 9 * kGeoWords;
       for (int t = tid; t < nv; t += 64) dst[t] = src[t];
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   const bool active = tid < 9 * NSLOT;
   const int g = active ? tid / NSLOT : 0;
@@ -856,9 +903,15 @@ vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ 
     sRl[tid] = (int32_t)(fo.rowptr[r6 + 1] - rb);
   }
   {
+    constexpr int NW = 9 * kGeoWords, NI = (NW + 255) / 256;
     const double* src = fo.geo + e * 9 * kGeoWords;  // written by the residual pass
     double* dst = reinterpret_cast<double*>(&sG[0]);
-    for (int t = tid; t < 9 * kGeoWords; t += 256) dst[t] = src[t];
+    double gv[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) gv[i] = src[min(tid + 256 * i, NW - 1)];
+#pragma unroll
+    for (int i = 0; i < NI; i++)
+      if (tid + 256 * i < NW) dst[tid + 256 * i] = gv[i];
   }
   __syncthreads();
   if (tid < 9) {
@@ -1324,13 +1377,39 @@ vef_stab_prep_kernel(const double* __restrict__ coords, const int32_t* __restric
   __shared__ Geo sG[63];
   const int tid = threadIdx.x;
   const int64_t e0 = (int64_t)blockIdx.x * 7;
-  for (int t = tid; t < 7 * 54; t += 64) {
-    const int el = t / 54, i = t % 54;
-    if (e0 + el < n_el) sU[el][i] = x[(int64_t)conn[(e0 + el) * 9 + i / 6] * 6 + i % 6];
-  }
-  for (int t = tid; t < 7 * 18; t += 64) {
-    const int el = t / 18, i = t % 18;
-    if (e0 + el < n_el) sX[el][i] = coords[(int64_t)conn[(e0 + el) * 9 + i / 2] * 2 + (i & 1)];
+  const int64_t elast = (int64_t)n_el - 1;
+  // All of a lane's gather loads are issued before the first is used (indices
+  // clamped into range, only the LDS writes predicated).
+  {
+    constexpr int NU = 7 * 54, NX = 7 * 18, IU = (NU + 63) / 64, IX = (NX + 63) / 64;
+    int32_t gu[IU], gx[IX];
+#pragma unroll
+    for (int i = 0; i < IU; i++) {
+      const int t = min(tid + 64 * i, NU - 1);
+      const int64_t e = e0 + t / 54;
+      gu[i] = conn[(e < elast ? e : elast) * 9 + (t % 54) / 6];
+    }
+#pragma unroll
+    for (int i = 0; i < IX; i++) {
+      const int t = min(tid + 64 * i, NX - 1);
+      const int64_t e = e0 + t / 18;
+      gx[i] = conn[(e < elast ? e : elast) * 9 + (t % 18) / 2];
+    }
+    double vu[IU], vx[IX];
+#pragma unroll
+    for (int i = 0; i < IU; i++) vu[i] = x[(int64_t)gu[i] * 6 + (min(tid + 64 * i, NU - 1) % 54) % 6];
+#pragma unroll
+    for (int i = 0; i < IX; i++) vx[i] = coords[(int64_t)gx[i] * 2 + (min(tid + 64 * i, NX - 1) & 1)];
+#pragma unroll
+    for (int i = 0; i < IU; i++) {
+      const int t = tid + 64 * i;
+      if (t < NU) sU[t / 54][t % 54] = vu[i];
+    }
+#pragma unroll
+    for (int i = 0; i < IX; i++) {
+      const int t = tid + 64 * i;
+      if (t < NX) sX[t / 18][t % 18] = vx[i];
+    }
   }
   __syncthreads();
   const int el = tid / 9, g = tid % 9;
@@ -1422,9 +1501,20 @@ vef_stab_kernel(const double* __restrict__ prep, const int32_t* __restrict__ con
   const int kg = (int)((blk >> 3) & 7);
   if (e >= n_el) return;  // uniform over the workgroup
   const double2* src = reinterpret_cast<const double2*>(prep + e * kPrepPerElement);
-  for (int t = tid; t < kPrepCoef / 2; t += 64) reinterpret_cast<double2*>(sW.pre)[t] = src[t];
+  {
+    // all loads first, then the LDS writes
+    constexpr int NP = kPrepCoef / 2, IP = (NP + 63) / 64;
+    double2 pv[IP];
+#pragma unroll
+    for (int i = 0; i < IP; i++) pv[i] = src[min(tid + 64 * i, NP - 1)];
+    const int32_t c0 = code[e * 81 + tid], c1 = code[e * 81 + min(tid + 64, 80)];
+#pragma unroll
+    for (int i = 0; i < IP; i++)
+      if (tid + 64 * i < NP) reinterpret_cast<double2*>(sW.pre)[tid + 64 * i] = pv[i];
+    sCode[tid] = c0;
+    if (tid + 64 < 81) sCode[tid + 64] = c1;
+  }
   const vefstab::NodeCoef* gK = reinterpret_cast<const vefstab::NodeCoef*>(prep + e * kPrepPerElement + kPrepCoef);
-  for (int t = tid; t < 81; t += 64) sCode[t] = code[e * 81 + t];
   if (tid < 9) {
     const int64_t r6 = (int64_t)conn[e * 9 + tid] * 6;
     const int64_t rb = rowptr_s[r6];

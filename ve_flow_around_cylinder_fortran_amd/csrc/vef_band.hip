@@ -1,0 +1,406 @@
+// vef_band.hip — banded LU without pivoting (see include/vef_band.h).
+//
+// Blocked right-looking elimination in blocks of NB = 64 columns.  Step k
+// (columns k0 .. k0+kb-1, m2 = min(w, n - k0 - kb) rows/columns below/right):
+//   1. band_diag_lu     LU of the kb x kb diagonal block and the inverses of
+//                       its L and U, one workgroup, rows in registers;
+//   2. rocblas_dgemm    T1 = A21 U11^-1 and T2 = L11^-1 A12 (the panels);
+//   3. rocblas_dgemm    A22 -= T1 T2 (the m2 x m2 trailing block);
+//   4. band_panel_store T1 and T2 into the band (L21, U12 for the solves).
+// The inverses are kept for the solves.  Without pivoting the factors have no
+// bitwise target (the caller checks the first solve's residual), so this
+// file is compiled with FMA contraction.
+// The band view A(i, j) = AB[K + i + j*(ld - 1)] makes every block a plain
+// column-major matrix with leading dimension ld - 1, which is what the dgemm
+// takes.  A block of the last step with kb < NB is padded with the identity
+// so every loop has the compile-time length NB.
+// The solves push the updates: forward, per block, y_k = L11^-1 b_k and
+// b[below] -= L21 y_k; backward, per block from the last, x_k = U11^-1 y_k and
+// y[above] -= U(above, k) x_k.  One launch per block and sweep, threads over
+// the rows (coalesced down the band's columns).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <utility>
+#include <cstring>
+#include <string>
+
+#include "vef_band.h"
+
+namespace {
+
+constexpr int NB = 64;
+
+// LU of the diagonal block and the inverses of its L and U.  Thread
+// (i, g) = (tid / 4, tid % 4) keeps entries j = g, g+4, ... of row i of the
+// block, of L^-1 and of U^-1 in registers.  At each step the finished row is
+// published to LDS (double-buffered, so a step needs one barrier) and read
+// back as a broadcast; the element of a thread's own row that a step needs
+// comes from the lane of the same row that holds it (a shuffle: the four
+// threads of a row are adjacent lanes of one wavefront).  The steps are
+// template instances, so every register index is a compile-time constant.
+constexpr int RC = NB / 4;  // register entries per thread and row
+
+struct DiagRows {
+  double d[RC], x[RC], y[RC];
+  int i, g, row0;
+  double (*P)[NB];
+  double (*Q)[NB];
+};
+
+template <int K>
+__device__ __forceinline__ void lu_step(DiagRows& R) {
+  constexpr int b = K & 1, kq = K >> 2, kr = K & 3;
+  if (R.i == K) {
+#pragma unroll
+    for (int q = 0; q < RC; q++) R.P[b][R.g + 4 * q] = R.d[q];
+  }
+  __syncthreads();
+  const double dik = __shfl(R.d[kq], R.row0 | kr);  // D[i][K]
+  if (R.i > K) {
+    const double l = dik * (1.0 / R.P[b][K]);
+#pragma unroll
+    for (int q = 0; q < RC; q++) {
+      const int j = R.g + 4 * q;
+      if (j > K) R.d[q] -= l * R.P[b][j];
+    }
+    if (R.g == kr) R.d[kq] = l;
+  }
+}
+
+// Step K of L^-1 (X[i] -= L[i][K] X[K] over j <= K) together with step
+// KU = NB-1-K of U^-1 (Y[KU] /= U[KU][KU]; Y[i] -= U[i][KU] Y[KU] over j >= KU
+// for the rows above).
+template <int K>
+__device__ __forceinline__ void inv_step(DiagRows& R) {
+  constexpr int b = K & 1, KU = NB - 1 - K;
+  const double uik = __shfl(R.d[KU >> 2], R.row0 | (KU & 3));  // U[i][KU]
+  if (R.i == KU) {
+    const double inv = 1.0 / uik;
+#pragma unroll
+    for (int q = 0; q < RC; q++) {
+      R.y[q] *= inv;
+      R.Q[b][R.g + 4 * q] = R.y[q];
+    }
+  }
+  if (R.i == K) {
+#pragma unroll
+    for (int q = 0; q < RC; q++) R.P[b][R.g + 4 * q] = R.x[q];
+  }
+  __syncthreads();
+  const double lik = __shfl(R.d[K >> 2], R.row0 | (K & 3));  // L[i][K]
+  if (R.i > K) {
+#pragma unroll
+    for (int q = 0; q < RC; q++) {
+      const int j = R.g + 4 * q;
+      if (j <= K) R.x[q] -= lik * R.P[b][j];
+    }
+  }
+  if (R.i < KU) {
+#pragma unroll
+    for (int q = 0; q < RC; q++) {
+      const int j = R.g + 4 * q;
+      if (j >= KU) R.y[q] -= uik * R.Q[b][j];
+    }
+  }
+}
+
+template <int... Ks>
+__device__ __forceinline__ void lu_steps(DiagRows& R, std::integer_sequence<int, Ks...>) {
+  (lu_step<Ks>(R), ...);
+}
+template <int... Ks>
+__device__ __forceinline__ void inv_steps(DiagRows& R, std::integer_sequence<int, Ks...>) {
+  (inv_step<Ks>(R), ...);
+}
+
+__global__ void __launch_bounds__(256)
+band_diag_lu(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, double* __restrict__ invL,
+             double* __restrict__ invU) {
+  __shared__ double P[2][NB];  // published row: of L\U, then of L^-1
+  __shared__ double Q[2][NB];  // published row of U^-1
+  DiagRows R;
+  const int tid = threadIdx.x;
+  R.i = tid >> 2;
+  R.g = tid & 3;
+  R.row0 = (tid & 63) & ~3;
+  R.P = P;
+  R.Q = Q;
+  const int64_t lda = ld - 1;
+  double* base = AB + K + k0 + k0 * lda;  // A(k0, k0)
+#pragma unroll
+  for (int q = 0; q < RC; q++) {
+    const int j = R.g + 4 * q;
+    R.d[q] = (R.i < kb && j < kb) ? base[R.i + j * lda] : (R.i == j ? 1.0 : 0.0);  // identity padding
+    R.x[q] = (R.i == j) ? 1.0 : 0.0;
+    R.y[q] = R.x[q];
+  }
+  lu_steps(R, std::make_integer_sequence<int, NB - 1>{});
+  inv_steps(R, std::make_integer_sequence<int, NB>{});
+#pragma unroll
+  for (int q = 0; q < RC; q++) {
+    const int j = R.g + 4 * q;
+    if (R.i < kb && j < kb) base[R.i + j * lda] = R.d[q];
+    invL[R.i + j * NB] = R.x[q];  // column-major NB x NB
+    invU[R.i + j * NB] = R.y[q];
+  }
+}
+
+// Loads a column-major NB x NB block into LDS (blockDim.x = 256, all loads in
+// flight first).
+__device__ __forceinline__ void stage_block(const double* __restrict__ src, double (*M)[NB + 1]) {
+  constexpr int Q = NB * NB / 256;
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) v[q] = src[threadIdx.x + 256 * q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int e = threadIdx.x + 256 * q;
+    M[e % NB][e / NB] = v[q];
+  }
+}
+
+// The panels, computed by dgemm into T1 (m2 x kb, ld m2) and T2 (kb x m2, ld
+// NB), go back into the band: A21 = T1, A12 = T2.
+__global__ void __launch_bounds__(256)
+band_panel_store(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t m2,
+                 const double* __restrict__ T1, const double* __restrict__ T2) {
+  const int64_t lda = ld - 1;
+  const int64_t total = m2 * kb;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e % m2, c = e / m2;             // A21(r, c) = T1[r + c*m2]
+    AB[K + (k0 + kb + r) + (k0 + c) * lda] = T1[e];
+    const int64_t i = e % kb, jc = e / kb;             // A12(i, jc) = T2[i + jc*NB]
+    AB[K + (k0 + i) + (k0 + kb + jc) * lda] = T2[i + jc * NB];
+  }
+}
+
+// The block inverse (column-major NB x NB) and the block's right-hand side go
+// to LDS with all loads in flight; v = M r (padding rows of r are zero).
+__device__ __forceinline__ void block_gemv(const double* __restrict__ M, const double* __restrict__ rhs, int kb,
+                                           double (*sM)[NB + 1], double* sr, double* out) {
+  const int t = threadIdx.x;
+  const double rv = (t < kb) ? rhs[t] : 0.0;
+  stage_block(M, sM);
+  if (t < NB) sr[t] = rv;
+  __syncthreads();
+  if (t < NB) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) s += sM[t][j] * sr[j];
+    out[t] = s;
+  }
+  __syncthreads();
+}
+
+// Forward sweep, block k: y_k = L11^-1 b_k; b[k0+kb+r] -= L21(r, :) y_k.
+__global__ void __launch_bounds__(256)
+band_fwd(const double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t m2,
+         const double* __restrict__ invLk, double* __restrict__ b, double* __restrict__ y) {
+  __shared__ double sM[NB][NB + 1];
+  __shared__ double sr[NB], yk[NB];
+  const int t = threadIdx.x;
+  const int64_t lda = ld - 1;
+  // the row's band entries are loaded before the block product needs the wait
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + t;
+  const int64_t rr = r < m2 ? r : 0;
+  const double* a = AB + K + (k0 + kb + rr) + k0 * lda;
+  double av[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) av[j] = (r < m2 && j < kb) ? a[j * lda] : 0.0;
+  block_gemv(invLk, b + k0, kb, sM, sr, yk);
+  if (blockIdx.x == 0 && t < kb) y[k0 + t] = yk[t];
+  if (r < m2) {
+    double s = b[k0 + kb + r];
+#pragma unroll
+    for (int j = 0; j < NB; j++) s -= av[j] * yk[j];
+    b[k0 + kb + r] = s;
+  }
+}
+
+// Backward sweep, block k: x_k = U11^-1 y_k; y[k0-n2+r] -= U(k0-n2+r, k0+j) x_k.
+__global__ void __launch_bounds__(256)
+band_bwd(const double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t n2,
+         const double* __restrict__ invUk, double* __restrict__ y, double* __restrict__ x) {
+  __shared__ double sM[NB][NB + 1];
+  __shared__ double sr[NB], xk[NB];
+  const int t = threadIdx.x;
+  const int64_t lda = ld - 1;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + t;
+  const int64_t row = k0 - n2 + (r < n2 ? r : 0);
+  const double* a = AB + K + row + k0 * lda;  // A(row, k0)
+  double av[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) av[j] = (r < n2 && j < kb) ? a[j * lda] : 0.0;
+  block_gemv(invUk, y + k0, kb, sM, sr, xk);
+  if (blockIdx.x == 0 && t < kb) x[k0 + t] = xk[t];
+  if (r < n2) {
+    double s = y[row];
+#pragma unroll
+    for (int j = 0; j < NB; j++) s -= av[j] * xk[j];
+    y[row] = s;
+  }
+}
+
+__global__ void band_zero(double* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+
+__global__ void band_scatter(double* __restrict__ AB, int64_t count, const int64_t* __restrict__ src,
+                             const int64_t* __restrict__ dst, const double* __restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    AB[dst[i]] = v[src[i]];
+}
+
+}  // namespace
+
+struct vef_band {
+  int64_t n = 0;
+  int32_t w = 0, K = 0;
+  int64_t ld = 0;
+  int64_t nblk = 0;
+  double* AB = nullptr;
+  double* invL = nullptr;
+  double* invU = nullptr;
+  double* y = nullptr;
+  double* T1 = nullptr;  // panel products of one step: w x NB and NB x w
+  double* T2 = nullptr;
+  rocblas_handle rb = nullptr;
+  std::string err;
+};
+
+namespace {
+int fail(vef_band* h, const std::string& m) {
+  if (h) h->err = m;
+  return 1;
+}
+#define BAND_HIP(h, call)                                                       \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) return fail(h, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+}  // namespace
+
+extern "C" {
+
+int vef_band_create(int64_t n, int32_t w, int32_t nb, vef_band** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (n < 1 || w < 0 || nb != NB) return 1;
+  vef_band* h = new vef_band();
+  h->n = n;
+  h->w = w;
+  h->K = w + NB;
+  h->ld = 2 * (int64_t)h->K + 1;
+  h->nblk = (n + NB - 1) / NB;
+  int rc = 0;
+  if (hipMalloc((void**)&h->AB, sizeof(double) * n * h->ld) != hipSuccess ||
+      hipMalloc((void**)&h->invL, sizeof(double) * h->nblk * NB * NB) != hipSuccess ||
+      hipMalloc((void**)&h->invU, sizeof(double) * h->nblk * NB * NB) != hipSuccess ||
+      hipMalloc((void**)&h->y, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&h->T1, sizeof(double) * std::max<int64_t>(w, 1) * NB) != hipSuccess ||
+      hipMalloc((void**)&h->T2, sizeof(double) * std::max<int64_t>(w, 1) * NB) != hipSuccess)
+    rc = 1;
+  if (!rc && hipMemset(h->AB, 0, sizeof(double) * n * h->ld) != hipSuccess) rc = 1;
+  if (!rc && rocblas_create_handle(&h->rb) != rocblas_status_success) rc = 1;
+  if (rc) {
+    vef_band_destroy(h);
+    return 1;
+  }
+  *out = h;
+  return 0;
+}
+
+int vef_band_storage(vef_band* h, double** ab, int64_t* ld, int32_t* K) {
+  if (!h || !ab || !ld || !K) return 1;
+  *ab = h->AB;
+  *ld = h->ld;
+  *K = h->K;
+  return 0;
+}
+
+int vef_band_set(vef_band* h, int64_t count, const int64_t* src, const int64_t* dst, const double* values,
+                 void* stream) {
+  if (!h) return 1;
+  if (count < 0 || (count > 0 && (!src || !dst || !values))) return fail(h, "vef_band_set: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(band_zero, dim3(2048), dim3(256), 0, st, h->AB, h->n * h->ld);
+  BAND_HIP(h, hipGetLastError());
+  if (count > 0) {
+    hipLaunchKernelGGL(band_scatter, dim3(1024), dim3(256), 0, st, h->AB, count, src, dst, values);
+    BAND_HIP(h, hipGetLastError());
+  }
+  return 0;
+}
+
+int vef_band_factor(vef_band* h, void* stream) {
+  if (!h) return 1;
+  hipStream_t st = (hipStream_t)stream;
+  if (rocblas_set_stream(h->rb, st) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
+  const int64_t lda = h->ld - 1;
+  auto A = [&](int64_t i, int64_t j) { return h->AB + h->K + i + j * lda; };
+  const double one = 1.0, mone = -1.0;
+  for (int64_t b = 0; b < h->nblk; b++) {
+    const int64_t k0 = b * NB;
+    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
+    const int64_t m2 = std::min<int64_t>(h->w, h->n - k0 - kb);
+    hipLaunchKernelGGL(band_diag_lu, dim3(1), dim3(256), 0, st, h->AB, h->ld, h->K, k0, kb, h->invL + b * NB * NB,
+                       h->invU + b * NB * NB);
+    BAND_HIP(h, hipGetLastError());
+    if (m2 > 0) {
+      const rocblas_int M2 = (rocblas_int)m2, KB = kb, LDA = (rocblas_int)lda;
+      const double zero = 0.0;
+      // T1 = A21 U11^-1 (m2 x kb), T2 = L11^-1 A12 (kb x m2), A22 -= T1 T2
+      if (rocblas_dgemm(h->rb, rocblas_operation_none, rocblas_operation_none, M2, KB, KB, &one, A(k0 + kb, k0), LDA,
+                        h->invU + b * NB * NB, NB, &zero, h->T1, M2) != rocblas_status_success ||
+          rocblas_dgemm(h->rb, rocblas_operation_none, rocblas_operation_none, KB, M2, KB, &one,
+                        h->invL + b * NB * NB, NB, A(k0, k0 + kb), LDA, &zero, h->T2, NB) != rocblas_status_success ||
+          rocblas_dgemm(h->rb, rocblas_operation_none, rocblas_operation_none, M2, M2, KB, &mone, h->T1, M2, h->T2,
+                        NB, &one, A(k0 + kb, k0 + kb), LDA) != rocblas_status_success)
+        return fail(h, "rocblas_dgemm failed");
+      hipLaunchKernelGGL(band_panel_store, dim3((unsigned)std::min<int64_t>(1024, (m2 * kb + 255) / 256)), dim3(256),
+                         0, st, h->AB, h->ld, h->K, k0, kb, m2, h->T1, h->T2);
+      BAND_HIP(h, hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+int vef_band_solve(vef_band* h, double* b, double* x, void* stream) {
+  if (!h || !b || !x) return 1;
+  hipStream_t st = (hipStream_t)stream;
+  for (int64_t k = 0; k < h->nblk; k++) {
+    const int64_t k0 = k * NB;
+    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
+    const int64_t m2 = std::min<int64_t>(h->w, h->n - k0 - kb);
+    const unsigned g = (unsigned)std::max<int64_t>(1, (m2 + 255) / 256);
+    hipLaunchKernelGGL(band_fwd, dim3(g), dim3(256), 0, st, h->AB, h->ld, h->K, k0, kb, m2, h->invL + k * NB * NB, b,
+                       h->y);
+    BAND_HIP(h, hipGetLastError());
+  }
+  for (int64_t k = h->nblk - 1; k >= 0; k--) {
+    const int64_t k0 = k * NB;
+    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
+    const int64_t n2 = std::min<int64_t>(h->w, k0);
+    const unsigned g = (unsigned)std::max<int64_t>(1, (n2 + 255) / 256);
+    hipLaunchKernelGGL(band_bwd, dim3(g), dim3(256), 0, st, h->AB, h->ld, h->K, k0, kb, n2, h->invU + k * NB * NB,
+                       h->y, x);
+    BAND_HIP(h, hipGetLastError());
+  }
+  return 0;
+}
+
+const char* vef_band_last_error(const vef_band* h) { return h ? h->err.c_str() : "handle is NULL"; }
+
+void vef_band_destroy(vef_band* h) {
+  if (!h) return;
+  if (h->rb) rocblas_destroy_handle(h->rb);
+  for (double* p : {h->AB, h->invL, h->invU, h->y, h->T1, h->T2})
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+}  // extern "C"

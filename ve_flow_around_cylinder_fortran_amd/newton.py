@@ -121,11 +121,11 @@ class NewtonSolver:
     (the Newton iteration absorbs a less accurate solve)."""
 
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
-                 reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = "hybrid",
+                 reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = None,
                  jacobian: str = "fd64"):
         import torch
 
-        from .linsolve import BlockLU, HybridLU, RfLU
+        from .linsolve import BandLU, BlockLU, HybridLU, RfLU
 
         if params.VeN != 0.0 or params.model != "L-PTT":
             raise NewtonError("the closed-form inflow covers Oldroyd-B (L-PTT with VeN = 0) only")
@@ -135,10 +135,13 @@ class NewtonSolver:
         self.device = device
         self.asm = Assembler(mesh, params)
         rowptr, col = self.asm.pattern()
-        classes = {"rf": RfLU, "block": BlockLU, "hybrid": HybridLU}
+        classes = {"rf": RfLU, "block": BlockLU, "hybrid": HybridLU, "band": BandLU}
+        if factorization is None:  # the fastest that supports the precision
+            factorization = "band" if linear_precision == "f64" else "hybrid"
         if factorization not in classes:
-            raise NewtonError("factorization must be 'rf' (scalar re-factorisation), 'block' (6x6 node blocks) or "
-                              "'hybrid' (scalar re-factorisation in node-block order, block solves)")
+            raise NewtonError("factorization must be 'rf' (scalar re-factorisation), 'block' (6x6 node blocks), "
+                              "'hybrid' (scalar re-factorisation in node-block order, block solves) or 'band' "
+                              "(banded LU in reverse Cuthill-McKee node order)")
         lu_class = classes[factorization]
         self.lu = lu_class(rowptr, col, device=device, precision=linear_precision,
                            check_tol=1e-6 if (reuse_factors and linear_precision == "f64") else None)
