@@ -56,7 +56,15 @@ STAB_FLOPS_PER_EVAL = 35289
 STAB_FLOPS_BASE = 9261
 STAB_COEF_FLOPS = 79
 STAB_QUOTIENT_FLOPS = 2 * 2916 * 3
+# The FP32 analytic Jacobian's work per element (VEF_RES_JAC_FP32): 207 dual-
+# number evaluations of the Gauss-point terms (9 Gauss points x 23
+# directions, AD32_DUAL_FLOPS each), the field interpolations, the WET
+# products and the two contractions with the basis functions.  Pinned by
+# tests/test_ad_jacobian_host.py.
+AD32_DUAL_FLOPS = 1649
+AD32_FLOPS_PER_ELEMENT = 207 * AD32_DUAL_FLOPS + 9 * 22 * 9 * 2 + 207 * 18 + 224532
 PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector = matrix), FMA counted as 2
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 (vector = matrix), FMA counted as 2
 PEAK_HBM_GBS = 8000.0
 N_ELEMENTS = 100000
 
@@ -471,6 +479,14 @@ def main():
                 "jacobian_kernel_ms": jac32_ms / max(launches32, 1),
                 "scatter_ms": scatter32_ms / max(launches32, 1),
                 "relative_difference_vs_fp64_fd": rel32,
+                "roofline": {
+                    "kernel": "vef_jac32_kernel", "bound": "fp32 compute",
+                    "achieved": ne * AD32_FLOPS_PER_ELEMENT / (jac32_ms / max(launches32, 1) * 1e-3) / 1e12,
+                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ne * AD32_FLOPS_PER_ELEMENT / (jac32_ms / max(launches32, 1) * 1e-3) / 1e12
+                            / PEAK_FP32_TFLOPS,
+                    "flops_per_element": AD32_FLOPS_PER_ELEMENT,
+                },
                 "note": "VEF_RES_JAC_FP32: the same FP64 residual, the Jacobian from forward-mode "
                         "differentiation of the Gauss-point terms in FP32 contracted with the basis functions",
             },

@@ -4,6 +4,13 @@
 // from caller-supplied Gauss-point geometry and basis tables.
 #include "vef_ad_math.h"
 
+#include "flop_real.hpp"
+
+// The counting scalar's square root and magnitude for the dual-number
+// functions (found by argument-dependent lookup).
+inline FlopReal sqrt(FlopReal a) { return vsqrt(a); }
+inline FlopReal fabs(FlopReal a) { return fabsr(a); }
+
 namespace {
 using vefad::Dual;
 using vefad::kNumFields;
@@ -53,5 +60,31 @@ extern "C" int ad_host_element(const double* prm5, int model, const double* u54,
         }
     }
   }
+  return 0;
+}
+
+// Operations of one dual-number evaluation of gauss_terms (one seeded
+// direction), counted with the oracle's instrumented scalar: out = add, mul,
+// div, sqrt, total.  Every operation the kernel executes is counted, also
+// those on zero derivative parts.
+extern "C" int ad_count_gauss_terms(const double* prm5, int model, const double* F, const double* dFx,
+                                    const double* dFy, double H2, int direction, unsigned long long* out) {
+  vefad::Params<FlopReal> prm{FlopReal(prm5[0]), FlopReal(prm5[1]), FlopReal(prm5[2]), FlopReal(prm5[3]),
+                              FlopReal(prm5[4]), FlopReal(1.0 - prm5[1]), model};
+  Dual<FlopReal> Fd[kNumFields];
+  for (int i = 0; i < kNumFields; i++) Fd[i] = Dual<FlopReal>(FlopReal(F[i]), FlopReal(i == direction ? 1.0 : 0.0));
+  FlopReal fx[9], fy[9];
+  for (int n = 0; n < 9; n++) {
+    fx[n] = FlopReal(dFx[n]);
+    fy[n] = FlopReal(dFy[n]);
+  }
+  Dual<FlopReal> T[3][6];
+  FlopCounter::reset();
+  vefad::gauss_terms(Fd, fx, fy, FlopReal(H2), prm, T);
+  out[0] = FlopCounter::add;
+  out[1] = FlopCounter::mul;
+  out[2] = FlopCounter::div;
+  out[3] = FlopCounter::sqrt;
+  out[4] = FlopCounter::total();
   return 0;
 }

@@ -31,11 +31,14 @@ def ad_lib(tmp_path_factory):
     if not os.path.exists(hipcc):
         pytest.skip("hipcc not available")
     out = str(tmp_path_factory.mktemp("ad") / "libad_host.so")
-    subprocess.run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-I", CSRC, "-o", out,
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950", "-I", CSRC, "-I",
+                    os.path.join(os.path.dirname(HERE), "oracle"), "-o", out,
                     os.path.join(HERE, "jacobian", "ad_host_check.hip")], check=True, capture_output=True)
     L = ctypes.CDLL(out)
     L.ad_host_element.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double] + \
         [ctypes.c_void_p] * 10
+    L.ad_count_gauss_terms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
     return L
 
 
@@ -90,3 +93,30 @@ def test_fd_tolerance_catches_a_dropped_derivative(ad_lib, oracle):
     J2 = J.copy()
     J2[:, 3] = 0.0
     assert np.linalg.norm(J2 - A) > 1e-5 * np.linalg.norm(A)
+
+
+def test_fp32_jacobian_operation_count(ad_lib):
+    """The FP32 Jacobian's work per element that bench.py reports against the
+    FP32 peak: 207 dual-number evaluations of the Gauss-point terms (9 Gauss
+    points x 23 directions; every executed operation counted with the
+    oracle's instrumented scalar, the same for every direction), the 22 field
+    interpolations per Gauss point, the 18 products by WET per evaluation and
+    the two contractions with the basis functions."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(os.path.dirname(HERE), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    prm5 = np.array([0.0, 0.59, 0.0, 1e-3, 1.0])  # the bench workload: Oldroyd-B, Re = 0, DfN = 1e-3
+    rng = np.random.default_rng(9)
+    F = rng.uniform(0.2, 1.2, 23)
+    dFx, dFy = rng.standard_normal(9), rng.standard_normal(9)
+    counts = set()
+    for direction in (0, 5, 11, 22):
+        out = np.zeros(5, dtype=np.uint64)
+        ad_lib.ad_count_gauss_terms(_p(prm5), 0, _p(F), _p(dFx), _p(dFy), 0.3, direction, _p(out))
+        counts.add(int(out[4]))
+    assert counts == {bench.AD32_DUAL_FLOPS}
+    contraction = 9 * 18 * 9 * 2 * (5 + 5 + 3 + 3 + 3 + 3) + 9 * (2916 + 54) * 6
+    assert contraction == 224532
+    assert bench.AD32_FLOPS_PER_ELEMENT == 207 * bench.AD32_DUAL_FLOPS + 9 * 22 * 9 * 2 + 207 * 18 + contraction

@@ -30,6 +30,12 @@
 
 #include <cmath>
 
+// No bitwise target on this path (it is compared with the forward differences
+// to FP32 accuracy), so multiply-adds may contract into FMAs here; the pragma
+// is switched back off at the end of the header, leaving the FP64 code that
+// follows the include as compiled (-ffp-contract=off).
+#pragma clang fp contract(fast)
+
 namespace vefad {
 
 template <class T>
@@ -203,5 +209,7 @@ __host__ __device__ inline void gauss_terms(const R* F, const S* dFx, const S* d
 }
 
 }  // namespace vefad
+
+#pragma clang fp contract(off)
 
 #endif

@@ -875,6 +875,7 @@ __host__ __device__ constexpr int field_of(int p, int j) {
 __global__ void __launch_bounds__(256)
 vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
                  int64_t eu_index, int32_t n_el, vefad::Params<float> prm, FdOut fo) {
+#pragma clang fp contract(fast)  // FP32 path: compared to FP32 accuracy, FMAs allowed
   constexpr int NF = vefad::kNumFields;
   constexpr int NOUT = 2916 + 54;          // matrix entries and the dR/dEU column
   constexpr int PER = (NOUT + 255) / 256;  // entries per thread
