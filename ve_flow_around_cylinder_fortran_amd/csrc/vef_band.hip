@@ -60,7 +60,10 @@ __device__ __forceinline__ void lu_step(DiagRows& R) {
   __syncthreads();
   const double dik = __shfl(R.d[kq], R.row0 | kr);  // D[i][K]
   if (R.i > K) {
-    const double l = dik * (1.0 / R.P[b][K]);
+    const double pv = R.P[b][K];
+    double rp = __builtin_amdgcn_rcp(pv);  // reciprocal estimate and one Newton step
+    rp = rp * (2.0 - pv * rp);
+    const double l = dik * rp;
 #pragma unroll
     for (int q = 0; q < RC; q++) {
       const int j = R.g + 4 * q;
@@ -78,7 +81,8 @@ __device__ __forceinline__ void inv_step(DiagRows& R) {
   constexpr int b = K & 1, KU = NB - 1 - K;
   const double uik = __shfl(R.d[KU >> 2], R.row0 | (KU & 3));  // U[i][KU]
   if (R.i == KU) {
-    const double inv = 1.0 / uik;
+    double inv = __builtin_amdgcn_rcp(uik);
+    inv = inv * (2.0 - uik * inv);
 #pragma unroll
     for (int q = 0; q < RC; q++) {
       R.y[q] *= inv;

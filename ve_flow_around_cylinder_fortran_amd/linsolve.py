@@ -631,7 +631,8 @@ def _band_lib():
     """libvef_band.so (banded LU kernels, include/vef_band.h)."""
     global _band
     if _band is None:
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libvef_band.so")
+        path = os.environ.get("VEF_BAND_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                   "libvef_band.so")
         if not os.path.exists(path):
             raise RfLUError(f"{path} not found: build it with __graft_entry__.build()")
         L = ctypes.CDLL(path)
