@@ -49,6 +49,8 @@ def lib(flops: bool = False):
             L.vo_tables.argtypes = [vp]
             L.vo_basis2d.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
             L.vo_gauss_fields.argtypes = [vp, vp, ctypes.c_int, vp]
+            L.vo_gauss_tau.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, vp]
+            L.vo_tau_plant.argtypes = [ctypes.c_int]
             L.vo_point_strong_form.argtypes = [ctypes.POINTER(OracleParams), vp, vp, vp, ctypes.c_double, vp, vp, vp,
                                                ctypes.c_double, vp]
             L.vo_element.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -222,6 +224,22 @@ def nodal_stress(BvN, WiN, n_nodes, x):
 # ---------------------------------------------------------------------------
 # Linear stability assembly (Cylinder_Stability_2D, vef_oracle.cpp's f3 part).
 # ---------------------------------------------------------------------------
+def gauss_tau(prm, xy, TL, EU, gp):
+    """Stabilization quantities at bulk Gauss point gp of one element."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    TL = np.ascontiguousarray(TL, dtype=np.float64)
+    out = np.zeros(6)
+    if lib().vo_gauss_tau(ctypes.byref(prm), _p(xy), _p(TL), float(EU), int(gp), _p(out)) != 0:
+        raise ValueError(f"gauss_tau: Gauss point {gp} out of range")
+    return dict(H2=out[0], tsugn=out[1], tlsme=out[2], tlsic=out[3], tlsce=out[4], tdcs=out[5])
+
+
+def tau_plant(which: int):
+    """Plants a known mistake in the tau coefficients (0 removes it)."""
+    if lib().vo_tau_plant(int(which)) != 0:
+        raise ValueError(f"tau_plant: no plant {which}")
+
+
 def _stab_rc(rc, what):
     if rc == -2:
         raise ValueError(f"{what}: the stability assembly needs DfN > 1e-12 (discontinuity capturing off)")

@@ -214,6 +214,7 @@ struct GsPt {
   vreal detJ, WET;
   vreal mag_re, mag_im;  // mag_con_EQ (complex sqrt of a real; purely imaginary if negative)
   vreal tlsic, tlsme, tlsce, tdcs;
+  vreal H2, tsugn;       // Helem**2 and the streamline sum of setStabilizationTerms
   vreal WiN;             // module global WiN, rewritten per Gauss point
 };
 
@@ -500,6 +501,10 @@ static void strongFormMomentum(const Phys& ph, GsPt& G) {
 // ----------------------------------------------------------------------------
 // a7: setStabilizationTerms, Element_Calculations_mod.F90:701-773
 // ----------------------------------------------------------------------------
+// Planted mistakes in the tau coefficients, for the tests that must catch
+// them (0: none; vo_tau_plant lists them).
+static int g_tau_plant = 0;
+
 // Helem**2 of setStabilizationTerms (Element_Calculations_mod.F90:718-727):
 // geometry only.
 static vreal elementH2(const Elem& E) {
@@ -507,7 +512,7 @@ static vreal elementH2(const Elem& E) {
   for (int n = 0; n < 9; n++)
     Helem = Helem + E.dFdXi[0][n] * E.dFdXi[0][n] + E.dFdXi[1][n] * E.dFdXi[1][n] +
             E.dFdXi[2][n] * E.dFdXi[2][n];
-  Helem = vreal(1.0) / vsqrt(Helem);
+  Helem = vreal(1.0) / (g_tau_plant == 1 ? Helem : vsqrt(Helem));
   return Helem * Helem;
 }
 
@@ -515,23 +520,26 @@ static void setStabilizationTerms(const Phys& ph, const Elem& E, GsPt& G) {
   vreal tsugn = Z();
   for (int n = 0; n < 9; n++) {
     vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
-    tsugn = tsugn + fabsr(dot3(G.U_f, dF));
+    vreal un = dot3(G.U_f, dF);
+    tsugn = tsugn + (g_tau_plant == 2 ? un : fabsr(un));
   }
   vreal H2 = elementH2(E);
+  G.H2 = H2;
+  G.tsugn = tsugn;
 
   vreal ddT = ddot33(G.T_ve, G.T_ve);
   vreal ddG = ddot33(G.G_dot_tr, G.G_dot_tr);
   vreal x = (vreal(1.0) - ph.BvN) / G.WiN;
-  vreal Ha = vsqrt(x * x + vreal(0.5) * ddT);
-  Ha = cdiv(Ha, vsqrt(vreal(1.0) + vreal(0.5) * ddG));
+  vreal Ha = vsqrt(x * x + (g_tau_plant == 4 ? ddT : vreal(0.5) * ddT));
+  if (g_tau_plant != 5) Ha = cdiv(Ha, vsqrt(vreal(1.0) + vreal(0.5) * ddG));
 
-  vreal r1 = ph.ReN / ph.dt;
+  vreal r1 = (g_tau_plant == 13 ? vreal(1.0) : ph.ReN) / ph.dt;
   vreal r2 = ph.ReN * tsugn;
-  vreal r3 = ph.BvN / H2;
+  vreal r3 = ph.BvN / (g_tau_plant == 3 ? vsqrt(H2) : H2);
   vreal r4 = cdiv(Ha, H2);
   G.tlsme = r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4;
   G.tlsme = cdiv(vreal(1.0), vsqrt(G.tlsme));
-  G.tlsic = cdiv(H2, G.tlsme);
+  G.tlsic = g_tau_plant == 6 ? H2 * G.tlsme : cdiv(H2, G.tlsme);
 
   // tdcs = (Helem**2)/sqrt(0.5*S:S), zeroed when DfN > 1e-12 (:746-760).  Work
   // whose result the reference overwrites or multiplies by a zero coefficient
@@ -541,9 +549,11 @@ static void setStabilizationTerms(const Phys& ph, const Elem& E, GsPt& G) {
   } else {
     vreal ddS = ddot33(G.S, G.S);
     G.tdcs = vreal(0.5) * ddS;
-    G.tdcs = cdiv(H2, vsqrt(G.tdcs));
+    G.tdcs = cdiv(H2, g_tau_plant == 11 ? G.tdcs : vsqrt(G.tdcs));
   }
-  vreal ddC = (ph.model == 0) ? Z() : ddot33(G.C, G.C);  // multiplied by f(3) = 0 for L-PTT
+  vreal ddC = (ph.model == 0)      ? Z()
+              : g_tau_plant == 10 ? ddot33(G.T_ve, G.T_ve)
+                                  : ddot33(G.C, G.C);  // multiplied by f(3) = 0 for L-PTT
 
   // mag_con_EQ = sqrt(0.5*Con_EQ:Con_EQ); complex sqrt of a real argument.  Only
   // used multiplied by tdcs.
@@ -555,17 +565,17 @@ static void setStabilizationTerms(const Phys& ph, const Elem& E, GsPt& G) {
     else G.mag_im = vsqrt(-mc);
   }
 
-  vreal t1 = G.WiN / ph.dt;
-  vreal t2 = G.WiN * tsugn;
-  vreal t3 = G.WiN * vsqrt(vreal(0.5) * ddG);
-  vreal t4 = G.f[1];
+  vreal t1 = (g_tau_plant == 14 ? vreal(1.0) : G.WiN) / ph.dt;
+  vreal t2 = g_tau_plant == 7 ? tsugn : G.WiN * tsugn;
+  vreal t3 = G.WiN * vsqrt(g_tau_plant == 8 ? ddG : vreal(0.5) * ddG);
+  vreal t4 = g_tau_plant == 9 ? G.f[0] : G.f[1];
   vreal t5 = G.f[2] * vsqrt(vreal(0.5) * ddC);
   // ((tdcs*mag)/H2)**2 with mag complex: exactly one of mag_re, mag_im is nonzero,
   // so the square is real: +re^2 or -im^2.
   vreal z6r = cdiv(G.tdcs * G.mag_re, H2);
   vreal z6i = cdiv(G.tdcs * G.mag_im, H2);
   vreal t6 = z6r * z6r - z6i * z6i;
-  vreal t7 = ph.DfN / H2;
+  vreal t7 = ph.DfN / (g_tau_plant == 12 ? vsqrt(H2) : H2);
   G.tlsce = t1 * t1 + t2 * t2 + t3 * t3 + t4 * t4 + t5 * t5 + t6 + t7 * t7;
   G.tlsce = cdiv(vreal(1.0), vsqrt(G.tlsce));
 }
@@ -1623,6 +1633,45 @@ static void load_elem(Elem& E, const double* xy, const double* TL, double EU) {
 
 int vo_stab_plant(int which) {
   g_stab_plant = which;
+  return 0;
+}
+
+// Plants a mistake in setStabilizationTerms (0 removes it):
+//  1 Helem = 1/sum (no square root)      2 tsugn without abs()
+//  3 BvN/Helem in tau_LSME               4 T:T without the 1/2 in Ha
+//  5 Ha not divided by sqrt(1+G:G/2)     6 tau_LSIC = Helem**2 * tau_LSME
+//  7 tau_LSCE's WiN*tsugn without WiN    8 G:G without the 1/2 in tau_LSCE
+//  9 f(1) in place of f(2)               10 T:T in place of C:C
+// 11 tdcs without the square root        12 DfN/Helem in tau_LSCE
+// 13 1/dt in place of ReN/dt             14 1/dt in place of WiN/dt
+int vo_tau_plant(int which) {
+  if (which < 0 || which > 14) return -1;
+  g_tau_plant = which;
+  return 0;
+}
+
+// One bulk Gauss point of an element: the quantities setStabilizationTerms
+// forms (Element_Calculations_mod.F90:701-773) from the element's nodal values.
+// out: Helem**2, tsugn, tlsme, tlsic, tlsce, tdcs.
+int vo_gauss_tau(const vo_params* p, const double* xy, const double* TL, double EU, int gp, double* out) {
+  if (gp < 0 || gp > 8) return -1;
+  Phys ph = make_phys(p);
+  const Tables& T = tables();
+  Elem E;
+  load_elem(E, xy, TL, EU);
+  E.bfn = T.bfn[gp]; E.dfdc = T.dfdc[gp]; E.dfde = T.dfde[gp];
+  E.d2fdc2 = T.d2fdc2[gp]; E.d2fde2 = T.d2fde2[gp]; E.d2fdce = T.d2fdce[gp];
+  vreal I[3][3];
+  identity(I);
+  GsPt G;
+  basis2D(E, G);
+  setFlowQuantities(ph, E, G, I);
+  out[0] = G.H2;
+  out[1] = G.tsugn;
+  out[2] = G.tlsme;
+  out[3] = G.tlsic;
+  out[4] = G.tlsce;
+  out[5] = G.tdcs;
   return 0;
 }
 
