@@ -171,6 +171,9 @@ def newton_bench(args):
         "inexact_newton_learned": run(reuse=True, learned=True),
         "inexact_newton_fp32_jacobian": run(reuse=True, jacobian="ad32"),
         "inexact_newton_learned_fp32_jacobian": run(reuse=True, learned=True, jacobian="ad32"),
+        "inexact_newton_fp32_jacobian_fp32_factorization": run(reuse=True, precision="f32", jacobian="ad32"),
+        "inexact_newton_learned_fp32_jacobian_fp32_factorization": run(
+            reuse=True, learned=True, precision="f32", jacobian="ad32"),
         "inexact_newton_learned_fp32_jacobian_fp32_factorization_hybrid_lu": run(
             factorization="hybrid", reuse=True, learned=True, precision="f32", jacobian="ad32"),
     }

@@ -43,7 +43,7 @@ def test_band_library_exports_every_declared_symbol():
 
     lib = linsolve._band_lib()
     names = declared_functions("vef_band.h")
-    assert len(names) == 7
+    assert len(names) == 9
     for n in names:
         assert hasattr(lib, n), n
 
@@ -57,7 +57,9 @@ def test_band_library_rejects_bad_arguments_without_a_gpu():
     h = ctypes.c_void_p()
     assert lib.vef_band_create(100, 10, 32, ctypes.byref(h)) != 0 and not h.value
     assert lib.vef_band_create(0, 10, 64, ctypes.byref(h)) != 0 and not h.value
+    assert lib.vef_band_create_prec(100, 10, 64, 16, ctypes.byref(h)) != 0 and not h.value
     assert lib.vef_band_factor(None, None) != 0
+    assert lib.vef_band_uses_graph(None) == 0
     assert lib.vef_band_solve(None, None, None, None) != 0
     assert lib.vef_band_set(None, 0, None, None, None, None) != 0
 

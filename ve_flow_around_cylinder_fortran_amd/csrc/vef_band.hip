@@ -18,6 +18,16 @@
 // b[below] -= L21 y_k; backward, per block from the last, x_k = U11^-1 y_k and
 // y[above] -= U(above, k) x_k.  One launch per block and sweep, threads over
 // the rows (coalesced down the band's columns).
+//
+// The factorisation's launch sequence depends only on the handle (its
+// buffers are fixed), so from the second call on it is replayed as a HIP
+// graph captured once on a private stream: the 5 launches per step otherwise
+// cost more host time than their GPU time.  If the capture fails the
+// launches stay direct.
+//
+// Every kernel is a template on the stored type T: double, or float for a
+// single-precision factorisation (the band, the block inverses and the panels
+// in FP32, sgemm for the products; the solves' vectors and sums stay FP64).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -43,27 +53,35 @@ constexpr int NB = 64;
 // template instances, so every register index is a compile-time constant.
 constexpr int RC = NB / 4;  // register entries per thread and row
 
+template <class T>
 struct DiagRows {
-  double d[RC], x[RC], y[RC];
+  T d[RC], x[RC], y[RC];
   int i, g, row0;
-  double (*P)[NB];
-  double (*Q)[NB];
+  T (*P)[NB];
+  T (*Q)[NB];
 };
 
-template <int K>
-__device__ __forceinline__ void lu_step(DiagRows& R) {
+// Reciprocal estimate refined by one Newton step.
+__device__ __forceinline__ double recip(double v) {
+  double r = __builtin_amdgcn_rcp(v);
+  return r * (2.0 - v * r);
+}
+__device__ __forceinline__ float recip(float v) {
+  float r = __builtin_amdgcn_rcpf(v);
+  return r * (2.0f - v * r);
+}
+
+template <int K, class T>
+__device__ __forceinline__ void lu_step(DiagRows<T>& R) {
   constexpr int b = K & 1, kq = K >> 2, kr = K & 3;
   if (R.i == K) {
 #pragma unroll
     for (int q = 0; q < RC; q++) R.P[b][R.g + 4 * q] = R.d[q];
   }
   __syncthreads();
-  const double dik = __shfl(R.d[kq], R.row0 | kr);  // D[i][K]
+  const T dik = __shfl(R.d[kq], R.row0 | kr);  // D[i][K]
   if (R.i > K) {
-    const double pv = R.P[b][K];
-    double rp = __builtin_amdgcn_rcp(pv);  // reciprocal estimate and one Newton step
-    rp = rp * (2.0 - pv * rp);
-    const double l = dik * rp;
+    const T l = dik * recip(R.P[b][K]);
 #pragma unroll
     for (int q = 0; q < RC; q++) {
       const int j = R.g + 4 * q;
@@ -76,13 +94,12 @@ __device__ __forceinline__ void lu_step(DiagRows& R) {
 // Step K of L^-1 (X[i] -= L[i][K] X[K] over j <= K) together with step
 // KU = NB-1-K of U^-1 (Y[KU] /= U[KU][KU]; Y[i] -= U[i][KU] Y[KU] over j >= KU
 // for the rows above).
-template <int K>
-__device__ __forceinline__ void inv_step(DiagRows& R) {
+template <int K, class T>
+__device__ __forceinline__ void inv_step(DiagRows<T>& R) {
   constexpr int b = K & 1, KU = NB - 1 - K;
-  const double uik = __shfl(R.d[KU >> 2], R.row0 | (KU & 3));  // U[i][KU]
+  const T uik = __shfl(R.d[KU >> 2], R.row0 | (KU & 3));  // U[i][KU]
   if (R.i == KU) {
-    double inv = __builtin_amdgcn_rcp(uik);
-    inv = inv * (2.0 - uik * inv);
+    const T inv = recip(uik);
 #pragma unroll
     for (int q = 0; q < RC; q++) {
       R.y[q] *= inv;
@@ -94,7 +111,7 @@ __device__ __forceinline__ void inv_step(DiagRows& R) {
     for (int q = 0; q < RC; q++) R.P[b][R.g + 4 * q] = R.x[q];
   }
   __syncthreads();
-  const double lik = __shfl(R.d[K >> 2], R.row0 | (K & 3));  // L[i][K]
+  const T lik = __shfl(R.d[K >> 2], R.row0 | (K & 3));  // L[i][K]
   if (R.i > K) {
 #pragma unroll
     for (int q = 0; q < RC; q++) {
@@ -111,21 +128,22 @@ __device__ __forceinline__ void inv_step(DiagRows& R) {
   }
 }
 
-template <int... Ks>
-__device__ __forceinline__ void lu_steps(DiagRows& R, std::integer_sequence<int, Ks...>) {
+template <class T, int... Ks>
+__device__ __forceinline__ void lu_steps(DiagRows<T>& R, std::integer_sequence<int, Ks...>) {
   (lu_step<Ks>(R), ...);
 }
-template <int... Ks>
-__device__ __forceinline__ void inv_steps(DiagRows& R, std::integer_sequence<int, Ks...>) {
+template <class T, int... Ks>
+__device__ __forceinline__ void inv_steps(DiagRows<T>& R, std::integer_sequence<int, Ks...>) {
   (inv_step<Ks>(R), ...);
 }
 
+template <class T>
 __global__ void __launch_bounds__(256)
-band_diag_lu(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, double* __restrict__ invL,
-             double* __restrict__ invU) {
-  __shared__ double P[2][NB];  // published row: of L\U, then of L^-1
-  __shared__ double Q[2][NB];  // published row of U^-1
-  DiagRows R;
+band_diag_lu(T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, T* __restrict__ invL,
+             T* __restrict__ invU) {
+  __shared__ T P[2][NB];  // published row: of L\U, then of L^-1
+  __shared__ T Q[2][NB];  // published row of U^-1
+  DiagRows<T> R;
   const int tid = threadIdx.x;
   R.i = tid >> 2;
   R.g = tid & 3;
@@ -133,12 +151,12 @@ band_diag_lu(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb,
   R.P = P;
   R.Q = Q;
   const int64_t lda = ld - 1;
-  double* base = AB + K + k0 + k0 * lda;  // A(k0, k0)
+  T* base = AB + K + k0 + k0 * lda;  // A(k0, k0)
 #pragma unroll
   for (int q = 0; q < RC; q++) {
     const int j = R.g + 4 * q;
-    R.d[q] = (R.i < kb && j < kb) ? base[R.i + j * lda] : (R.i == j ? 1.0 : 0.0);  // identity padding
-    R.x[q] = (R.i == j) ? 1.0 : 0.0;
+    R.d[q] = (R.i < kb && j < kb) ? base[R.i + j * lda] : T(R.i == j ? 1 : 0);  // identity padding
+    R.x[q] = T(R.i == j ? 1 : 0);
     R.y[q] = R.x[q];
   }
   lu_steps(R, std::make_integer_sequence<int, NB - 1>{});
@@ -154,9 +172,10 @@ band_diag_lu(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb,
 
 // Loads a column-major NB x NB block into LDS (blockDim.x = 256, all loads in
 // flight first).
-__device__ __forceinline__ void stage_block(const double* __restrict__ src, double (*M)[NB + 1]) {
+template <class T>
+__device__ __forceinline__ void stage_block(const T* __restrict__ src, T (*M)[NB + 1]) {
   constexpr int Q = NB * NB / 256;
-  double v[Q];
+  T v[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) v[q] = src[threadIdx.x + 256 * q];
 #pragma unroll
@@ -168,9 +187,10 @@ __device__ __forceinline__ void stage_block(const double* __restrict__ src, doub
 
 // The panels, computed by dgemm into T1 (m2 x kb, ld m2) and T2 (kb x m2, ld
 // NB), go back into the band: A21 = T1, A12 = T2.
+template <class T>
 __global__ void __launch_bounds__(256)
-band_panel_store(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t m2,
-                 const double* __restrict__ T1, const double* __restrict__ T2) {
+band_panel_store(T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t m2, const T* __restrict__ T1,
+                 const T* __restrict__ T2) {
   const int64_t lda = ld - 1;
   const int64_t total = m2 * kb;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -182,9 +202,11 @@ band_panel_store(double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int
 }
 
 // The block inverse (column-major NB x NB) and the block's right-hand side go
-// to LDS with all loads in flight; v = M r (padding rows of r are zero).
-__device__ __forceinline__ void block_gemv(const double* __restrict__ M, const double* __restrict__ rhs, int kb,
-                                           double (*sM)[NB + 1], double* sr, double* out) {
+// to LDS with all loads in flight; v = M r (padding rows of r are zero), summed
+// in FP64.
+template <class T>
+__device__ __forceinline__ void block_gemv(const T* __restrict__ M, const double* __restrict__ rhs, int kb,
+                                           T (*sM)[NB + 1], double* sr, double* out) {
   const int t = threadIdx.x;
   const double rv = (t < kb) ? rhs[t] : 0.0;
   stage_block(M, sM);
@@ -193,70 +215,88 @@ __device__ __forceinline__ void block_gemv(const double* __restrict__ M, const d
   if (t < NB) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < NB; j++) s += sM[t][j] * sr[j];
+    for (int j = 0; j < NB; j++) s += (double)sM[t][j] * sr[j];
     out[t] = s;
   }
   __syncthreads();
 }
 
 // Forward sweep, block k: y_k = L11^-1 b_k; b[k0+kb+r] -= L21(r, :) y_k.
+template <class T>
 __global__ void __launch_bounds__(256)
-band_fwd(const double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t m2,
-         const double* __restrict__ invLk, double* __restrict__ b, double* __restrict__ y) {
-  __shared__ double sM[NB][NB + 1];
+band_fwd(const T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t m2,
+         const T* __restrict__ invLk, double* __restrict__ b, double* __restrict__ y) {
+  __shared__ T sM[NB][NB + 1];
   __shared__ double sr[NB], yk[NB];
   const int t = threadIdx.x;
   const int64_t lda = ld - 1;
-  // the row's band entries are loaded before the block product needs the wait
+  // The row's band entries are loaded before the block product needs the
+  // wait, unconditionally (rows past m2 read row k0 + kb; columns past n are
+  // the storage's zero padding) so that all NB loads are in flight together.
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + t;
   const int64_t rr = r < m2 ? r : 0;
-  const double* a = AB + K + (k0 + kb + rr) + k0 * lda;
-  double av[NB];
+  const T* a = AB + K + (k0 + kb + rr) + k0 * lda;
+  T av[NB];
 #pragma unroll
-  for (int j = 0; j < NB; j++) av[j] = (r < m2 && j < kb) ? a[j * lda] : 0.0;
+  for (int j = 0; j < NB; j++) av[j] = a[j * lda];
   block_gemv(invLk, b + k0, kb, sM, sr, yk);
   if (blockIdx.x == 0 && t < kb) y[k0 + t] = yk[t];
   if (r < m2) {
     double s = b[k0 + kb + r];
 #pragma unroll
-    for (int j = 0; j < NB; j++) s -= av[j] * yk[j];
+    for (int j = 0; j < NB; j++) s -= (double)av[j] * yk[j];  // yk[j] = 0 and av[j] = 0 past kb
     b[k0 + kb + r] = s;
   }
 }
 
 // Backward sweep, block k: x_k = U11^-1 y_k; y[k0-n2+r] -= U(k0-n2+r, k0+j) x_k.
+template <class T>
 __global__ void __launch_bounds__(256)
-band_bwd(const double* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t n2,
-         const double* __restrict__ invUk, double* __restrict__ y, double* __restrict__ x) {
-  __shared__ double sM[NB][NB + 1];
+band_bwd(const T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, int64_t n2,
+         const T* __restrict__ invUk, double* __restrict__ y, double* __restrict__ x) {
+  __shared__ T sM[NB][NB + 1];
   __shared__ double sr[NB], xk[NB];
   const int t = threadIdx.x;
   const int64_t lda = ld - 1;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + t;
   const int64_t row = k0 - n2 + (r < n2 ? r : 0);
-  const double* a = AB + K + row + k0 * lda;  // A(row, k0)
-  double av[NB];
+  const T* a = AB + K + row + k0 * lda;  // A(row, k0); unconditional loads as in band_fwd
+  T av[NB];
 #pragma unroll
-  for (int j = 0; j < NB; j++) av[j] = (r < n2 && j < kb) ? a[j * lda] : 0.0;
+  for (int j = 0; j < NB; j++) av[j] = a[j * lda];
   block_gemv(invUk, y + k0, kb, sM, sr, xk);
   if (blockIdx.x == 0 && t < kb) x[k0 + t] = xk[t];
   if (r < n2) {
     double s = y[row];
 #pragma unroll
-    for (int j = 0; j < NB; j++) s -= av[j] * xk[j];
+    for (int j = 0; j < NB; j++) s -= (double)av[j] * xk[j];  // xk[j] = 0 and av[j] = 0 past kb
     y[row] = s;
   }
 }
 
-__global__ void band_zero(double* __restrict__ a, int64_t n) {
+template <class T>
+__global__ void band_zero(T* __restrict__ a, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    a[i] = 0.0;
+    a[i] = T(0);
 }
 
-__global__ void band_scatter(double* __restrict__ AB, int64_t count, const int64_t* __restrict__ src,
+// AB[dst[i]] = values[src[i]], rounded to T.
+template <class T>
+__global__ void band_scatter(T* __restrict__ AB, int64_t count, const int64_t* __restrict__ src,
                              const int64_t* __restrict__ dst, const double* __restrict__ v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-    AB[dst[i]] = v[src[i]];
+    AB[dst[i]] = (T)v[src[i]];
+}
+
+rocblas_status gemm(rocblas_handle h, rocblas_int m, rocblas_int n, rocblas_int k, double alpha, const double* A,
+                    rocblas_int lda, const double* B, rocblas_int ldb, double beta, double* C, rocblas_int ldc) {
+  return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C,
+                       ldc);
+}
+rocblas_status gemm(rocblas_handle h, rocblas_int m, rocblas_int n, rocblas_int k, float alpha, const float* A,
+                    rocblas_int lda, const float* B, rocblas_int ldb, float beta, float* C, rocblas_int ldc) {
+  return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C,
+                       ldc);
 }
 
 }  // namespace
@@ -266,13 +306,18 @@ struct vef_band {
   int32_t w = 0, K = 0;
   int64_t ld = 0;
   int64_t nblk = 0;
-  double* AB = nullptr;
-  double* invL = nullptr;
-  double* invU = nullptr;
+  int32_t bits = 64;     // stored type: 64 double, 32 float
+  void* AB = nullptr;    // band, block inverses and panel products in the stored type
+  void* invL = nullptr;
+  void* invU = nullptr;
+  void* T1 = nullptr;    // panel products of one step: w x NB and NB x w
+  void* T2 = nullptr;
   double* y = nullptr;
-  double* T1 = nullptr;  // panel products of one step: w x NB and NB x w
-  double* T2 = nullptr;
   rocblas_handle rb = nullptr;
+  hipStream_t cs = nullptr;       // capture stream of the factorisation graph
+  hipGraphExec_t gexec = nullptr; // the factorisation as one graph (second call on)
+  int64_t factor_calls = 0;
+  bool graph_off = false;         // capture failed: launch directly
   std::string err;
 };
 
@@ -288,27 +333,100 @@ int fail(vef_band* h, const std::string& m) {
   } while (0)
 }  // namespace
 
+namespace {
+
+template <class T>
+int factor_impl(vef_band* h, hipStream_t st) {
+  const int64_t lda = h->ld - 1;
+  T* AB = (T*)h->AB;
+  T* invL = (T*)h->invL;
+  T* invU = (T*)h->invU;
+  T* T1 = (T*)h->T1;
+  T* T2 = (T*)h->T2;
+  auto A = [&](int64_t i, int64_t j) { return AB + h->K + i + j * lda; };
+  for (int64_t b = 0; b < h->nblk; b++) {
+    const int64_t k0 = b * NB;
+    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
+    const int64_t m2 = std::min<int64_t>(h->w, h->n - k0 - kb);
+    hipLaunchKernelGGL(band_diag_lu<T>, dim3(1), dim3(256), 0, st, AB, h->ld, h->K, k0, kb, invL + b * NB * NB,
+                       invU + b * NB * NB);
+    BAND_HIP(h, hipGetLastError());
+    if (m2 > 0) {
+      const rocblas_int M2 = (rocblas_int)m2, KB = kb, LDA = (rocblas_int)lda;
+      // T1 = A21 U11^-1 (m2 x kb), T2 = L11^-1 A12 (kb x m2), A22 -= T1 T2
+      if (gemm(h->rb, M2, KB, KB, T(1), A(k0 + kb, k0), LDA, invU + b * NB * NB, NB, T(0), T1, M2) !=
+              rocblas_status_success ||
+          gemm(h->rb, KB, M2, KB, T(1), invL + b * NB * NB, NB, A(k0, k0 + kb), LDA, T(0), T2, NB) !=
+              rocblas_status_success ||
+          gemm(h->rb, M2, M2, KB, T(-1), T1, M2, T2, NB, T(1), A(k0 + kb, k0 + kb), LDA) != rocblas_status_success)
+        return fail(h, "rocblas gemm failed");
+      hipLaunchKernelGGL(band_panel_store<T>, dim3((unsigned)std::min<int64_t>(1024, (m2 * kb + 255) / 256)),
+                         dim3(256), 0, st, AB, h->ld, h->K, k0, kb, m2, (const T*)T1, (const T*)T2);
+      BAND_HIP(h, hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+template <class T>
+int solve_impl(vef_band* h, double* b, double* x, hipStream_t st) {
+  const T* AB = (const T*)h->AB;
+  const T* invL = (const T*)h->invL;
+  const T* invU = (const T*)h->invU;
+  for (int64_t k = 0; k < h->nblk; k++) {
+    const int64_t k0 = k * NB;
+    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
+    const int64_t m2 = std::min<int64_t>(h->w, h->n - k0 - kb);
+    const unsigned g = (unsigned)std::max<int64_t>(1, (m2 + 255) / 256);
+    hipLaunchKernelGGL(band_fwd<T>, dim3(g), dim3(256), 0, st, AB, h->ld, h->K, k0, kb, m2, invL + k * NB * NB, b,
+                       h->y);
+    BAND_HIP(h, hipGetLastError());
+  }
+  for (int64_t k = h->nblk - 1; k >= 0; k--) {
+    const int64_t k0 = k * NB;
+    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
+    const int64_t n2 = std::min<int64_t>(h->w, k0);
+    const unsigned g = (unsigned)std::max<int64_t>(1, (n2 + 255) / 256);
+    hipLaunchKernelGGL(band_bwd<T>, dim3(g), dim3(256), 0, st, AB, h->ld, h->K, k0, kb, n2, invU + k * NB * NB, h->y,
+                       x);
+    BAND_HIP(h, hipGetLastError());
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int vef_band_create(int64_t n, int32_t w, int32_t nb, vef_band** out) {
+  return vef_band_create_prec(n, w, nb, 64, out);
+}
+
+int vef_band_create_prec(int64_t n, int32_t w, int32_t nb, int32_t bits, vef_band** out) {
   if (!out) return 1;
   *out = nullptr;
-  if (n < 1 || w < 0 || nb != NB) return 1;
+  if (n < 1 || w < 0 || nb != NB || (bits != 64 && bits != 32)) return 1;
   vef_band* h = new vef_band();
   h->n = n;
   h->w = w;
   h->K = w + NB;
   h->ld = 2 * (int64_t)h->K + 1;
   h->nblk = (n + NB - 1) / NB;
+  h->bits = bits;
+  const size_t es = bits / 8;
+  // NB columns of zero padding past column n - 1 (never written: every
+  // factorisation step stays inside the n x n matrix) let the solves load a
+  // whole block row without bounds tests at the last block.
+  const size_t band_bytes = es * (n + NB) * h->ld;
   int rc = 0;
-  if (hipMalloc((void**)&h->AB, sizeof(double) * n * h->ld) != hipSuccess ||
-      hipMalloc((void**)&h->invL, sizeof(double) * h->nblk * NB * NB) != hipSuccess ||
-      hipMalloc((void**)&h->invU, sizeof(double) * h->nblk * NB * NB) != hipSuccess ||
+  if (hipMalloc(&h->AB, band_bytes) != hipSuccess ||
+      hipMalloc(&h->invL, es * h->nblk * NB * NB) != hipSuccess ||
+      hipMalloc(&h->invU, es * h->nblk * NB * NB) != hipSuccess ||
       hipMalloc((void**)&h->y, sizeof(double) * n) != hipSuccess ||
-      hipMalloc((void**)&h->T1, sizeof(double) * std::max<int64_t>(w, 1) * NB) != hipSuccess ||
-      hipMalloc((void**)&h->T2, sizeof(double) * std::max<int64_t>(w, 1) * NB) != hipSuccess)
+      hipMalloc(&h->T1, es * std::max<int64_t>(w, 1) * NB) != hipSuccess ||
+      hipMalloc(&h->T2, es * std::max<int64_t>(w, 1) * NB) != hipSuccess)
     rc = 1;
-  if (!rc && hipMemset(h->AB, 0, sizeof(double) * n * h->ld) != hipSuccess) rc = 1;
+  if (!rc && hipMemset(h->AB, 0, band_bytes) != hipSuccess) rc = 1;
   if (!rc && rocblas_create_handle(&h->rb) != rocblas_status_success) rc = 1;
   if (rc) {
     vef_band_destroy(h);
@@ -319,8 +437,11 @@ int vef_band_create(int64_t n, int32_t w, int32_t nb, vef_band** out) {
 }
 
 int vef_band_storage(vef_band* h, double** ab, int64_t* ld, int32_t* K) {
-  if (!h || !ab || !ld || !K) return 1;
-  *ab = h->AB;
+  if (!h || !ld || !K) return 1;
+  if (ab) {
+    if (h->bits != 64) return fail(h, "vef_band_storage: the band of an FP32 handle is not double (pass ab = NULL)");
+    *ab = (double*)h->AB;
+  }
   *ld = h->ld;
   *K = h->K;
   return 0;
@@ -331,78 +452,68 @@ int vef_band_set(vef_band* h, int64_t count, const int64_t* src, const int64_t* 
   if (!h) return 1;
   if (count < 0 || (count > 0 && (!src || !dst || !values))) return fail(h, "vef_band_set: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(band_zero, dim3(2048), dim3(256), 0, st, h->AB, h->n * h->ld);
-  BAND_HIP(h, hipGetLastError());
-  if (count > 0) {
-    hipLaunchKernelGGL(band_scatter, dim3(1024), dim3(256), 0, st, h->AB, count, src, dst, values);
+  if (h->bits == 64) {
+    hipLaunchKernelGGL(band_zero<double>, dim3(2048), dim3(256), 0, st, (double*)h->AB, h->n * h->ld);
     BAND_HIP(h, hipGetLastError());
+    if (count > 0) hipLaunchKernelGGL(band_scatter<double>, dim3(1024), dim3(256), 0, st, (double*)h->AB, count, src, dst,
+                                      values);
+  } else {
+    hipLaunchKernelGGL(band_zero<float>, dim3(2048), dim3(256), 0, st, (float*)h->AB, h->n * h->ld);
+    BAND_HIP(h, hipGetLastError());
+    if (count > 0) hipLaunchKernelGGL(band_scatter<float>, dim3(1024), dim3(256), 0, st, (float*)h->AB, count, src, dst,
+                                      values);
   }
+  BAND_HIP(h, hipGetLastError());
   return 0;
 }
 
 int vef_band_factor(vef_band* h, void* stream) {
   if (!h) return 1;
   hipStream_t st = (hipStream_t)stream;
-  if (rocblas_set_stream(h->rb, st) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
-  const int64_t lda = h->ld - 1;
-  auto A = [&](int64_t i, int64_t j) { return h->AB + h->K + i + j * lda; };
-  const double one = 1.0, mone = -1.0;
-  for (int64_t b = 0; b < h->nblk; b++) {
-    const int64_t k0 = b * NB;
-    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
-    const int64_t m2 = std::min<int64_t>(h->w, h->n - k0 - kb);
-    hipLaunchKernelGGL(band_diag_lu, dim3(1), dim3(256), 0, st, h->AB, h->ld, h->K, k0, kb, h->invL + b * NB * NB,
-                       h->invU + b * NB * NB);
-    BAND_HIP(h, hipGetLastError());
-    if (m2 > 0) {
-      const rocblas_int M2 = (rocblas_int)m2, KB = kb, LDA = (rocblas_int)lda;
-      const double zero = 0.0;
-      // T1 = A21 U11^-1 (m2 x kb), T2 = L11^-1 A12 (kb x m2), A22 -= T1 T2
-      if (rocblas_dgemm(h->rb, rocblas_operation_none, rocblas_operation_none, M2, KB, KB, &one, A(k0 + kb, k0), LDA,
-                        h->invU + b * NB * NB, NB, &zero, h->T1, M2) != rocblas_status_success ||
-          rocblas_dgemm(h->rb, rocblas_operation_none, rocblas_operation_none, KB, M2, KB, &one,
-                        h->invL + b * NB * NB, NB, A(k0, k0 + kb), LDA, &zero, h->T2, NB) != rocblas_status_success ||
-          rocblas_dgemm(h->rb, rocblas_operation_none, rocblas_operation_none, M2, M2, KB, &mone, h->T1, M2, h->T2,
-                        NB, &one, A(k0 + kb, k0 + kb), LDA) != rocblas_status_success)
-        return fail(h, "rocblas_dgemm failed");
-      hipLaunchKernelGGL(band_panel_store, dim3((unsigned)std::min<int64_t>(1024, (m2 * kb + 255) / 256)), dim3(256),
-                         0, st, h->AB, h->ld, h->K, k0, kb, m2, h->T1, h->T2);
-      BAND_HIP(h, hipGetLastError());
-    }
+  if (h->gexec) {
+    BAND_HIP(h, hipGraphLaunch(h->gexec, st));
+    return 0;
   }
-  return 0;
+  if (h->factor_calls++ > 0 && !h->graph_off) {
+    // Capture the sequence once (nothing runs while capturing), then launch it.
+    if (!h->cs && hipStreamCreateWithFlags(&h->cs, hipStreamNonBlocking) != hipSuccess) h->cs = nullptr;
+    if (h->cs && rocblas_set_stream(h->rb, h->cs) == rocblas_status_success &&
+        hipStreamBeginCapture(h->cs, hipStreamCaptureModeRelaxed) == hipSuccess) {
+      const int rc = h->bits == 64 ? factor_impl<double>(h, h->cs) : factor_impl<float>(h, h->cs);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(h->cs, &g);
+      hipGraphExec_t x = nullptr;
+      if (rc == 0 && e == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) h->gexec = x;
+      if (g) (void)hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();  // a failed capture leaves its error here; the direct path below reports its own
+    h->err.clear();
+    if (h->gexec) {
+      BAND_HIP(h, hipGraphLaunch(h->gexec, st));
+      return 0;
+    }
+    h->graph_off = true;
+  }
+  if (rocblas_set_stream(h->rb, st) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
+  return h->bits == 64 ? factor_impl<double>(h, st) : factor_impl<float>(h, st);
 }
+
+int vef_band_uses_graph(const vef_band* h) { return h && h->gexec ? 1 : 0; }
 
 int vef_band_solve(vef_band* h, double* b, double* x, void* stream) {
   if (!h || !b || !x) return 1;
   hipStream_t st = (hipStream_t)stream;
-  for (int64_t k = 0; k < h->nblk; k++) {
-    const int64_t k0 = k * NB;
-    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
-    const int64_t m2 = std::min<int64_t>(h->w, h->n - k0 - kb);
-    const unsigned g = (unsigned)std::max<int64_t>(1, (m2 + 255) / 256);
-    hipLaunchKernelGGL(band_fwd, dim3(g), dim3(256), 0, st, h->AB, h->ld, h->K, k0, kb, m2, h->invL + k * NB * NB, b,
-                       h->y);
-    BAND_HIP(h, hipGetLastError());
-  }
-  for (int64_t k = h->nblk - 1; k >= 0; k--) {
-    const int64_t k0 = k * NB;
-    const int kb = (int)std::min<int64_t>(NB, h->n - k0);
-    const int64_t n2 = std::min<int64_t>(h->w, k0);
-    const unsigned g = (unsigned)std::max<int64_t>(1, (n2 + 255) / 256);
-    hipLaunchKernelGGL(band_bwd, dim3(g), dim3(256), 0, st, h->AB, h->ld, h->K, k0, kb, n2, h->invU + k * NB * NB,
-                       h->y, x);
-    BAND_HIP(h, hipGetLastError());
-  }
-  return 0;
+  return h->bits == 64 ? solve_impl<double>(h, b, x, st) : solve_impl<float>(h, b, x, st);
 }
 
 const char* vef_band_last_error(const vef_band* h) { return h ? h->err.c_str() : "handle is NULL"; }
 
 void vef_band_destroy(vef_band* h) {
   if (!h) return;
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->cs) (void)hipStreamDestroy(h->cs);
   if (h->rb) rocblas_destroy_handle(h->rb);
-  for (double* p : {h->AB, h->invL, h->invU, h->y, h->T1, h->T2})
+  for (void* p : {h->AB, h->invL, h->invU, (void*)h->y, h->T1, h->T2})
     if (p) (void)hipFree(p);
   delete h;
 }

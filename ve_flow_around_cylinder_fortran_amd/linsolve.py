@@ -638,9 +638,11 @@ def _band_lib():
         L = ctypes.CDLL(path)
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         L.vef_band_create.argtypes = [i64, i32, i32, ctypes.POINTER(vp)]
+        L.vef_band_create_prec.argtypes = [i64, i32, i32, i32, ctypes.POINTER(vp)]
         L.vef_band_storage.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(i64), ctypes.POINTER(i32)]
         L.vef_band_set.argtypes = [vp, i64, vp, vp, vp, vp]
         L.vef_band_factor.argtypes = [vp, vp]
+        L.vef_band_uses_graph.argtypes = [vp]
         L.vef_band_solve.argtypes = [vp, vp, vp, vp]
         L.vef_band_last_error.argtypes = [vp]
         L.vef_band_last_error.restype = ctypes.c_char_p
@@ -664,7 +666,9 @@ class BandLU:
     solves A11 z = a12 and keeps the Schur complement s = a22 - a21.z.  A
     solve is x2 = (b2 - a21.A11^-1 b1)/s, x1 = A11^-1 b1 - z x2.  The first
     solve after a factorisation is checked as in RfLU (host fallback with
-    threshold pivoting)."""
+    threshold pivoting).  precision="f32" stores and factorises the band in
+    FP32 (sgemm panels and updates) with FP64 solve vectors and border: the
+    single-precision factorisation of an inexact Newton iteration."""
 
     NB = 64
 
@@ -674,12 +678,12 @@ class BandLU:
         import torch
         from scipy.sparse.csgraph import reverse_cuthill_mckee
 
-        if precision != "f64":
-            raise RfLUError("BandLU factorises in FP64 only")
+        if precision not in ("f64", "f32"):
+            raise RfLUError("BandLU precision must be 'f64' or 'f32'")
         self.torch = torch
         self.device = device
         self.precision = precision
-        self.check_tol = 1e-10 if check_tol is None else check_tol
+        self.check_tol = (1e-10 if precision == "f64" else 1e-3) if check_tol is None else check_tol
         self.rowptr_h = np.asarray(rowptr, dtype=np.int64)
         self.col_h = np.asarray(col, dtype=np.int32)
         n = self.n = len(rowptr) - 1
@@ -699,10 +703,11 @@ class BandLU:
         self.w = int(np.abs(pi - pj).max()) if len(pi) else 0
         L = _band_lib()
         self.h = ctypes.c_void_p()
-        if L.vef_band_create(n1, self.w, self.NB, ctypes.byref(self.h)) != 0:
-            raise RfLUError("vef_band_create failed (device memory?)")
-        ab, ld, K = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_int32()
-        L.vef_band_storage(self.h, ctypes.byref(ab), ctypes.byref(ld), ctypes.byref(K))
+        bits = 64 if precision == "f64" else 32
+        if L.vef_band_create_prec(n1, self.w, self.NB, bits, ctypes.byref(self.h)) != 0:
+            raise RfLUError("vef_band_create_prec failed (device memory?)")
+        ld, K = ctypes.c_int64(), ctypes.c_int32()
+        self._err(L.vef_band_storage(self.h, None, ctypes.byref(ld), ctypes.byref(K)), "vef_band_storage")
         self.ld, self.K = ld.value, K.value
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int64)).to(device)  # noqa: E731
         self.src_in = dev(np.nonzero(inner)[0])
@@ -741,6 +746,11 @@ class BandLU:
     def _err(self, rc, what):
         if rc != 0:
             raise RfLUError(f"{what}: {_band_lib().vef_band_last_error(self.h).decode()}")
+
+    @property
+    def uses_graph(self) -> bool:
+        """True once the factorisation replays its captured HIP graph."""
+        return bool(_band_lib().vef_band_uses_graph(self.h))
 
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)

@@ -136,8 +136,8 @@ class NewtonSolver:
         self.asm = Assembler(mesh, params)
         rowptr, col = self.asm.pattern()
         classes = {"rf": RfLU, "block": BlockLU, "hybrid": HybridLU, "band": BandLU}
-        if factorization is None:  # the fastest that supports the precision
-            factorization = "band" if linear_precision == "f64" else "hybrid"
+        if factorization is None:  # the fastest for both precisions
+            factorization = "band"
         if factorization not in classes:
             raise NewtonError("factorization must be 'rf' (scalar re-factorisation), 'block' (6x6 node blocks), "
                               "'hybrid' (scalar re-factorisation in node-block order, block solves) or 'band' "
