@@ -132,11 +132,12 @@ def newton_bench(args):
     x0 = initial_state(mesh, LN0)
     edges = boundary_edges(mesh, "Cylinder")
 
-    def run(factorization="band", reuse=False, learned=False, precision="f64", jacobian="fd64"):
+    def run(factorization="band", reuse=False, learned=False, precision="f64", jacobian="fd64", overlap=False):
         solver = NewtonSolver(mesh, Params(**pset), factorization=factorization, reuse_factors=reuse,
                               linear_precision=precision, jacobian=jacobian)
         cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN,
-                                     predictor=LearnedPredictor(mesh, seed=0, epochs=50) if learned else None)
+                                     predictor=LearnedPredictor(mesh, seed=0, epochs=50, overlap=overlap)
+                                     if learned else None)
         cont.start(x0)
         times, flags, EU = [], [], []
         for _ in range(args.arc_steps):
@@ -174,6 +175,8 @@ def newton_bench(args):
         "inexact_newton_fp32_jacobian_fp32_factorization": run(reuse=True, precision="f32", jacobian="ad32"),
         "inexact_newton_learned_fp32_jacobian_fp32_factorization": run(
             reuse=True, learned=True, precision="f32", jacobian="ad32"),
+        "inexact_newton_learned_overlapped_training_fp32_jacobian_fp32_factorization": run(
+            reuse=True, learned=True, precision="f32", jacobian="ad32", overlap=True),
         "inexact_newton_learned_fp32_jacobian_fp32_factorization_hybrid_lu": run(
             factorization="hybrid", reuse=True, learned=True, precision="f32", jacobian="ad32"),
     }

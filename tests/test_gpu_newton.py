@@ -280,6 +280,23 @@ def test_learned_predictor_gains_over_tangent():
     assert steps_l < steps_t and iters_l < iters_t
 
 
+def test_learned_predictor_overlapped_training():
+    """Training on a second stream during the next Newton solve
+    (LearnedPredictor(overlap=True)): every step converges, the network is
+    trained once per observation after the first proposal, and its
+    proposals are taken."""
+    from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    print("[gpu newton] continuation to Wi 0.145, learned predictor trained on a second stream", flush=True)
+    pred = LearnedPredictor(mesh, seed=0, overlap=True)
+    steps_l, iters_l, cont = _continue_to(mesh, 0.145, pred)
+    print(f"[gpu newton] overlapped: {steps_l} steps, {iters_l} iterations; choices {cont.choices}", flush=True)
+    assert "learned" in cont.choices
+    assert pred._rounds >= len(pred.losses) >= pred._rounds - 1 >= 1
+    assert all(np.isfinite(pred.losses))
+
+
 def test_band_lu_workload_mesh_and_partial_last_block():
     """BandLU on the 1,600-element Newton matrix (39,792 node unknowns: the
     last 64-column block is partial) against scipy, and a Newton solve with it

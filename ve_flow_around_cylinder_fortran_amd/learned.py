@@ -127,6 +127,8 @@ class NodeMLP:
 
     def close(self):
         if getattr(self, "_h", None):
+            # work on any stream may still use the handle's buffers
+            self.torch.cuda.synchronize(self.device)
             self.lib.vef_mlp_destroy(self._h)
             self._h = None
 
@@ -150,10 +152,17 @@ class LearnedPredictor:
     x_{k+1} - x_t.  Every converged step adds the error its tangent
     prediction made to a short buffer and trains the network on it; the
     proposal is x_t plus the predicted error with its component along t
-    removed, so the pseudo-arclength row t.(x - x_k) = dS still holds."""
+    removed, so the pseudo-arclength row t.(x - x_k) = dS still holds.
+
+    overlap=True trains on a second stream (default priority) while the next
+    step's Newton solve runs on the main one: the training round for an
+    observation is launched after the proposal that follows it, so each
+    proposal uses the network trained up to the observation before (the
+    handle is never used by both streams at once: training waits for the
+    proposal, the next proposal waits for the training)."""
 
     def __init__(self, mesh: Mesh, seed: int = 0, epochs: int = 200, lr: float = 2e-3, buffer: int = 4,
-                 device: str = "cuda:0"):
+                 device: str = "cuda:0", overlap: bool = False):
         import torch
 
         self.torch = torch
@@ -168,6 +177,12 @@ class LearnedPredictor:
         self.in_scales = None   # state (6), increment (6), dEU, dS
         self.err_scale = None   # per field
         self.losses = []
+        self.overlap = overlap
+        self._side = torch.cuda.Stream(device=device) if overlap else None
+        self._trained = None    # event of the training round in flight (overlap)
+        self._loss = None
+        self._rounds = 0        # training rounds launched
+        self._untrained = False  # data added since the last round (overlap)
 
     @staticmethod
     def _nodal(x):
@@ -195,19 +210,54 @@ class LearnedPredictor:
             self.err_scale = e.abs().max(dim=0).values.clamp_min(1e-12)
         self.data.append((X, (e / self.err_scale).contiguous()))
         self.data = self.data[-self.buffer:]
+        if self.overlap:
+            self._untrained = True
+            return
         Xa = torch.cat([d[0] for d in self.data]).contiguous()
         Ta = torch.cat([d[1] for d in self.data]).contiguous()
         loss = None
         for _ in range(self.epochs):
             loss = self.net.adam_step(Xa, Ta, self.lr)
+        self._rounds += 1
         self.losses.append(float(loss.item()))
 
     def ready(self) -> bool:
-        return len(self.data) >= 1
+        return self._rounds >= 1
+
+    def _wait_training(self):
+        if self._trained is not None:
+            self.torch.cuda.current_stream(self.device).wait_event(self._trained)
+            self.losses.append(float(self._loss.item()))
+            self._trained = None
+
+    def train_pending(self):
+        """overlap=True: launches the training round for the observations not yet
+        trained on, on the second stream, after the main stream's work so far."""
+        if not (self.overlap and self._untrained):
+            return
+        torch = self.torch
+        self._wait_training()
+        main = torch.cuda.current_stream(self.device)
+        self._side.wait_stream(main)
+        with torch.cuda.stream(self._side):
+            for X, T in self.data:
+                X.record_stream(self._side)
+                T.record_stream(self._side)
+            Xa = torch.cat([d[0] for d in self.data]).contiguous()
+            Ta = torch.cat([d[1] for d in self.data]).contiguous()
+            loss = None
+            for _ in range(self.epochs):
+                loss = self.net.adam_step(Xa, Ta, self.lr)
+            self._loss = loss
+            self._trained = torch.cuda.Event()
+            self._trained.record(self._side)
+        self._rounds += 1
+        self._untrained = False
 
     def propose(self, X, x_t, tangent):
         """x_t plus the predicted error, projected orthogonally to the tangent."""
         torch = self.torch
+        self._wait_training()
         c = torch.zeros_like(x_t)
         c[:-1] = (self.net.forward(X).double() * self.err_scale.double()).reshape(-1)
         c = c - torch.dot(c, tangent) * tangent

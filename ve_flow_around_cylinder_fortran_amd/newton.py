@@ -306,5 +306,6 @@ class ArclengthContinuation:
                 if r_l < r_t:
                     self.x = cand
                     choice = "learned"
+            self.predictor.train_pending()
         self.choices.append(choice)
         return x, hist
