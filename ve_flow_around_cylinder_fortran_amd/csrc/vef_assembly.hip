@@ -856,9 +856,14 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
 // 9 Gauss points x 23 directions (the fields and EU) evaluate the Gauss-point
 // coefficients T(F) of the test functions (vef_ad_math.h) with one-component
 // dual numbers, giving D = WET * dT/dF in LDS; (3) every matrix entry is the
-// contraction sum_g sum_k phi^n_k(g) sum_{f in p} D[g][f][k,q] beta^m_f(g)
-// (FP32 accumulation), stored through the same scatter map as the
-// forward-difference pass; the EU direction gives dR/dEU into acbank.
+// contraction sum_g sum_k phi^n_k(g) E_g[k][q,m,p] with
+// E_g[k][q,m,p] = sum_{f in p} D[g][f][k,q] beta^m_f(g): per Gauss point E_g is
+// formed in LDS (VALU), then the outer sum runs on the matrix cores as
+// J += Phi_g^T E_g, a 16 x 4 by 4 x 336 product (test nodes padded to 16, the
+// three basis kinds and a zero row, the 324 entries of a row block and the 6
+// of dR/dEU padded to 336) in 21 v_mfma_f32_16x16x4f32 tiles whose FP32
+// accumulators stay in registers over the 9 Gauss points; stored through the
+// same scatter map as the forward-difference pass, dR/dEU into acbank.
 // ----------------------------------------------------------------------------
 // Field index of basis kind j (0 value, 1 d/dx, 2 d/dy, 3 d2/dx2, 4 d2/dy2) of
 // unknown p (-1 where the unknown has no such field).
@@ -872,13 +877,11 @@ __host__ __device__ constexpr int field_of(int p, int j) {
                 : (j == 0 ? vefad::kS11 : j == 1 ? vefad::kS11x : vefad::kS11y);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
                  int64_t eu_index, int32_t n_el, vefad::Params<float> prm, FdOut fo) {
 #pragma clang fp contract(fast)  // FP32 path: compared to FP32 accuracy, FMAs allowed
   constexpr int NF = vefad::kNumFields;
-  constexpr int NOUT = 2916 + 54;          // matrix entries and the dR/dEU column
-  constexpr int PER = (NOUT + 255) / 256;  // entries per thread
   __shared__ double sU[54];
   __shared__ double sX[18];
   __shared__ Geo sG[9];
@@ -886,7 +889,9 @@ vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ 
   __shared__ float sWET[9], sH2[9];
   __shared__ float sF[9][NF];
   __shared__ float sD[9][NF][18];  // WET * dT[k][q]/dF_f
-  __shared__ float sM[18 * 54];    // one Gauss point's D.B: [k*6+q][m*6+p]
+  constexpr int ECOL = 336;        // columns of E: 324 entries (q*54 + m*6 + p), 6 of dR/dEU, padding
+  constexpr int NTILE = ECOL / 16;
+  __shared__ float sM[3 * ECOL];   // one Gauss point's E: [k][column]
   __shared__ int32_t sCode[81];
   __shared__ int64_t sRb[9];
   __shared__ int32_t sRl[9];
@@ -953,50 +958,66 @@ vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 6; q++) sD[g][f][k * 6 + q] = w * T[k][q].d;
   }
-  // (3) J = sum_g Phi_g^T (D_g B_g): per Gauss point, M = D.B into LDS, then
-  //     each thread adds Phi^T M to its PER entries (FP32 accumulation)
-  float acc[PER];
+  // (3) J = sum_g Phi_g^T E_g on the matrix cores.  v_mfma_f32_16x16x4f32
+  //     lane layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j]
+  //     in register i % 4 of lane j + 16(i / 4).  Wavefront w owns the tiles
+  //     w, w + 4, ... (at most 6).
+  const int wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;  // A row / B column, and k
+  typedef float floatx4 __attribute__((ext_vector_type(4)));
+  floatx4 accj[6];
 #pragma unroll
-  for (int r = 0; r < PER; r++) acc[r] = 0.0f;
+  for (int t = 0; t < 6; t++) accj[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int g = 0; g < 9; g++) {
-    __syncthreads();  // D is complete / the previous M has been read
-    for (int t = tid; t < 18 * 54; t += 256) {
-      const int kq = t / 54, m = (t % 54) / 6, p = t % 6;
+    __syncthreads();  // D is complete / the previous E has been read
+    for (int t = tid; t < 3 * ECOL; t += 256) {
+      const int k = t / ECOL, col = t % ECOL;
       float v = 0.0f;
+      if (col < 324) {
+        const int q = col / 54, m = (col % 54) / 6, p = col % 6;
 #pragma unroll
-      for (int j = 0; j < 5; j++) {
-        const int f = field_of(p, j);
-        if (f >= 0) v += sD[g][f][kq] * sBas[g][j][m];
+        for (int j = 0; j < 5; j++) {
+          const int f = field_of(p, j);
+          if (f >= 0) v += sD[g][f][k * 6 + q] * sBas[g][j][m];
+        }
+      } else if (col < 330) {
+        v = sD[g][vefad::kEU][k * 6 + (col - 324)];
       }
       sM[t] = v;
     }
     __syncthreads();
+    const float a = (li < 9 && lk < 3) ? sBas[g][lk][li] : 0.0f;  // Phi_g^T: node li, basis kind lk
 #pragma unroll
-    for (int r = 0; r < PER; r++) {
-      const int o = tid + 256 * r;
-      if (o < 2916) {
-        const int n = o / 324, q = (o / 54) % 6, mp = o % 54;
-        acc[r] += sBas[g][0][n] * sM[q * 54 + mp] + sBas[g][1][n] * sM[(6 + q) * 54 + mp] +
-                  sBas[g][2][n] * sM[(12 + q) * 54 + mp];
-      } else if (o < NOUT) {
-        const int n = (o - 2916) / 6, q = (o - 2916) % 6;
-        const float* d = sD[g][vefad::kEU];
-        acc[r] += sBas[g][0][n] * d[q] + sBas[g][1][n] * d[6 + q] + sBas[g][2][n] * d[12 + q];
+    for (int t = 0; t < 6; t++) {
+      const int tile = wave + 4 * t;
+      if (tile < NTILE) {
+        const float b = lk < 3 ? sM[lk * ECOL + tile * 16 + li] : 0.0f;
+        accj[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accj[t], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < PER; r++) {
-    const int o = tid + 256 * r;
-    if (o < 2916) {
-      const int n = o / 324, q = (o / 54) % 6, m = (o % 54) / 6, p = o % 6;  // row (n,q), column (m,p)
-      const int32_t cd = sCode[n * 9 + m];
-      if ((cd & 1) == 0)
-        fo.values[sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 1) * 6 + p] = 0.0 + (double)acc[r];
-      else
-        fo.cbank[(int64_t)(cd >> 1) * 36 + q * 6 + p] = (double)acc[r];
-    } else if (o < NOUT) {
-      fo.acbank[e * 54 + (o - 2916)] = (double)acc[r];
+  for (int t = 0; t < 6; t++) {
+    const int tile = wave + 4 * t;
+    if (tile < NTILE) {
+      const int col = tile * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int n = 4 * lk + r;  // test node
+        if (n < 9 && col < 330) {
+          const float v = accj[t][r];
+          if (col < 324) {
+            const int q = col / 54, m = (col % 54) / 6, p = col % 6;  // row (n,q), column (m,p)
+            const int32_t cd = sCode[n * 9 + m];
+            if ((cd & 1) == 0)
+              fo.values[sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 1) * 6 + p] = 0.0 + (double)v;
+            else
+              fo.cbank[(int64_t)(cd >> 1) * 36 + q * 6 + p] = (double)v;
+          } else {
+            fo.acbank[e * 54 + n * 6 + (col - 324)] = (double)v;
+          }
+        }
+      }
     }
   }
 }
