@@ -125,6 +125,15 @@ def test_band_factor_graph_replays_match_direct_launches(precision):
         mats.append((b.clone(), v.clone()))
     lu = BandLU(rowptr, col, precision=precision)
     print(f"[gpu newton] band {precision} graph replays", flush=True)
+    import ctypes
+
+    from ve_flow_around_cylinder_fortran_amd.linsolve import _band_lib
+
+    ab, ld, K = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_int32()
+    rc = _band_lib().vef_band_storage(lu.h, ctypes.byref(ab), ctypes.byref(ld), ctypes.byref(K))
+    assert (rc == 0) == (precision == "f64")  # an FP32 band is not double: ab must be NULL
+    assert _band_lib().vef_band_storage(lu.h, None, ctypes.byref(ld), ctypes.byref(K)) == 0
+    assert (ld.value, K.value) == (lu.ld, lu.K)
     ref = {}
     for k in range(5):
         m = 1 if k == 2 else 0  # the other matrix in between

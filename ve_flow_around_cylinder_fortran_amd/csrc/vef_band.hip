@@ -203,7 +203,8 @@ band_panel_store(T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, 
 
 // The block inverse (column-major NB x NB) and the block's right-hand side go
 // to LDS with all loads in flight; v = M r (padding rows of r are zero), summed
-// in FP64.
+// in FP64 by four adjacent threads per row (16 terms each, two chains) and a
+// shuffle reduction, so the dependent chain is 8 deep instead of 64.
 template <class T>
 __device__ __forceinline__ void block_gemv(const T* __restrict__ M, const double* __restrict__ rhs, int kb,
                                            T (*sM)[NB + 1], double* sr, double* out) {
@@ -212,13 +213,32 @@ __device__ __forceinline__ void block_gemv(const T* __restrict__ M, const double
   stage_block(M, sM);
   if (t < NB) sr[t] = rv;
   __syncthreads();
-  if (t < NB) {
-    double s = 0.0;
+  const int row = t >> 2, part = t & 3;
+  double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int j = 0; j < NB; j++) s += (double)sM[t][j] * sr[j];
-    out[t] = s;
+  for (int i = 0; i < NB / 4; i += 2) {
+    s0 += (double)sM[row][part + 4 * i] * sr[part + 4 * i];
+    s1 += (double)sM[row][part + 4 * i + 4] * sr[part + 4 * i + 4];
   }
+  double s = s0 + s1;
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  if (part == 0) out[row] = s;
   __syncthreads();
+}
+
+// sum_j a[j] y[j] over NB terms in four interleaved chains.
+template <class T>
+__device__ __forceinline__ double row_dot(const T (&a)[NB], const double* y) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+  for (int j = 0; j < NB; j += 4) {
+    s0 += (double)a[j] * y[j];
+    s1 += (double)a[j + 1] * y[j + 1];
+    s2 += (double)a[j + 2] * y[j + 2];
+    s3 += (double)a[j + 3] * y[j + 3];
+  }
+  return (s0 + s1) + (s2 + s3);
 }
 
 // Forward sweep, block k: y_k = L11^-1 b_k; b[k0+kb+r] -= L21(r, :) y_k.
@@ -241,12 +261,7 @@ band_fwd(const T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, in
   for (int j = 0; j < NB; j++) av[j] = a[j * lda];
   block_gemv(invLk, b + k0, kb, sM, sr, yk);
   if (blockIdx.x == 0 && t < kb) y[k0 + t] = yk[t];
-  if (r < m2) {
-    double s = b[k0 + kb + r];
-#pragma unroll
-    for (int j = 0; j < NB; j++) s -= (double)av[j] * yk[j];  // yk[j] = 0 and av[j] = 0 past kb
-    b[k0 + kb + r] = s;
-  }
+  if (r < m2) b[k0 + kb + r] -= row_dot(av, yk);  // yk[j] = 0 and av[j] = 0 past kb
 }
 
 // Backward sweep, block k: x_k = U11^-1 y_k; y[k0-n2+r] -= U(k0-n2+r, k0+j) x_k.
@@ -266,12 +281,7 @@ band_bwd(const T* __restrict__ AB, int64_t ld, int32_t K, int64_t k0, int kb, in
   for (int j = 0; j < NB; j++) av[j] = a[j * lda];
   block_gemv(invUk, y + k0, kb, sM, sr, xk);
   if (blockIdx.x == 0 && t < kb) x[k0 + t] = xk[t];
-  if (r < n2) {
-    double s = y[row];
-#pragma unroll
-    for (int j = 0; j < NB; j++) s -= (double)av[j] * xk[j];  // xk[j] = 0 and av[j] = 0 past kb
-    y[row] = s;
-  }
+  if (r < n2) y[row] -= row_dot(av, xk);  // xk[j] = 0 and av[j] = 0 past kb
 }
 
 template <class T>
