@@ -125,6 +125,7 @@ def newton_bench(args):
     from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
     from ve_flow_around_cylinder_fortran_amd.mesh import boundary_edges
     from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation, NewtonSolver, initial_state
+    from ve_flow_around_cylinder_fortran_amd.post import peak_wall_stress
 
     pset = dict(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1.0e-3, WiN0=1.0, model="L-PTT")
     LN0, dArN = 0.1, 0.02
@@ -150,17 +151,20 @@ def newton_bench(args):
             EU.append(float(x[-1]))
         solver.asm.set_drag_edges(edges)
         Fd = float(solver.asm.drag(x).item())
+        Pw = peak_wall_stress(mesh, solver.asm.nodal_stress(x, pset["WiN0"] * float(x[-1])).cpu().numpy())
         # FP64 Newton (reference policy) at the same Wi, started from this state
         ref = NewtonSolver(mesh, Params(**pset))
         xr, _ = ref.solve(x, float(x[-1]))
         ref.asm.set_drag_edges(edges)
         Fr = float(ref.asm.drag(xr).item())
+        Pr = peak_wall_stress(mesh, ref.asm.nodal_stress(xr, pset["WiN0"] * float(xr[-1])).cpu().numpy())
         arc = times[2:]
         return {"s_per_step": float(np.mean(arc)), "step_seconds": times, "newton_flags_per_step": flags,
                 "EU_per_step": EU, "Wi_gain_per_second": (EU[-1] - EU[1]) * pset["WiN0"] / sum(arc),
                 "factorizations": solver.lu.refactorizations, "host_fallbacks": solver.lu.host_fallbacks,
                 "solver": type(solver.lu).__name__,
                 "Fd_last_step": Fd, "drag_rel_diff_vs_fp64_newton": abs(Fd - Fr) / abs(Fr),
+                "peak_wall_Txx_last_step": Pw, "peak_wall_Txx_rel_diff_vs_fp64_newton": abs(Pw - Pr) / abs(Pr),
                 "choices": cont.choices if learned else None, "factor_pattern_nnz":
                     getattr(solver.lu, "nnzS", None) or getattr(solver.lu, "nnzT", None)}
 
