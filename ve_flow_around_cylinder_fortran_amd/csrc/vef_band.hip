@@ -74,12 +74,14 @@ __device__ __forceinline__ float recip(float v) {
 template <int K, class T>
 __device__ __forceinline__ void lu_step(DiagRows<T>& R) {
   constexpr int b = K & 1, kq = K >> 2, kr = K & 3;
+  // D[i][K] is final after step K-1: fetched before the barrier so the
+  // shuffle's latency overlaps the wait
+  const T dik = __shfl(R.d[kq], R.row0 | kr);
   if (R.i == K) {
 #pragma unroll
     for (int q = 0; q < RC; q++) R.P[b][R.g + 4 * q] = R.d[q];
   }
   __syncthreads();
-  const T dik = __shfl(R.d[kq], R.row0 | kr);  // D[i][K]
   if (R.i > K) {
     const T l = dik * recip(R.P[b][K]);
 #pragma unroll
@@ -98,6 +100,7 @@ template <int K, class T>
 __device__ __forceinline__ void inv_step(DiagRows<T>& R) {
   constexpr int b = K & 1, KU = NB - 1 - K;
   const T uik = __shfl(R.d[KU >> 2], R.row0 | (KU & 3));  // U[i][KU]
+  const T lik = __shfl(R.d[K >> 2], R.row0 | (K & 3));    // L[i][K] (D is final here)
   if (R.i == KU) {
     const T inv = recip(uik);
 #pragma unroll
@@ -111,7 +114,6 @@ __device__ __forceinline__ void inv_step(DiagRows<T>& R) {
     for (int q = 0; q < RC; q++) R.P[b][R.g + 4 * q] = R.x[q];
   }
   __syncthreads();
-  const T lik = __shfl(R.d[K >> 2], R.row0 | (K & 3));  // L[i][K]
   if (R.i > K) {
 #pragma unroll
     for (int q = 0; q < RC; q++) {
