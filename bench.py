@@ -81,6 +81,11 @@ def parse():
                     help="measure seconds per converged arclength step (fixed-pattern GPU Newton + continuation on "
                          "the 1,600-element mesh) instead of the assembly throughput")
     ap.add_argument("--arc-steps", type=int, default=10, help="continuation steps for --newton (2 pinned + the rest)")
+    ap.add_argument("--newton-model", default="L-PTT", choices=["L-PTT", "GIESEKUS"],
+                    help="constitutive model of --newton (L-PTT with --newton-ven 0 is Oldroyd-B)")
+    ap.add_argument("--newton-ven", type=float, default=0.0,
+                    help="L-PTT epsilon or Giesekus alpha of --newton; other than Oldroyd-B, the inlet values come "
+                         "from the fully developed inflow problem at each Wi")
     ap.add_argument("--stability", action="store_true",
                     help="linear stability assembly (A and B) of the 40,000-element mesh instead of the Newton "
                          "assembly")
@@ -127,7 +132,10 @@ def newton_bench(args):
     from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation, NewtonSolver, initial_state
     from ve_flow_around_cylinder_fortran_amd.post import peak_wall_stress
 
-    pset = dict(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1.0e-3, WiN0=1.0, model="L-PTT")
+    pset = dict(ReN=0.0, BvN=0.59, VeN=args.newton_ven, DfN=1.0e-3, WiN0=1.0, model=args.newton_model)
+    oldroyd_b = pset["VeN"] == 0.0 and pset["model"] == "L-PTT"
+    model_name = "Oldroyd-B" if oldroyd_b else (f"linear PTT eps={pset['VeN']}" if pset["model"] == "L-PTT"
+                                                else f"Giesekus alpha={pset['VeN']}")
     LN0, dArN = 0.1, 0.02
     mesh = workload_mesh(1600)
     x0 = initial_state(mesh, LN0)
@@ -201,7 +209,8 @@ def newton_bench(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "cylinder-in-channel Q2 mesh, 1600 elements, Oldroyd-B beta=0.59 Re=0 DfN=1e-3, arclength "
+            "workload": f"cylinder-in-channel Q2 mesh, 1600 elements, {model_name} beta=0.59 Re=0 DfN=1e-3, "
+                        + ("" if oldroyd_b else "fully developed inflow solved at each Wi, ") + "arclength "
                         f"continuation in Wi from {LN0} (pinned steps {LN0}, {LN0 + dArN}), Newton tolerance 1e-8 on "
                         "||b|| and ||s||; reference Newton policy, FP64, banded LU in reverse Cuthill-McKee node "
                         "order (libvef_band.so)",
@@ -218,7 +227,10 @@ def newton_bench(args):
         threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1), 16))
         n_cpu = min(args.arc_steps, 6)
         cpu_times = []
-        states, _ = continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, n_cpu, nthreads=threads, times=cpu_times)
+        from ve_flow_around_cylinder_fortran_amd.inflow import FullyDevelopedInflow
+
+        states, _ = continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, n_cpu, nthreads=threads, times=cpu_times,
+                                     inflow=None if oldroyd_b else FullyDevelopedInflow.from_mesh(mesh, Params(**pset)))
         cpu_arc = float(np.mean(cpu_times[2:]))
         out["cpu_baseline"] = {"value": cpu_arc, "unit": "s", "cores": threads, "kind": "port",
                                "sample": f"the same continuation, steps 3..{n_cpu}: oracle assembly on {threads} "

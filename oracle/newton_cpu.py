@@ -30,11 +30,11 @@ def apply_dirichlet(rows, vals, x, b, values, rowptr, col):
             values[s:e] = np.where(col[s:e] == r, 1.0, 0.0)
 
 
-def newton_cpu(oracle, mesh, pset, x0, LN0, max_iter=NITER, nthreads=8, xF=1.0):
+def newton_cpu(oracle, mesh, pset, x0, LN0, max_iter=NITER, nthreads=8, xF=1.0, inflow=None):
     prm = oracle.params(**pset)
     rowptr, col = oracle.pattern(mesh)
     N = mesh.n_unknowns
-    rows, vals = dirichlet_rows(*cylinder_dirichlet(mesh, pset["WiN0"] * LN0), N)
+    rows, vals = dirichlet_rows(*cylinder_dirichlet(mesh, pset["WiN0"] * LN0, inflow), N)
     x = np.array(x0, dtype=np.float64)
     flag, res_norm, cor_new, cor_old, it = "NRP", 1.0, 1.0, 1.0, 0
     lu = None
@@ -66,7 +66,7 @@ def newton_cpu(oracle, mesh, pset, x0, LN0, max_iter=NITER, nthreads=8, xF=1.0):
     return x, hist
 
 
-def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent, dS, nthreads=8):
+def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent, dS, nthreads=8, inflow=None):
     """One Newton solve of a continuation step (pinned for increments 1-2,
     pseudo-arclength after); the inflow values follow the current EU.
     Returns (x, history, last SuperLU factorisation)."""
@@ -85,7 +85,7 @@ def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent,
         if it % 10 == 0:
             flag = "NRP"
         jac = flag == "NRP"
-        rows, vals = dirichlet_rows(*cylinder_dirichlet(mesh, pset["WiN0"] * x[-1]), N)
+        rows, vals = dirichlet_rows(*cylinder_dirichlet(mesh, pset["WiN0"] * x[-1], inflow), N)
         b, v = oracle.assemble(prm, mesh, x, rowptr=rowptr, col=col, jac=jac, nthreads=nthreads)
         vv = v if jac else np.zeros(len(col))
         oracle.arclength(mode, mesh.n_nodes, x, xo, t, increment, LN0, dArN, dS, rowptr, col, vv, b, jac=jac)
@@ -109,7 +109,7 @@ def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent,
 
 
 def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=50.0, eps_pr=1.0e-3, nthreads=8,
-                     times=None):
+                     times=None, inflow=None):
     """continuationStepArclength restated with newton_cpu_arc; returns the
     converged states and the step sizes used (and appends each step's wall
     time to `times` when given)."""
@@ -122,7 +122,8 @@ def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=5
         t0 = time.perf_counter()
         predicted = x.copy()
         dss.append(dS)
-        xc, hist, lu = newton_cpu_arc(oracle, mesh, pset, x, LN0, k, dArN, x_old, tangent, dS, nthreads=nthreads)
+        xc, hist, lu = newton_cpu_arc(oracle, mesh, pset, x, LN0, k, dArN, x_old, tangent, dS, nthreads=nthreads,
+                                      inflow=inflow)
         if k > 1:
             if k == 2:
                 d = xc - x_old

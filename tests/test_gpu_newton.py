@@ -175,6 +175,41 @@ def test_newton_matches_cpu_iteration(oracle):
     assert abs(Fd - oFd) <= 1e-8 * abs(oFd)
 
 
+@pytest.mark.parametrize("model,VeN", [("L-PTT", 0.25), ("GIESEKUS", 0.1)])
+def test_newton_other_models_match_cpu_iteration(oracle, model, VeN):
+    """Linear PTT and Giesekus, whose inlet values come from the fully
+    developed inflow problem (inflow.FullyDevelopedInflow, solved at the
+    state's Wi): the GPU Newton solve takes the CPU restatement's iteration
+    flags (the CPU side with its own inflow solver), the states agree within
+    the Newton tolerance, and so does the drag."""
+    from newton_cpu import newton_cpu
+
+    from ve_flow_around_cylinder_fortran_amd.inflow import FullyDevelopedInflow
+
+    pset = dict(OLDROYD_B, VeN=VeN, model=model)
+    mesh = cylinder_channel(8, 4, 6, 8)
+    LN0 = 0.1  # from rest, as the continuation starts
+    x0 = initial_state(mesh, LN0)
+    print(f"[gpu newton] {model} full solve, small mesh", flush=True)
+    solver = NewtonSolver(mesh, Params(**pset))
+    assert solver.inflow is not None
+    xg, hist = solver.solve(x0, LN0)
+    xc, hc = newton_cpu(oracle, mesh, pset, x0, LN0, inflow=FullyDevelopedInflow.from_mesh(mesh, Params(**pset)))
+    assert [h.flag for h in hist] == [h[1] for h in hc]
+    xg = xg.cpu().numpy()
+    assert np.abs(xg - xc).max() <= 1e-8 * np.abs(xc).max()
+    assert hist[-1].cor_norm < 1e-8 and hist[-1].res_norm < 1e-8
+    # the inlet rows hold the inflow solution
+    (U, _, Sxx, Sxy, Syy), _ = solver.inflow.solve(LN0)
+    nodes = solver.inflow.nodes
+    assert np.abs(xg[6 * nodes] - U).max() < 1e-12 and np.abs(xg[6 * nodes + 4] - Sxy).max() < 1e-12
+    edges = boundary_edges(mesh, "Cylinder")
+    solver.asm.set_drag_edges(edges)
+    Fd = solver.asm.drag(torch.from_numpy(xg).cuda()).item()
+    oFd, _ = oracle.drag(oracle.params(**pset), mesh, xc, edges)
+    assert abs(Fd - oFd) <= 1e-8 * abs(oFd)
+
+
 @pytest.mark.parametrize("factorization", ["band", "hybrid"])
 def test_single_precision_factorisation_newton(factorization):
     """Inexact Newton: the FP64 Jacobian factorised in FP32 (BandLU: FP32 band

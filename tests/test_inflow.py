@@ -1,0 +1,132 @@
+"""The fully developed inflow (the reference's Inflow_1D problem,
+inflow.FullyDevelopedInflow) against solutions built independently of its
+square-root algebra:
+
+* Oldroyd-B: U = 1.5 (1 - (y/H)^2) and dPL = -3/H^2 (total viscosity 1), up to
+  the projection error of the stresses;
+* linear PTT and Giesekus: the exact channel profile from the textbook
+  conformation equation in simple shear, Wi (-L C - C L^T) + f2 C - f1 I +
+  f3 C^2 = 0 with L = [[0, g], [0, 0]] (linear PTT: C_yy = 1, C_xy = Wi g / f,
+  C_xx = 1 + 2 (Wi g / f)^2, f^3 - f^2 - 2 eps (Wi g)^2 = 0; Giesekus solved
+  numerically), the momentum balance tau_xy(g(y)) = dPL y and the flow rate
+  2H fixing dPL; S = C^(1/2).  The finite-element solution must approach it
+  at the nodes, and faster than linearly when the elements are halved.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import sqrtm
+from scipy.optimize import brentq, fsolve
+
+from ve_flow_around_cylinder_fortran_amd import Params
+from ve_flow_around_cylinder_fortran_amd.inflow import FullyDevelopedInflow, InflowError
+
+H = 2.0
+
+
+def conformation(g, Wi, model, VeN, x0=None):
+    """C (xx, xy, yy) of steady simple shear at rate g (textbook form); x0 a
+    starting point for the Giesekus solve."""
+    if model == "L-PTT":
+        x = (Wi * g) ** 2
+        f = brentq(lambda f: f ** 3 - f ** 2 - 2 * VeN * x, 1.0, 2.0 + 2 * VeN * x)
+        return np.array([1 + 2 * x / f ** 2, Wi * g / f, 1.0])
+    a = VeN
+    L = np.array([[0.0, g], [0.0, 0.0]])
+    I = np.eye(2)
+
+    def R(c):
+        C = np.array([[c[0], c[1]], [c[1], c[2]]])
+        M = Wi * (-L @ C - C @ L.T) + (1 - 2 * a) * C - (1 - a) * I + a * C @ C
+        return [M[0, 0], M[0, 1], M[1, 1]]
+
+    if x0 is None:
+        x0 = [1 + 2 * (Wi * g) ** 2, Wi * g, 1.0]
+    c = fsolve(R, x0, xtol=1e-13)
+    assert np.abs(R(c)).max() < 1e-11
+    return np.array(c)
+
+
+def exact_profile(y_nodes, BvN, Wi, model, VeN):
+    """u and S (xx, xy, yy) at y_nodes and dPL of the fully developed flow."""
+    gmax = 12.0
+    gg = np.linspace(0.0, gmax, 6001)
+    Cs = []
+    for g in gg:  # continuation along the rates
+        Cs.append(conformation(g, Wi, model, VeN, Cs[-1] if Cs else None))
+    Cs = np.array(Cs)
+    tau = (1 - BvN) / Wi * Cs[:, 1] + BvN * gg  # monotone shear stress
+    assert np.all(np.diff(tau) > 0)
+    yf = np.linspace(-H, H, 40001)
+
+    def rate(dPL, y):  # tau(g) = dPL y, odd in y
+        t = dPL * y
+        return np.sign(t) * np.interp(np.abs(t), tau, gg)
+
+    def u_of(dPL, y):
+        gy = rate(dPL, y)
+        u = np.concatenate([[0.0], np.cumsum((gy[1:] + gy[:-1]) / 2 * np.diff(y))])
+        return u
+
+    def flow(dPL):
+        u = u_of(dPL, yf)
+        return np.sum((u[1:] + u[:-1]) / 2 * np.diff(yf)) - 2 * H
+
+    dPL = brentq(flow, -0.999 * tau[-1] / H, -1e-6)
+    u = np.interp(y_nodes, yf, u_of(dPL, yf))
+    S = []
+    for g in rate(dPL, np.asarray(y_nodes)):
+        c = conformation(abs(g), Wi, model, VeN, Cs[np.searchsorted(gg, abs(g))])
+        C = np.array([[c[0], np.sign(g) * c[1]], [np.sign(g) * c[1], c[2]]])
+        s = np.real(sqrtm(C))
+        S.append([s[0, 0], s[0, 1], s[1, 1]])
+    return u, np.array(S), dPL
+
+
+def fe_solution(n_el, BvN, Wi, model, VeN):
+    y = np.linspace(-H, H, 2 * n_el + 1)
+    inf = FullyDevelopedInflow(y, H, Params(ReN=0.0, BvN=BvN, VeN=VeN, DfN=1e-3, WiN0=1.0, model=model))
+    (U, Uy, S00, S01, S11), dPL = inf.solve(Wi)
+    return y, U, Uy, np.stack([S00, S01, S11], axis=1), dPL
+
+
+def test_oldroyd_b_parabola():
+    y, U, Uy, S, dPL = fe_solution(8, 0.59, 0.5, "L-PTT", 0.0)
+    # the stresses are the Galerkin projection of the pointwise relation, so
+    # the velocity is the parabola only up to their projection error
+    assert np.abs(U - 1.5 * (1 - (y / H) ** 2)).max() < 5e-5
+    assert abs(dPL + 3.0 / H ** 2) < 1e-6
+    assert np.abs(Uy).max() == 0.0
+    from ve_flow_around_cylinder_fortran_amd.newton import oldroyd_b_inflow
+
+    _, _, sxx, sxy, syy = oldroyd_b_inflow(y, H, 0.5)
+    assert np.abs(S - np.stack([sxx, sxy, syy], axis=1)).max() < 2e-3
+
+
+@pytest.mark.parametrize("model,VeN", [("L-PTT", 0.25), ("GIESEKUS", 0.1)])
+def test_models_approach_the_exact_channel_profile(model, VeN):
+    BvN, Wi = 0.59, 0.6
+    errs = []
+    for n_el in (8, 16):
+        y, U, Uy, S, dPL = fe_solution(n_el, BvN, Wi, model, VeN)
+        ue, Se, dPLe = exact_profile(y, BvN, Wi, model, VeN)
+        e = max(np.abs(U - ue).max(), np.abs(S - Se).max(), abs(dPL - dPLe))
+        errs.append(e)
+        assert np.abs(Uy).max() == 0.0
+    print(f"[inflow] {model}: errors {errs[0]:.2e} (8 elements), {errs[1]:.2e} (16)")
+    assert errs[1] < 2e-3
+    assert errs[1] < errs[0] / 2.5  # quadratic elements: better than first order
+
+
+def test_shear_thinning_flattens_the_profile():
+    """Linear PTT at eps > 0 is shear thinning: the centre velocity drops below
+    the Oldroyd-B 1.5 and |dPL| below 3/H^2 (same flow rate)."""
+    y, U, _, _, dPL = fe_solution(8, 0.59, 0.6, "L-PTT", 0.25)
+    assert U[len(U) // 2] < 1.5
+    assert -dPL < 3.0 / H ** 2
+
+
+def test_bad_nodes_rejected():
+    with pytest.raises(InflowError):
+        FullyDevelopedInflow(np.array([0.0, 1.0]), H, Params())
+    with pytest.raises(InflowError):
+        FullyDevelopedInflow(np.array([0.0, 2.0, 1.0]), H, Params())

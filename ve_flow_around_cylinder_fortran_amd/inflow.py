@@ -1,0 +1,202 @@
+"""Fully developed inflow: the reference's Inflow_1D problem, whose solution
+gives the inlet's Dirichlet values for any constitutive model.
+
+The reference solves it inside every main-problem assembly of an arclength
+continuation (loopNewtonRaphsonInflow, src/Newton_Raphson_mod.F90:176-277,
+called at :369-372 with WiN = WiN0*EU of the main state,
+src/Element_Calculations_mod.F90:592-597) and inserts the result as Dirichlet
+rows of Ux, Uy, Sxx, Sxy, Syy at the inlet nodes
+(applyDirichletBCFullyDeveloped, src/Dirichlet_mod.F90:149-190).  The problem
+(src/Physical_mod.F90:31-52): quadratic line elements along y over the inlet
+nodes, unknowns Ux, Uy and the three square-root-conformation components per
+node plus the pressure gradient dPL, walls (Ux = Uy = 0) at both ends, and
+the flow rate 2*Height.  At its Gauss points every x-derivative, the Uy
+gradient, P and all stress gradients are zero
+(src/Element_Calculations_mod.F90:516-525), so the equations are
+    momentum x:  (dPL phi + P_tot(1,2) dphi/dy) WET        (momentumInflow,
+                 src/Residuals_mod.F90:494-540; ReN dUf/dt = 0 when steady)
+    Uy:          Uy phi WET
+    stresses:    Con_EQ(j,k) phi WET                        (stressesInflow,
+                 :544-595; the DfN term multiplies dT/dXi = 0)
+    flow rate:   sum of Ux WET = 2*Height                    (extraEquations-
+                 ResidualsBulkInflow, src/Extra_Equations_mod.F90:133-150)
+with Con_EQ and P_tot those of the square-root formulation in simple shear.
+
+Host code on the inlet's few hundred unknowns (a dense Newton iteration with
+a forward-difference Jacobian, as small as the reference's own PETSc solve of
+it); the main problem's assembly stays on the GPU.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .mesh import MESH_TOLERANCE, Mesh
+
+# 3-point Gauss rule with the centre abscissa shifted by 1e-8, as the
+# reference's line and quadrangle rules have it (Gauss_Quadrangle_mod.F90:69-72).
+_XI = np.array([0.0 + 1.0e-8, math.sqrt(3.0 / 5.0), -math.sqrt(3.0 / 5.0)])
+_W = np.array([8.0 / 9.0, 5.0 / 9.0, 5.0 / 9.0])
+# The reference stops at 1e-8 on both norms (Cor_Norm_eps); iterating on to
+# 1e-12 makes the values independent of the starting point (the previous
+# solution), so every caller at the same WiN gets the same Dirichlet values.
+INFLOW_TOL = 1.0e-12
+
+
+class InflowError(RuntimeError):
+    pass
+
+
+def simple_shear_terms(g, S00, S01, S11, WiN, BvN, VeN, model):
+    """Con_EQ (xx, xy, yy) and P_tot(1,2) of the square-root formulation for
+    U = (u(y), 0) with du/dy = g and spatially uniform S (strongFormConEQ_SQRT,
+    src/Constitutive_Models_SQRT_mod.F90:120-509, with GU(2,1) = g the only
+    velocity gradient and dS/dXi = 0).  Arrays broadcast."""
+    # C = S.S (the 2x2 block; C33 = 0 because S33 is never set)
+    C00 = S00 * S00 + S01 * S01
+    C01 = S00 * S01 + S01 * S11
+    C11 = S01 * S01 + S11 * S11
+    if model == 0:  # L-PTT: f(1) = f(2) = 1 + eps (tr C - 2), f(3) = 0
+        fT = 1.0 + VeN * ((C00 + C11) - 2.0)
+        f0, f1, f2 = fT, fT, 0.0
+    else:  # Giesekus
+        f0, f1, f2 = 1.0 - VeN, 1.0 - 2.0 * VeN, VeN
+    # omega(1,2) from S and the (traceless, here plain) transposed gradient
+    w = (S11 * g) / (S00 + S11)
+    # UCD = U.grad S (= 0) - S.GU_tr - W.S, GU_tr = [[0, 0], [g, 0]]
+    U00 = -(S01 * g) - w * S01
+    U01 = -(w * S11)
+    U11 = w * S01
+    det = S00 * S11 - S01 * S01
+    SI00, SI01, SI11 = S11 / det, -S01 / det, S00 / det
+    t00 = f1 * S00 - f0 * SI00
+    t01 = f1 * S01 - f0 * SI01
+    t11 = f1 * S11 - f0 * SI11
+    if model != 0:
+        t00 = t00 + f2 * (S00 * C00 + S01 * C01)
+        t01 = t01 + f2 * (S00 * C01 + S01 * C11)
+        t11 = t11 + f2 * (S01 * C01 + S11 * C11)
+    CE00 = WiN * U00 + t00 / 2.0
+    CE01 = WiN * U01 + t01 / 2.0
+    CE11 = WiN * U11 + t11 / 2.0
+    a = (1.0 - BvN) / WiN
+    P01 = a * C01 + BvN * g  # T_ve(1,2) + BvN*G_dot_tr(1,2)
+    return CE00, CE01, CE11, P01
+
+
+class FullyDevelopedInflow:
+    """The Inflow_1D problem on given node positions y (ascending; an odd
+    count, consecutive triples form the quadratic elements) of a channel of
+    half-width H, for Params (ReN is irrelevant: the flow is steady)."""
+
+    def __init__(self, y: np.ndarray, H: float, params):
+        y = np.asarray(y, dtype=np.float64)
+        if y.ndim != 1 or len(y) < 3 or len(y) % 2 == 0 or np.any(np.diff(y) <= 0):
+            raise InflowError("inflow nodes: an odd number >= 3 of ascending y positions")
+        self.y, self.H, self.params = y, float(H), params
+        self.model = {"L-PTT": 0, "GIESEKUS": 1}[params.model]
+        nn = len(y)
+        self.nn = nn
+        self.el = np.stack([np.arange(0, nn - 2, 2), np.arange(1, nn - 1, 2), np.arange(2, nn, 2)], axis=1)
+        xi = _XI[:, None]
+        phi = np.concatenate([xi * (xi - 1) / 2, 1 - xi * xi, xi * (xi + 1) / 2], axis=1)  # [g][3]
+        dphi = np.concatenate([xi - 0.5, -2 * xi, xi + 0.5], axis=1)
+        ye = y[self.el]                                   # [e][3]
+        J = dphi @ ye.T                                   # [g][e] dy/dxi
+        self.phi = phi                                    # [g][n]
+        self.dphi_dy = dphi[:, None, :] / J[:, :, None]   # [g][e][n]
+        self.WET = _W[:, None] * np.abs(J)                # [g][e]
+        self.walls = np.array([0, nn - 1])
+        self.z = None  # last solution (warm start)
+        self._lu = None  # factorised Jacobian, kept across calls
+        self.newton_iterations = []
+        self.jacobians = 0
+
+    @classmethod
+    def from_mesh(cls, mesh: Mesh, params):
+        """The inlet nodes of a cylinder mesh (x = -L_up), in ascending y."""
+        g = mesh.geometry
+        nodes = np.nonzero(np.abs(mesh.coords[:, 0] + g["L_up"]) < MESH_TOLERANCE)[0]
+        order = np.argsort(mesh.coords[nodes, 1])
+        obj = cls(mesh.coords[nodes[order], 1], g["H"], params)
+        obj.nodes = nodes[order]
+        return obj
+
+    def residual(self, z: np.ndarray, WiN: float) -> np.ndarray:
+        """Residual of the unknowns z = [node-major (Ux, Uy, Sxx, Sxy, Syy)..., dPL]
+        with the wall rows replaced by Ux and Uy themselves."""
+        p = self.params
+        nod = z[:-1].reshape(self.nn, 5)
+        dPL = z[-1]
+        v = nod[self.el]                                          # [e][3][5]
+        Fg = np.einsum("gn,enq->geq", self.phi, v)                # values at [g][e]
+        gU = np.einsum("gen,en->ge", self.dphi_dy, v[:, :, 0])    # du/dy
+        CE00, CE01, CE11, P01 = simple_shear_terms(gU, Fg[..., 2], Fg[..., 3], Fg[..., 4], WiN, p.BvN, p.VeN,
+                                                   self.model)
+        W = self.WET
+        R = np.zeros((self.nn, 5))
+        terms = np.stack([
+            dPL * self.phi[:, None, :] + P01[..., None] * self.dphi_dy,   # momentum x
+            Fg[..., 1:2] * self.phi[:, None, :],                           # Uy
+            CE00[..., None] * self.phi[:, None, :],
+            CE01[..., None] * self.phi[:, None, :],
+            CE11[..., None] * self.phi[:, None, :],
+        ], axis=-1) * W[..., None, None]                                   # [g][e][n][5]
+        np.add.at(R, self.el, terms.sum(axis=0))
+        R[self.walls, 0] = nod[self.walls, 0]
+        R[self.walls, 1] = nod[self.walls, 1]
+        flow = float(np.sum(Fg[..., 0] * W)) - 2.0 * self.H
+        return np.concatenate([R.ravel(), [flow]])
+
+    def initial_guess(self, WiN: float) -> np.ndarray:
+        """The Oldroyd-B closed form: U = 1.5 (1 - (y/H)^2), S = sqrt(C), dPL = -3/H^2."""
+        from .newton import oldroyd_b_inflow
+
+        U, Uy, Sxx, Sxy, Syy = oldroyd_b_inflow(self.y, self.H, WiN)
+        return np.concatenate([np.stack([U, Uy, Sxx, Sxy, Syy], axis=1).ravel(), [-3.0 / self.H ** 2]])
+
+    def _factor(self, z: np.ndarray, WiN: float, r: np.ndarray):
+        """LU of the forward-difference Jacobian at z (r = residual(z))."""
+        from scipy.linalg import lu_factor
+
+        n = len(z)
+        J = np.empty((n, n))
+        for j in range(n):
+            h = 1.0e-7 * max(1.0, abs(z[j]))
+            zp = z.copy()
+            zp[j] += h
+            J[:, j] = (self.residual(zp, WiN) - r) / h
+        self.jacobians += 1
+        return lu_factor(J)
+
+    def solve(self, WiN: float, max_iter: int = 100):
+        """Newton on the inflow problem at WiN from the last solution (or the
+        Oldroyd-B closed form) until both norms are below INFLOW_TOL; returns
+        (Ux, Uy, Sxx, Sxy, Syy) at the nodes and dPL.  The factorised Jacobian
+        is kept across calls (WiN moves little between them) and formed again
+        when a correction shrinks by less than a factor 0.3."""
+        from scipy.linalg import lu_solve
+
+        z = self.z.copy() if self.z is not None else self.initial_guess(WiN)
+        prev = None
+        for it in range(1, max_iter + 1):
+            r = self.residual(z, WiN)
+            if self._lu is None:
+                self._lu = self._factor(z, WiN, r)
+            dz = lu_solve(self._lu, r)
+            z = z - dz
+            nd = float(np.linalg.norm(dz))
+            if not np.all(np.isfinite(z)):
+                raise InflowError(f"inflow Newton diverged at WiN = {WiN}")
+            if nd < INFLOW_TOL and np.linalg.norm(r) < INFLOW_TOL:
+                break
+            if prev is not None and nd > 0.3 * prev:
+                self._lu = None
+            prev = nd
+        else:
+            raise InflowError(f"inflow Newton: no convergence in {max_iter} iterations at WiN = {WiN}")
+        self.z = z
+        self.newton_iterations.append(it)
+        nod = z[:-1].reshape(self.nn, 5)
+        return tuple(nod[:, q].copy() for q in range(5)), float(z[-1])

@@ -52,16 +52,23 @@ def oldroyd_b_inflow(y: np.ndarray, H: float, WiN: float):
     return U, np.zeros_like(y), (c11 + s) / t, c12 / t, (c22 + s) / t
 
 
-def cylinder_dirichlet(mesh: Mesh, WiN: float):
+def cylinder_dirichlet(mesh: Mesh, WiN: float, inflow=None):
     """Rows and values of the Cylinder_2D Dirichlet conditions, in the order
-    the reference inserts them (a row listed twice keeps the later value)."""
+    the reference inserts them (a row listed twice keeps the later value).
+    The inlet takes the fully developed profile: the Oldroyd-B closed form, or
+    with `inflow` (an inflow.FullyDevelopedInflow of this mesh) the solution
+    of the reference's Inflow_1D problem at WiN, for any model."""
     g = mesh.geometry
     x, y = mesh.coords[:, 0], mesh.coords[:, 1]
     tol = MESH_TOLERANCE
     rows: List[np.ndarray] = []
     vals: List[np.ndarray] = []
-    inlet = np.nonzero(np.abs(x + g["L_up"]) < tol)[0]
-    prof = oldroyd_b_inflow(y[inlet], g["H"], WiN)
+    if inflow is None:
+        inlet = np.nonzero(np.abs(x + g["L_up"]) < tol)[0]
+        prof = oldroyd_b_inflow(y[inlet], g["H"], WiN)
+    else:
+        inlet = inflow.nodes
+        prof, _ = inflow.solve(WiN)
     for q, v in zip((0, 1, 3, 4, 5), prof):  # Ux, Uy, Sxx, Sxy, Syy (P is skipped)
         rows.append(6 * inlet + q)
         vals.append(v)
@@ -122,13 +129,21 @@ class NewtonSolver:
 
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
                  reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = None,
-                 jacobian: str = "fd64"):
+                 jacobian: str = "fd64", inflow: str = None):
         import torch
 
         from .linsolve import BandLU, BlockLU, HybridLU, RfLU
 
-        if params.VeN != 0.0 or params.model != "L-PTT":
-            raise NewtonError("the closed-form inflow covers Oldroyd-B (L-PTT with VeN = 0) only")
+        oldroyd_b = params.VeN == 0.0 and params.model == "L-PTT"
+        if inflow is None:
+            inflow = "closed_form" if oldroyd_b else "fe1d"
+        if inflow == "closed_form" and not oldroyd_b:
+            raise NewtonError("the closed-form inflow covers Oldroyd-B (L-PTT with VeN = 0) only; use inflow='fe1d'")
+        if inflow not in ("closed_form", "fe1d"):
+            raise NewtonError("inflow must be 'closed_form' (Oldroyd-B) or 'fe1d' (the reference's Inflow_1D problem)")
+        from .inflow import FullyDevelopedInflow
+
+        self.inflow = FullyDevelopedInflow.from_mesh(mesh, params) if inflow == "fe1d" else None
         self.torch = torch
         self.mesh = mesh
         self.params = params
@@ -180,7 +195,7 @@ class NewtonSolver:
             jac = flag == "NRP"
             EU = float(x[-1])
             if EU != last_EU:
-                rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * EU)
+                rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * EU, self.inflow)
                 self.asm.set_dirichlet(rows, vals)
                 last_EU = EU
             self.asm.assemble(x, self.jac_flag if jac else flag, b=self.b, values=self.values if jac else None)
@@ -221,7 +236,7 @@ class NewtonSolver:
         """||b|| of the step's system at x (residual, arclength and Dirichlet
         rows; no Jacobian): how far a predicted state is from the solution."""
         torch = self.torch
-        rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * float(x[-1]))
+        rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * float(x[-1]), self.inflow)
         self.asm.set_dirichlet(rows, vals)
         self.asm.assemble(x, "MNR", b=self.b)
         self.asm.arclength(x, self.b, None, increment=increment, LN0=LN0, dArN=dArN, x_old=x_old, tangent=tangent,
