@@ -171,6 +171,8 @@ def newton_bench(args):
                 "EU_per_step": EU, "Wi_gain_per_second": (EU[-1] - EU[1]) * pset["WiN0"] / sum(arc),
                 "factorizations": solver.lu.refactorizations, "host_fallbacks": solver.lu.host_fallbacks,
                 "solver": type(solver.lu).__name__,
+                "refined_solves": getattr(solver.lu, "refinements", 0),
+                "gmres_iterations": getattr(solver.lu, "gmres_iterations", 0),
                 "Fd_last_step": Fd, "drag_rel_diff_vs_fp64_newton": abs(Fd - Fr) / abs(Fr),
                 "peak_wall_Txx_last_step": Pw, "peak_wall_Txx_rel_diff_vs_fp64_newton": abs(Pw - Pr) / abs(Pr),
                 "choices": cont.choices if learned else None, "factor_pattern_nnz":

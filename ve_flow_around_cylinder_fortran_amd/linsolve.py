@@ -668,12 +668,17 @@ class BandLU:
     solve after a factorisation is checked as in RfLU (host fallback with
     threshold pivoting).  precision="f32" stores and factorises the band in
     FP32 (sgemm panels and updates) with FP64 solve vectors and border: the
-    single-precision factorisation of an inexact Newton iteration."""
+    single-precision factorisation of an inexact Newton iteration.  With
+    FP32 storage every solve also forms its FP64 residual (one sparse
+    product); above refine_tol (relative) it is refined by restarted GMRES in
+    FP64 with the FP32 factors as right preconditioner — what keeps the FP32
+    factorisation usable where its plain solves lose too much accuracy (large
+    meshes, whose condition number grows)."""
 
     NB = 64
 
     def __init__(self, rowptr: np.ndarray, col: np.ndarray, device="cuda:0", check_tol: float = None,
-                 precision: str = "f64"):
+                 precision: str = "f64", refine_tol: float = 1.0e-3):
         import scipy.sparse as sp
         import torch
         from scipy.sparse.csgraph import reverse_cuthill_mckee
@@ -684,6 +689,9 @@ class BandLU:
         self.device = device
         self.precision = precision
         self.check_tol = (1e-10 if precision == "f64" else 1e-3) if check_tol is None else check_tol
+        self.refine_tol = refine_tol
+        self.refinements = 0      # solves refined by GMRES
+        self.gmres_iterations = 0
         self.rowptr_h = np.asarray(rowptr, dtype=np.int64)
         self.col_h = np.asarray(col, dtype=np.int32)
         n = self.n = len(rowptr) - 1
@@ -788,6 +796,54 @@ class BandLU:
         r = torch.mv(A, x) - b
         return float(torch.linalg.vector_norm(r)), float(torch.linalg.vector_norm(b))
 
+    def _gmres(self, b, x, bn, tol, m=30, restarts=10):
+        """Restarted GMRES(m) in FP64 for A x = b from x, right-preconditioned
+        by the band solve; stops at ||b - A x|| <= tol ||b||."""
+        torch = self.torch
+        A = torch.sparse_csr_tensor(self.ptrA, self.indA, self._A, size=(self.n, self.n))
+        for _ in range(restarts):
+            r = b - torch.mv(A, x)
+            beta = float(torch.linalg.vector_norm(r))
+            if beta <= tol * bn:
+                return x, beta
+            V = [r / beta]
+            Z = []
+            H = np.zeros((m + 1, m))
+            g = np.zeros(m + 1)
+            g[0] = beta
+            y = None
+            for j in range(m):
+                z = self._precondition(V[j])
+                Z.append(z)
+                w = torch.mv(A, z)
+                for i in range(j + 1):  # modified Gram-Schmidt
+                    H[i, j] = float(torch.dot(w, V[i]))
+                    w = w - H[i, j] * V[i]
+                H[j + 1, j] = float(torch.linalg.vector_norm(w))
+                self.gmres_iterations += 1
+                y = np.linalg.lstsq(H[:j + 2, :j + 1], g[:j + 2], rcond=None)[0]
+                est = float(np.linalg.norm(g[:j + 2] - H[:j + 2, :j + 1] @ y))
+                if est <= tol * bn or H[j + 1, j] == 0.0:
+                    break
+                V.append(w / H[j + 1, j])
+            for i, yi in enumerate(y):
+                x = x + float(yi) * Z[i]
+        r = b - torch.mv(A, x)
+        return x, float(torch.linalg.vector_norm(r))
+
+    def _precondition(self, b):
+        """The band solve with the border (FP64 vectors)."""
+        torch = self.torch
+        n1 = self.n1
+        b1 = torch.empty(n1, dtype=torch.float64, device=self.device)
+        b1[self.pu] = b[:n1]
+        y = self._solve11(b1)
+        x2 = (b[n1] - torch.dot(self.a21, y)) / self.s
+        x = torch.empty_like(b)
+        x[:n1] = (y - self.z * x2)[self.pu]
+        x[n1] = x2
+        return x
+
     def solve(self, b):
         """x with A x = b for the last factorised matrix (b is not modified)."""
         torch = self.torch
@@ -795,6 +851,16 @@ class BandLU:
             raise RfLUError("factor() first")
         if self._host_lu is not None:
             return torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
+        if self.precision == "f32":
+            x = self._precondition(b)
+            rn, bn = self._residual(self._A, x, b)
+            if rn > self.refine_tol * bn:
+                self.refinements += 1
+                x, rn = self._gmres(b, x, bn, self.refine_tol)
+            if rn <= self.check_tol * max(bn, 1e-300):
+                self._checked = True
+                return x
+            self._checked = False  # the host factorisation below takes over
         n1 = self.n1
         b1 = torch.empty(n1, dtype=torch.float64, device=self.device)
         b1[self.pu] = b[:n1]
