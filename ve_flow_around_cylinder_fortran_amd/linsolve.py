@@ -669,9 +669,11 @@ class BandLU:
     threshold pivoting).  precision="f32" stores and factorises the band in
     FP32 (sgemm panels and updates) with FP64 solve vectors and border: the
     single-precision factorisation of an inexact Newton iteration.  With
-    FP32 storage every solve also forms its FP64 residual (one sparse
-    product); above refine_tol (relative) it is refined by restarted GMRES in
-    FP64 with the FP32 factors as right preconditioner — what keeps the FP32
+    FP32 storage the first solve after a factorisation also forms its FP64
+    residual (one sparse product), and so does every later one when that
+    first solve needed refining; above refine_tol (relative) a solve is
+    refined by restarted GMRES in FP64 with the FP32 factors as right
+    preconditioner — what keeps the FP32
     factorisation usable where its plain solves lose too much accuracy (large
     meshes, whose condition number grows)."""
 
@@ -776,6 +778,7 @@ class BandLU:
         L = _band_lib()
         self._A = values
         self._checked = False
+        self._refine_all = False
         self._host_lu = None
         st = self._stream()
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
@@ -853,9 +856,12 @@ class BandLU:
             return torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
         if self.precision == "f32":
             x = self._precondition(b)
+            if self._checked and not self._refine_all:
+                return x
             rn, bn = self._residual(self._A, x, b)
             if rn > self.refine_tol * bn:
                 self.refinements += 1
+                self._refine_all = True  # this factorisation's solves are checked from now on
                 x, rn = self._gmres(b, x, bn, self.refine_tol)
             if rn <= self.check_tol * max(bn, 1e-300):
                 self._checked = True
