@@ -73,6 +73,84 @@ template <class T> __host__ __device__ inline Dual<T> dsqrt(Dual<T> a) {
 template <class T> __host__ __device__ inline Dual<T> dabs(Dual<T> a) {
   return a.v > T(0) ? a : (a.v < T(0) ? -a : Dual<T>(T(0), T(0)));
 }
+// N-direction dual numbers: one primal value carried with N tangents, so that
+// a thread differentiating in N directions evaluates the primal once.  A
+// quotient or square root takes one reciprocal for all N tangents.
+template <class T, int N>
+struct DualV {
+  T v, d[N];
+  __host__ __device__ DualV() : v(0) {
+    for (int j = 0; j < N; j++) d[j] = T(0);
+  }
+  __host__ __device__ DualV(T a) : v(a) {
+    for (int j = 0; j < N; j++) d[j] = T(0);
+  }
+};
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator+(DualV<T, N> a, DualV<T, N> b) {
+  a.v = a.v + b.v;
+  for (int j = 0; j < N; j++) a.d[j] = a.d[j] + b.d[j];
+  return a;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator-(DualV<T, N> a, DualV<T, N> b) {
+  a.v = a.v - b.v;
+  for (int j = 0; j < N; j++) a.d[j] = a.d[j] - b.d[j];
+  return a;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator-(DualV<T, N> a) {
+  a.v = -a.v;
+  for (int j = 0; j < N; j++) a.d[j] = -a.d[j];
+  return a;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator*(DualV<T, N> a, DualV<T, N> b) {
+  DualV<T, N> r;
+  r.v = a.v * b.v;
+  for (int j = 0; j < N; j++) r.d[j] = a.d[j] * b.v + a.v * b.d[j];
+  return r;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator/(DualV<T, N> a, DualV<T, N> b) {
+  const T ib = T(1) / b.v, q = a.v * ib;
+  DualV<T, N> r;
+  r.v = q;
+  for (int j = 0; j < N; j++) r.d[j] = (a.d[j] - q * b.d[j]) * ib;
+  return r;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator+(DualV<T, N> a, T b) { a.v = a.v + b; return a; }
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator+(T a, DualV<T, N> b) { b.v = a + b.v; return b; }
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator-(DualV<T, N> a, T b) { a.v = a.v - b; return a; }
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator-(T a, DualV<T, N> b) {
+  b = -b;
+  b.v = a + b.v;
+  return b;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator*(DualV<T, N> a, T b) {
+  a.v = a.v * b;
+  for (int j = 0; j < N; j++) a.d[j] = a.d[j] * b;
+  return a;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator*(T a, DualV<T, N> b) { return b * a; }
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator/(DualV<T, N> a, T b) {
+  const T ib = T(1) / b;
+  a.v = a.v / b;
+  for (int j = 0; j < N; j++) a.d[j] = a.d[j] * ib;
+  return a;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> operator/(T a, DualV<T, N> b) {
+  const T ib = T(1) / b.v, q = a * ib;
+  DualV<T, N> r;
+  r.v = q;
+  for (int j = 0; j < N; j++) r.d[j] = -q * b.d[j] * ib;
+  return r;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> dsqrt(DualV<T, N> a) {
+  const T s = sqrt(a.v), h = T(0.5) / s;
+  DualV<T, N> r;
+  r.v = s;
+  for (int j = 0; j < N; j++) r.d[j] = a.d[j] * h;
+  return r;
+}
+template <class T, int N> __host__ __device__ inline DualV<T, N> dabs(DualV<T, N> a) {
+  return a.v > T(0) ? a : (a.v < T(0) ? -a : DualV<T, N>(T(0)));
+}
 __host__ __device__ inline double dsqrt(double a) { return sqrt(a); }
 __host__ __device__ inline float dsqrt(float a) { return sqrtf(a); }
 __host__ __device__ inline double dabs(double a) { return fabs(a); }

@@ -52,6 +52,20 @@
 #include "vef_stability_math.h"
 #include "vef_ad_math.h"
 
+// Directions per thread in the FP32 Jacobian's dual-number pass.  One is the
+// fastest at 100,000 elements on MI355X (4.00 ms): two took 7.29 ms (217 VGPRs,
+// 2 waves per SIMD) or 5.60 ms (3 waves, spilling), four 7.8-15 ms; the
+// registers of the extra tangents cost more occupancy than the shared primal
+// evaluation saves.
+#ifndef JAC32_DIRS
+#define JAC32_DIRS 1
+#endif
+// Minimum waves per SIMD the FP32 Jacobian kernel is compiled for (bounds its
+// VGPRs: 4 gives 128).
+#ifndef JAC32_MIN_WAVES
+#define JAC32_MIN_WAVES 4
+#endif
+
 namespace {
 
 constexpr double kEpsJac = 1.0e-8;  // perturbVariable, src/Tools_mod.F90:831-840
@@ -877,7 +891,7 @@ __host__ __device__ constexpr int field_of(int p, int j) {
                 : (j == 0 ? vefad::kS11 : j == 1 ? vefad::kS11x : vefad::kS11y);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JAC32_MIN_WAVES, 8)))
 vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
                  int64_t eu_index, int32_t n_el, vefad::Params<float> prm, FdOut fo) {
 #pragma clang fp contract(fast)  // FP32 path: compared to FP32 accuracy, FMAs allowed
@@ -944,19 +958,29 @@ vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ 
     sF[tid - 9 * vefad::kNumInterp][vefad::kEU] = (float)EU;
   }
   __syncthreads();
-  // (2) D = WET * dT/dF, one direction per lane
-  if (tid < 9 * NF) {
-    const int g = tid / NF, f = tid % NF;
-    vefad::Dual<float> F[NF];
+  // (2) D = WET * dT/dF, JAC32_DIRS directions per lane (directions f0 ..
+  //     f0 + JAC32_DIRS - 1 of Gauss point g; the last group's surplus
+  //     directions are computed and dropped)
+  constexpr int NDIR = JAC32_DIRS, NGRP = (NF + NDIR - 1) / NDIR;
+  if (tid < 9 * NGRP) {
+    const int g = tid / NGRP, f0 = (tid % NGRP) * NDIR;
+    vefad::DualV<float, NDIR> F[NF];
 #pragma unroll
-    for (int i = 0; i < NF; i++) F[i] = vefad::Dual<float>(sF[g][i], i == f ? 1.0f : 0.0f);
-    vefad::Dual<float> T[3][6];
+    for (int i = 0; i < NF; i++) {
+      F[i].v = sF[g][i];
+#pragma unroll
+      for (int j = 0; j < NDIR; j++) F[i].d[j] = i == f0 + j ? 1.0f : 0.0f;
+    }
+    vefad::DualV<float, NDIR> T[3][6];
     vefad::gauss_terms(F, sBas[g][1], sBas[g][2], sH2[g], prm, T);
     const float w = sWET[g];
 #pragma unroll
-    for (int k = 0; k < 3; k++)
+    for (int j = 0; j < NDIR; j++)
+      if (f0 + j < NF)
 #pragma unroll
-      for (int q = 0; q < 6; q++) sD[g][f][k * 6 + q] = w * T[k][q].d;
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+          for (int q = 0; q < 6; q++) sD[g][f0 + j][k * 6 + q] = w * T[k][q].d[j];
   }
   // (3) J = sum_g Phi_g^T E_g on the matrix cores.  v_mfma_f32_16x16x4f32
   //     lane layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j]
