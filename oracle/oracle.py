@@ -51,6 +51,8 @@ def lib(flops: bool = False):
             L.vo_gauss_fields.argtypes = [vp, vp, ctypes.c_int, vp]
             L.vo_gauss_tau.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, vp]
             L.vo_tau_plant.argtypes = [ctypes.c_int]
+            L.vo_term_plant.argtypes = [ctypes.c_int]
+            L.vo_node_terms.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, vp]
             L.vo_point_strong_form.argtypes = [ctypes.POINTER(OracleParams), vp, vp, vp, ctypes.c_double, vp, vp, vp,
                                                ctypes.c_double, vp]
             L.vo_element.argtypes = [ctypes.POINTER(OracleParams), vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -238,6 +240,28 @@ def tau_plant(which: int):
     """Plants a known mistake in the tau coefficients (0 removes it)."""
     if lib().vo_tau_plant(int(which)) != 0:
         raise ValueError(f"tau_plant: no plant {which}")
+
+
+def node_terms(prm, xy, TL, EU, gp):
+    """The summands of the element residual's test-node terms at bulk Gauss
+    point gp, as the oracle adds them (vo_node_terms): mom[n][i][t] (t:
+    ReN*Convection*phi, P_tot(i,:).grad phi, LSME, LSIC, LSCE), cont[n][t]
+    (Continuity*phi, LSME), str[n][s][t] (s = xx, xy, yy; t: Con_EQ*phi, DfN
+    term, LSME, LSCE, DCS), WET and tdcs*|Con_EQ|."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    TL = np.ascontiguousarray(TL, dtype=np.float64)
+    out = np.zeros(245)
+    if lib().vo_node_terms(ctypes.byref(prm), _p(xy), _p(TL), float(EU), int(gp), _p(out)) != 0:
+        raise ValueError(f"node_terms: Gauss point {gp} out of range")
+    t = out[:243].reshape(9, 27)
+    return dict(mom=t[:, :10].reshape(9, 2, 5), cont=t[:, 10:12], str=t[:, 12:].reshape(9, 3, 5), WET=out[243],
+                dcs_coef=out[244])
+
+
+def term_plant(which: int):
+    """Plants a known mistake in the test-node terms (0 removes it)."""
+    if lib().vo_term_plant(int(which)) != 0:
+        raise ValueError(f"term_plant: no plant {which}")
 
 
 def _stab_rc(rc, what):

@@ -601,26 +601,39 @@ static void identity(vreal I[3][3]) {
 // a8: test-node residual terms, Residuals_mod.F90 + equationsResidualsBulkMain
 // (Equations_mod.F90:102-144)
 // ----------------------------------------------------------------------------
-static void nodeResidual(const Phys& ph, const Elem& E, const GsPt& G, int n, vreal TERM[6]) {
+// The summands of TERM_RES for one test node at one Gauss point, in the order
+// the reference adds them.
+struct NodeTerms {
+  vreal mom[2][5];  // momentumMainSQRT :331-335: ReN*Convection(i)*bifn, P_tot(i,:).dFdXi, LSME, LSIC, LSCE
+  vreal cont[2];    // continuityBulkMain :90-110: Continuity*bifn, LSMEContinuity
+  vreal str[3][5];  // stressesBulkMainSQRT :375-380: Con_EQ(j,k)*bifn, DfN*dSdXi(j,k,:).dFdXi, LSME, LSCE, DCS
+};
+
+// Planted mistakes in the test-node terms, for the tests that must catch them
+// (0: none; vo_term_plant lists them).
+static int g_term_plant = 0;
+
+static void nodeTerms(const Phys& ph, const Elem& E, const GsPt& G, int n, NodeTerms& t) {
+  const int P = g_term_plant;
   vreal bifn = vreal(E.bfn[n]);
   vreal dF[3] = {E.dFdXi[0][n], E.dFdXi[1][n], E.dFdXi[2][n]};
   vreal one = vreal(1.0);
   // Values the reference recomputes with identical operands for each i or each
   // stress component are computed once here (bitwise the same).
-  vreal omb = one - ph.BvN;
+  vreal omb = P == 8 ? one : one - ph.BvN;
   vreal Udf = dot3(G.U_f, dF);
-  vreal lsme_m = G.tlsme * ph.ReN * Udf;  // LSMEMomentum :161-173
+  vreal lsme_m = G.tlsme * (P == 3 ? one : ph.ReN) * Udf;  // LSMEMomentum :161-173
   // momentumMainSQRT :295-337
   for (int i = 0; i < 2; i++) {
-    vreal lsme = lsme_m * G.Momentum[i];
-    vreal lsic = G.tlsic * dF[i];                      // LSICMomentum :177-187
+    vreal lsme = lsme_m * G.Momentum[P == 4 ? 1 - i : i];
+    vreal lsic = G.tlsic * dF[P == 5 ? 1 - i : i];  // LSICMomentum :177-187
     lsic = lsic * G.Continuity;
     // LSCEMomentumSQRT :388-433
     vreal GUT_l[3][3], GU_l[3][3];
     zero33(GUT_l);
     for (int k = 0; k < 3; k++) GUT_l[i][k] = dF[k];
     for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) GU_l[a][b] = GUT_l[b][a];
+      for (int b = 0; b < 3; b++) GU_l[a][b] = P == 7 ? GUT_l[a][b] : GUT_l[b][a];
     vreal W_l[3][3];
     zero33(W_l);
     W_l[0][1] = omega12(G.S, GUT_l);
@@ -631,36 +644,51 @@ static void nodeResidual(const Phys& ph, const Elem& E, const GsPt& G, int n, vr
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++) {
         vreal U = -M1[a][b] - M2[a][b];
-        lsce[a][b] = cdiv(G.tlsce * U * omb, G.WiN);
+        lsce[a][b] = P == 6 ? G.tlsce * U * omb * G.WiN : cdiv(G.tlsce * U * omb, G.WiN);
       }
-    vreal lsce_term = ddot33(lsce, G.Con_EQ);
     vreal Prow[3] = {G.P_tot[i][0], G.P_tot[i][1], G.P_tot[i][2]};
-    TERM[i] = ph.ReN * G.Convection[i] * bifn + dot3(Prow, dF) + lsme + lsic + lsce_term;
+    t.mom[i][0] = (P == 1 ? one : ph.ReN) * G.Convection[i] * bifn;
+    t.mom[i][1] = P == 2 ? -dot3(Prow, dF) : dot3(Prow, dF);
+    t.mom[i][2] = lsme;
+    t.mom[i][3] = lsic;
+    t.mom[i][4] = ddot33(lsce, G.Con_EQ);
   }
   // continuityBulkMain :90-110 with LSMEContinuity :223-234
-  {
-    vreal lsme = G.tlsme * dot3(dF, G.Momentum);
-    TERM[2] = G.Continuity * bifn + lsme;
-  }
+  t.cont[0] = (P == 9 ? -G.Continuity : G.Continuity) * bifn;
+  t.cont[1] = G.tlsme * dot3(dF, G.Momentum);
+  if (P == 10) t.cont[1] = -t.cont[1];
   // stressesBulkMainSQRT :341-384
   // LSCEStressSQRT :459-475 (independent of the stress component)
-  vreal lsce_s = G.WiN * Udf;
-  lsce_s = lsce_s + cdiv(G.f_T * bifn, vreal(2.0));
+  vreal lsce_s = (P == 16 ? one : G.WiN) * Udf;
+  lsce_s = lsce_s + (P == 15 ? G.f_T * bifn : cdiv(G.f_T * bifn, vreal(2.0)));
   lsce_s = G.tlsce * lsce_s;
   const int comp[3][2] = {{0, 0}, {0, 1}, {1, 1}};
   for (int s = 0; s < 3; s++) {
     int j = comp[s][0], k = comp[s][1];
     // LSMEStressSQRT :437-455
-    vreal lsme = -(dF[k] * G.Momentum[j]);
-    if (j != k) lsme = lsme - dF[j] * G.Momentum[k];
-    lsme = G.tlsme * lsme * G.WiN;
+    vreal lsme = P == 18 ? dF[k] * G.Momentum[j] : -(dF[k] * G.Momentum[j]);
+    if (j != k && P != 13) lsme = lsme - dF[j] * G.Momentum[k];
+    lsme = P == 14 ? cdiv(G.tlsme * lsme, G.WiN) : G.tlsme * lsme * G.WiN;
     vreal lsce = lsce_s * G.Con_EQ[j][k];
     // DCSStressSQRT :479-490 times mag_con_EQ (real part of a complex product)
     vreal dS[3] = {G.dSdXi[j][k][0], G.dSdXi[j][k][1], G.dSdXi[j][k][2]};
     vreal dcs = G.tdcs * dot3(dS, dF);
-    dcs = dcs * G.mag_re;
-    TERM[3 + s] = G.Con_EQ[j][k] * bifn + ph.DfN * dot3(dS, dF) + lsme + lsce + dcs;
+    if (P != 17) dcs = dcs * G.mag_re;
+    t.str[s][0] = G.Con_EQ[j][P == 11 ? j : k] * bifn;
+    t.str[s][1] = (P == 12 ? -ph.DfN : ph.DfN) * dot3(dS, dF);
+    t.str[s][2] = lsme;
+    t.str[s][3] = lsce;
+    t.str[s][4] = dcs;
   }
+}
+
+static void nodeResidual(const Phys& ph, const Elem& E, const GsPt& G, int n, vreal TERM[6]) {
+  NodeTerms t;
+  nodeTerms(ph, E, G, n, t);
+  for (int i = 0; i < 2; i++) TERM[i] = t.mom[i][0] + t.mom[i][1] + t.mom[i][2] + t.mom[i][3] + t.mom[i][4];
+  TERM[2] = t.cont[0] + t.cont[1];
+  for (int s = 0; s < 3; s++)
+    TERM[3 + s] = t.str[s][0] + t.str[s][1] + t.str[s][2] + t.str[s][3] + t.str[s][4];
 }
 
 // residualBulk, Equations_mod.F90:59-98
@@ -1672,6 +1700,58 @@ int vo_gauss_tau(const vo_params* p, const double* xy, const double* TL, double 
   out[3] = G.tlsic;
   out[4] = G.tlsce;
   out[5] = G.tdcs;
+  return 0;
+}
+
+// Plants a mistake in the test-node terms (0 removes it):
+//  1 ReN*Convection*bifn without ReN         2 P_tot(i,:).dFdXi with the wrong sign
+//  3 LSMEMomentum without ReN                4 LSMEMomentum times the other Momentum component
+//  5 LSICMomentum with the other dFdXi       6 LSCEMomentumSQRT times WiN instead of over WiN
+//  7 LSCEMomentumSQRT with GU_lsce untransposed   8 LSCEMomentumSQRT without (1-BvN)
+//  9 Continuity*bifn with the wrong sign     10 LSMEContinuity with the wrong sign
+// 11 Con_EQ(j,j) for Con_EQ(j,k)             12 DfN term with the wrong sign
+// 13 LSMEStressSQRT without its j /= k term  14 LSMEStressSQRT over WiN instead of times WiN
+// 15 LSCEStressSQRT f_T*bifn without the 1/2 16 LSCEStressSQRT without WiN on U.dFdXi
+// 17 DCS term without mag_con_EQ             18 LSMEStressSQRT with the wrong sign
+int vo_term_plant(int which) {
+  if (which < 0 || which > 18) return -1;
+  g_term_plant = which;
+  return 0;
+}
+
+// The summands of TERM_RES at bulk Gauss point gp for each test node, as
+// nodeResidual adds them: out[n*27 + c] with c = 5*i + t for the momentum
+// rows (t: ReN*Convection*bifn, P_tot.dFdXi, LSME, LSIC, LSCE), 10 + t for
+// the continuity row (Continuity*bifn, LSME), 12 + 5*s + t for the stress
+// rows s = xx, xy, yy (Con_EQ*bifn, DfN term, LSME, LSCE, DCS); out[243] =
+// WET, out[244] = tdcs * mag_con_EQ (real part).
+int vo_node_terms(const vo_params* p, const double* xy, const double* TL, double EU, int gp, double* out) {
+  if (gp < 0 || gp > 8) return -1;
+  Phys ph = make_phys(p);
+  const Tables& T = tables();
+  Elem E;
+  load_elem(E, xy, TL, EU);
+  E.bfn = T.bfn[gp]; E.dfdc = T.dfdc[gp]; E.dfde = T.dfde[gp];
+  E.d2fdc2 = T.d2fdc2[gp]; E.d2fde2 = T.d2fde2[gp]; E.d2fdce = T.d2fdce[gp];
+  vreal I[3][3];
+  identity(I);
+  GsPt G;
+  basis2D(E, G);
+  setFlowQuantities(ph, E, G, I);
+  G.WET = vreal(T.W2[gp]) * fabsr(G.detJ);
+  for (int n = 0; n < 9; n++) {
+    NodeTerms t;
+    nodeTerms(ph, E, G, n, t);
+    double* o = out + n * 27;
+    for (int i = 0; i < 2; i++)
+      for (int k = 0; k < 5; k++) o[5 * i + k] = val(t.mom[i][k]);
+    o[10] = val(t.cont[0]);
+    o[11] = val(t.cont[1]);
+    for (int s = 0; s < 3; s++)
+      for (int k = 0; k < 5; k++) o[12 + 5 * s + k] = val(t.str[s][k]);
+  }
+  out[243] = val(G.WET);
+  out[244] = val(G.tdcs * G.mag_re);
   return 0;
 }
 
