@@ -66,15 +66,21 @@ def newton_cpu(oracle, mesh, pset, x0, LN0, max_iter=NITER, nthreads=8, xF=1.0, 
     return x, hist
 
 
-def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent, dS, nthreads=8, inflow=None):
+def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent, dS, nthreads=8, inflow=None,
+                   max_iter=NITER, cor_norm_div=COR_NORM_DIV, xF=1.0, emergencies=None):
     """One Newton solve of a continuation step (pinned for increments 1-2,
-    pseudo-arclength after); the inflow values follow the current EU.
-    Returns (x, history, last SuperLU factorisation)."""
+    pseudo-arclength after); the inflow values follow the current EU.  On
+    divergence or too many iterations it restarts as emergencyNewtonRaphson
+    does (src/Newton_Raphson_mod.F90:533-584): from x0 with xF and dS halved
+    and the cap at 50*max_iter, stopping below xF = 1e-2.
+    Returns (x, history, last SuperLU factorisation); `emergencies` (a list)
+    receives (records so far, reason, xF, dS) per restart."""
     prm = oracle.params(**pset)
     rowptr, col = oracle.pattern(mesh)
     N = mesh.n_unknowns
     x = np.array(x0, dtype=np.float64)
     flag, res_norm, cor_new, cor_old, it = "NRP", 1.0, 1.0, 1.0, 0
+    cap = max_iter
     lu = None
     hist = []
     mode = 0 if increment <= 2 else 1
@@ -101,18 +107,29 @@ def newton_cpu_arc(oracle, mesh, pset, x0, LN0, increment, dArN, x_old, tangent,
             flag = "MNR" if cor_new < MNR_EPS else "NRP"
         else:
             flag = "NRP" if cor_new > cor_old else "MNR"
-        if cor_new > COR_NORM_DIV or it > NITER:
-            raise RuntimeError(f"no convergence: {hist}")
+        if cor_new > cor_norm_div or it > cap:
+            why = "divergence" if cor_new > cor_norm_div else "iterations"
+            xF = 0.5 * xF
+            dS = 0.5 * dS
+            if emergencies is not None:
+                emergencies.append((len(hist), why, xF, dS))
+            if xF < 1.0e-2:
+                raise RuntimeError(f"no convergence: relaxation factor below 1e-2: {hist}")
+            cap = 50 * max_iter
+            x = np.array(x0, dtype=np.float64)
+            flag, res_norm, cor_new, cor_old, it = "NRP", 1.0, 1.0, 1.0, 0
+            continue
         cor_old = cor_new
-        x = x - s
+        x = x - xF * s
     return x, hist, lu
 
 
 def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=50.0, eps_pr=1.0e-3, nthreads=8,
-                     times=None, inflow=None):
+                     times=None, inflow=None, max_iter=NITER, cor_norm_div=COR_NORM_DIV, emergencies=None):
     """continuationStepArclength restated with newton_cpu_arc; returns the
     converged states and the step sizes used (and appends each step's wall
-    time to `times` when given)."""
+    time to `times` when given, and each step's emergency restarts to
+    `emergencies`)."""
     import time
     x = np.array(x0, dtype=np.float64)
     x_old = tangent = None
@@ -122,8 +139,13 @@ def continuation_cpu(oracle, mesh, pset, x0, LN0, dArN, steps, dS0=1.0, dS_max=5
         t0 = time.perf_counter()
         predicted = x.copy()
         dss.append(dS)
+        em = []
         xc, hist, lu = newton_cpu_arc(oracle, mesh, pset, x, LN0, k, dArN, x_old, tangent, dS, nthreads=nthreads,
-                                      inflow=inflow)
+                                      inflow=inflow, max_iter=max_iter, cor_norm_div=cor_norm_div, emergencies=em)
+        if em:
+            dS = em[-1][3]  # dSo halved by each restart
+        if emergencies is not None:
+            emergencies.append(em)
         if k > 1:
             if k == 2:
                 d = xc - x_old

@@ -1,0 +1,128 @@
+"""Multifrontal LU on the GPU (libvef_mf.so through linsolve's bordered
+interface): Newton matrices of the cylinder problem with the arclength and
+Dirichlet rows against scipy, FP64 and FP32 (with FP64 GMRES refinement),
+deterministic repeats, a Newton solve against the CPU restatement, and the
+40,000-element Newton matrix (residual of a solve; scipy cannot factorise it
+in this memory)."""
+import time
+
+import numpy as np
+import pytest
+
+from ve_flow_around_cylinder_fortran_amd import Assembler, Params, cylinder_channel, smooth_state
+from ve_flow_around_cylinder_fortran_amd.mesh import workload_mesh
+from ve_flow_around_cylinder_fortran_amd.newton import NewtonSolver, cylinder_dirichlet, initial_state
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+OLDROYD_B = dict(ReN=0.0, BvN=0.59, VeN=0.0, DfN=1e-3, WiN0=1.0, model="L-PTT")
+
+
+def newton_system(mesh, x, EU=0.1):
+    asm = Assembler(mesh, Params(**OLDROYD_B))
+    rows, vals = cylinder_dirichlet(mesh, EU)
+    asm.set_dirichlet(rows, vals)
+    xd = torch.from_numpy(x).cuda()
+    b, v = asm.assemble(xd, "NRP")
+    asm.arclength(xd, b, v, increment=1, LN0=EU)
+    asm.apply_dirichlet(xd, b, v, "NRP")
+    return asm, b, v
+
+
+@pytest.mark.parametrize("dims,precision", [((8, 4, 6, 8), "f64"), ((8, 4, 6, 8), "f32"),
+                                            ((16, 12, 20, 32), "f64"), ((16, 12, 20, 32), "f32")])
+def test_multifrontal_matches_scipy(dims, precision):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from ve_flow_around_cylinder_fortran_amd.multifrontal import MultifrontalLU
+
+    mesh = cylinder_channel(*dims)
+    lu = None
+    for k, x in enumerate([initial_state(mesh, 0.1), smooth_state(mesh, EU=0.1)]):
+        print(f"[gpu mf] {dims} {precision} matrix {k}", flush=True)
+        asm, b, v = newton_system(mesh, x)
+        rowptr, col = asm.pattern()
+        if lu is None:
+            # FP32: refined by FP64 GMRES down to a 1e-10 relative residual
+            lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, precision=precision, refine_tol=1e-10)
+        lu.factor(v)
+        s = lu.solve(b)
+        torch.cuda.synchronize()
+        assert lu.zero_pivots() == 0
+        A = sp.csr_matrix((v.cpu().numpy(), col, rowptr), shape=(asm.N, asm.N))
+        ref = sla.spsolve(A.tocsc(), b.cpu().numpy())
+        err = np.abs(s.cpu().numpy() - ref).max() / np.abs(ref).max()
+        plain = np.abs(lu._precondition(b).cpu().numpy() - ref).max() / np.abs(ref).max()
+        print(f"[gpu mf] {dims} {precision} matrix {k}: max error {err:.2e} (unrefined {plain:.2e})", flush=True)
+        assert err < (1e-9 if precision == "f64" else 1e-7), err  # FP32: the 1e-10 residual times the conditioning
+        assert plain < (1e-9 if precision == "f64" else 1e-3), plain
+    assert lu.refactorizations == 2 and lu.host_fallbacks == 0
+
+
+def test_multifrontal_repeats_bitwise():
+    from ve_flow_around_cylinder_fortran_amd.multifrontal import MultifrontalLU
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    asm, b, v = newton_system(mesh, smooth_state(mesh, EU=0.1))
+    rowptr, col = asm.pattern()
+    lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn)
+    out = []
+    for _ in range(3):
+        lu.factor(v)
+        out.append(lu.solve(b).cpu().numpy())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_newton_with_multifrontal_matches_cpu(oracle):
+    from newton_cpu import newton_cpu
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    LN0 = 0.1
+    x0 = initial_state(mesh, LN0)
+    print("[gpu mf] Newton solve, small mesh", flush=True)
+    solver = NewtonSolver(mesh, Params(**OLDROYD_B), factorization="mf")
+    xg, hist = solver.solve(x0, LN0)
+    xc, hc = newton_cpu(oracle, mesh, OLDROYD_B, x0, LN0)
+    assert [h.flag for h in hist] == [h[1] for h in hc]
+    assert np.abs(xg.cpu().numpy() - xc).max() <= 1e-8 * np.abs(xc).max()
+    assert solver.lu.host_fallbacks == 0
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_multifrontal_40k_newton_matrix(precision):
+    """The 40,000-element Newton matrix (966,961 unknowns): factorisation
+    and solves within 1e-10 relative residual (FP32: after refinement)."""
+    from ve_flow_around_cylinder_fortran_amd.multifrontal import MultifrontalLU
+
+    mesh = workload_mesh(40000)
+    print(f"[gpu mf] 40k {precision}: assembling", flush=True)
+    asm, b, v = newton_system(mesh, smooth_state(mesh, EU=0.1))
+    rowptr, col = asm.pattern()
+    t0 = time.perf_counter()
+    lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, precision=precision,
+                        refine_tol=1e-10 if precision == "f32" else 1e-3)
+    t1 = time.perf_counter()
+    print(f"[gpu mf] 40k {precision}: analysis {t1 - t0:.1f} s, {lu.device_bytes / 1e9:.1f} GB, "
+          f"{lu.plan.flops:.3g} flops", flush=True)
+    times = []
+    for k in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        lu.factor(v)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    assert lu.zero_pivots() == 0
+    ts = []
+    for k in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s = lu.solve(b, check=(k == 0))
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    rn, bn = lu._residual(v, s, b)
+    print(f"[gpu mf] 40k {precision}: factor {min(times) * 1e3:.1f} ms, solve {min(ts) * 1e3:.1f} ms, "
+          f"relative residual {rn / bn:.2e}, refinements {lu.refinements}", flush=True)
+    assert rn <= 1e-10 * bn
+    assert lu.host_fallbacks == 0

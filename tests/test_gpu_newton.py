@@ -294,6 +294,54 @@ def test_arclength_continuation_matches_cpu(oracle):
     assert gpu_states[-1][-1] > gpu_states[2][-1] > 0.12
 
 
+def test_emergency_restarts_match_cpu(oracle):
+    """emergencyNewtonRaphson forced from the host: with the iteration cap at
+    2 every step of a four-step continuation (two pinned, two
+    pseudo-arclength) restarts from its prediction with xF = 1/2, the cap at
+    100 and dS halved, then converges.  The GPU driver takes the CPU
+    restatement's restarts, step sizes and states."""
+    from newton_cpu import continuation_cpu
+    from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    x0 = initial_state(mesh, 0.1)
+    print("[gpu newton] continuation with forced emergency restarts, small mesh", flush=True)
+    cont = ArclengthContinuation(NewtonSolver(mesh, Params(**OLDROYD_B)), LN0=0.1, dArN=0.02)
+    cont.max_iter = 2
+    cont.start(x0)
+    gpu_states, gpu_dS, gpu_em = [], [], []
+    for _ in range(4):
+        x, hist = cont.step()
+        gpu_states.append(x.cpu().numpy())
+        gpu_dS.append(cont.dS)
+        gpu_em.append([(e[1], e[2]) for e in cont.s.emergencies])
+        assert hist[-1].cor_norm < 1e-8 and hist[-1].res_norm < 1e-8
+    em = []
+    cpu_states, cpu_dS = continuation_cpu(oracle, mesh, OLDROYD_B, x0, 0.1, 0.02, 4, max_iter=2, emergencies=em)
+    assert gpu_em == [[(e[1], e[2]) for e in step] for step in em]
+    assert all(len(step) == 1 and step[0] == ("iterations", 0.5) for step in gpu_em)
+    for g, c in zip(gpu_states, cpu_states):
+        assert abs(g[-1] - c[-1]) <= 2e-8
+        assert np.abs(g - c).max() <= 1e-7 * np.abs(c).max()
+    # cpu_dS holds the step entering each increment; the GPU's the one after it
+    assert np.allclose(gpu_dS[:-1], cpu_dS[1:], rtol=1e-4, atol=0)
+
+
+def test_divergence_stops_after_seven_restarts():
+    """A divergence norm below any correction: the solve restarts with xF
+    halved each time and stops with NewtonError once xF < 1e-2 (the seventh
+    restart), as the reference's general stop does."""
+    from ve_flow_around_cylinder_fortran_amd.newton import NewtonError
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    solver = NewtonSolver(mesh, Params(**OLDROYD_B))
+    print("[gpu newton] forced divergence, small mesh", flush=True)
+    with pytest.raises(NewtonError, match="below 1e-2"):
+        solver.solve(initial_state(mesh, 0.1), 0.1, cor_norm_div=1e-30)
+    assert [e[2] for e in solver.emergencies] == [0.5 ** k for k in range(1, 8)]
+    assert all(e[1] == "divergence" for e in solver.emergencies)
+
+
 def test_inexact_newton_converges_to_reference_solutions():
     """The inexact Newton (the last factorisation carried across continuation
     steps) converges to the same solutions: after five steps, an FP64 Newton

@@ -1,8 +1,10 @@
 """Build steps of the package (run by __graft_entry__.build()).
 
 * libvef.so: the HIP assembler, libvef_mlp.so: the per-node MLP of the
-  learned predictor, and libvef_band.so: the banded LU of the Newton solve
-  (links rocBLAS for its trailing updates); hipcc for gfx950, in-tree.  FP64 results must
+  learned predictor, libvef_band.so: the banded LU of the Newton solve
+  (links rocBLAS for its trailing updates) and libvef_mf.so: the multifrontal
+  LU in nested-dissection order (links rocSOLVER and rocBLAS for the fronts'
+  dense blocks); hipcc for gfx950, in-tree.  FP64 results must
   follow the source's operation order, so contraction into FMA is off
   (-ffp-contract=off) and no fast-math is used; packed FP32 instructions are
   disabled for all HIP code.
@@ -25,6 +27,8 @@ MLP_SRC = os.path.join(PKG, "csrc", "vef_mlp.hip")
 MLP_LIB = os.path.join(PKG, "libvef_mlp.so")
 BAND_SRC = os.path.join(PKG, "csrc", "vef_band.hip")
 BAND_LIB = os.path.join(PKG, "libvef_band.so")
+MF_SRC = os.path.join(PKG, "csrc", "vef_mf.hip")
+MF_LIB = os.path.join(PKG, "libvef_mf.so")
 FORTRAN_DIR = os.path.join(PKG, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
 
@@ -49,7 +53,8 @@ def hipcc() -> str:
 
 
 def build_hip(verbose: bool = False) -> str:
-    for src, lib, extra in ((HIP_SRC, LIB, []), (MLP_SRC, MLP_LIB, []), (BAND_SRC, BAND_LIB, ["-ffp-contract=fast", "-lrocblas"])):
+    for src, lib, extra in ((HIP_SRC, LIB, []), (MLP_SRC, MLP_LIB, []), (BAND_SRC, BAND_LIB, ["-ffp-contract=fast", "-lrocblas"]),
+                             (MF_SRC, MF_LIB, ["-lrocsolver", "-lrocblas"])):
         tmp = lib + ".tmp"
         cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, src, *extra]
         if verbose:

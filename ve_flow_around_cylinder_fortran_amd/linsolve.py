@@ -651,42 +651,23 @@ def _band_lib():
     return _band
 
 
-class BandLU:
-    """Banded LU without pivoting on the GPU (libvef_band.so).
+class _BorderedLU:
+    """Shared part of the node-block factorisations with the dense arclength
+    row and column as a border: A = [A11 a12; a21 a22] with A11 the node
+    unknowns in the factorisation's own order (`pu`: unknown -> inner index).
+    A subclass factorises A11 (`_factor11`) and solves with it (`_solve11`);
+    here: z = A11^-1 a12 and the Schur complement s = a22 - a21.z once per
+    factorisation, solves x2 = (b2 - a21.A11^-1 b1)/s, x1 = A11^-1 b1 - z x2,
+    the residual check of the first solve after a factorisation (a host
+    factorisation with threshold pivoting takes over if it fails) and, for
+    FP32 storage, FP64 residuals and GMRES refinement with the FP32 factors
+    as right preconditioner."""
 
-    The node unknowns are ordered by reverse Cuthill-McKee on the node graph
-    (6 unknowns per node, consecutive), which on the cylinder meshes gives a
-    half-bandwidth of about 6 x 254 unknowns at 1,600 elements; the dense
-    arclength row and column form a border.  Each factorisation scatters the
-    node block of the matrix into the band and factorises it in 64-column
-    steps (a one-wavefront diagonal LU, thread-parallel triangular solves for
-    the panel and the diagonal blocks' inverses, a rocBLAS dgemm for the
-    trailing update — a dense, dependency-light schedule where the sparse
-    re-factorisation is bound by its 3,097-level elimination chain), then
-    solves A11 z = a12 and keeps the Schur complement s = a22 - a21.z.  A
-    solve is x2 = (b2 - a21.A11^-1 b1)/s, x1 = A11^-1 b1 - z x2.  The first
-    solve after a factorisation is checked as in RfLU (host fallback with
-    threshold pivoting).  precision="f32" stores and factorises the band in
-    FP32 (sgemm panels and updates) with FP64 solve vectors and border: the
-    single-precision factorisation of an inexact Newton iteration.  With
-    FP32 storage the first solve after a factorisation also forms its FP64
-    residual (one sparse product), and so does every later one when that
-    first solve needed refining; above refine_tol (relative) a solve is
-    refined by restarted GMRES in FP64 with the FP32 factors as right
-    preconditioner — what keeps the FP32
-    factorisation usable where its plain solves lose too much accuracy (large
-    meshes, whose condition number grows)."""
-
-    NB = 64
-
-    def __init__(self, rowptr: np.ndarray, col: np.ndarray, device="cuda:0", check_tol: float = None,
-                 precision: str = "f64", refine_tol: float = 1.0e-3):
-        import scipy.sparse as sp
+    def _border_setup(self, rowptr, col, device, precision, check_tol, refine_tol, pu):
         import torch
-        from scipy.sparse.csgraph import reverse_cuthill_mckee
 
         if precision not in ("f64", "f32"):
-            raise RfLUError("BandLU precision must be 'f64' or 'f32'")
+            raise RfLUError(f"{type(self).__name__} precision must be 'f64' or 'f32'")
         self.torch = torch
         self.device = device
         self.precision = precision
@@ -698,30 +679,9 @@ class BandLU:
         self.col_h = np.asarray(col, dtype=np.int32)
         n = self.n = len(rowptr) - 1
         n1 = self.n1 = n - 1
-        if n1 % 6 != 0:
-            raise RfLUError("BandLU expects 6 unknowns per node and one arclength unknown")
-        nn = n1 // 6
         rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(self.rowptr_h))
         cols = self.col_h.astype(np.int64)
-        inner = (rows < n1) & (cols < n1)
-        G = sp.csr_matrix((np.ones(int(inner.sum())), (rows[inner] // 6, cols[inner] // 6)), shape=(nn, nn))
-        order = reverse_cuthill_mckee(G, symmetric_mode=True).astype(np.int64)
-        pos = np.empty(nn, dtype=np.int64)
-        pos[order] = np.arange(nn)
-        pu = pos[np.arange(n1) // 6] * 6 + np.arange(n1) % 6   # new index of each node unknown
-        pi, pj = pu[rows[inner]], pu[cols[inner]]
-        self.w = int(np.abs(pi - pj).max()) if len(pi) else 0
-        L = _band_lib()
-        self.h = ctypes.c_void_p()
-        bits = 64 if precision == "f64" else 32
-        if L.vef_band_create_prec(n1, self.w, self.NB, bits, ctypes.byref(self.h)) != 0:
-            raise RfLUError("vef_band_create_prec failed (device memory?)")
-        ld, K = ctypes.c_int64(), ctypes.c_int32()
-        self._err(L.vef_band_storage(self.h, None, ctypes.byref(ld), ctypes.byref(K)), "vef_band_storage")
-        self.ld, self.K = ld.value, K.value
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int64)).to(device)  # noqa: E731
-        self.src_in = dev(np.nonzero(inner)[0])
-        self.dst_in = dev((self.K + pi - pj) + pj * self.ld)
         colb = (rows < n1) & (cols == n1)
         rowb = (rows == n1) & (cols < n1)
         corner = np.nonzero((rows == n1) & (cols == n1))[0]
@@ -740,51 +700,21 @@ class BandLU:
         self.host_fallbacks = 0
         self._A = None
         self._checked = False
+        self._refine_all = False
         self._host_lu = None
-
-    def close(self):
-        if getattr(self, "h", None):
-            _band_lib().vef_band_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
-
-    def _err(self, rc, what):
-        if rc != 0:
-            raise RfLUError(f"{what}: {_band_lib().vef_band_last_error(self.h).decode()}")
-
-    @property
-    def uses_graph(self) -> bool:
-        """True once the factorisation replays its captured HIP graph."""
-        return bool(_band_lib().vef_band_uses_graph(self.h))
+        return rows, cols
 
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
 
-    def _solve11(self, y):
-        """A11^-1 y in the band order (y is consumed)."""
-        x = self.torch.empty_like(y)
-        self._err(_band_lib().vef_band_solve(self.h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                                             self._stream()), "vef_band_solve")
-        return x
-
     def factor(self, values):
         """Factorises the matrix with the given device values (same pattern)."""
         torch = self.torch
-        L = _band_lib()
         self._A = values
         self._checked = False
         self._refine_all = False
         self._host_lu = None
-        st = self._stream()
-        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-        self._err(L.vef_band_set(self.h, int(self.src_in.numel()), p(self.src_in), p(self.dst_in), p(values), st),
-                  "vef_band_set")
-        self._err(L.vef_band_factor(self.h, st), "vef_band_factor")
+        self._factor11(values)
         self.a12.zero_()
         self.a12[self.dst_c] = values[self.src_c]
         self.a21.zero_()
@@ -801,7 +731,7 @@ class BandLU:
 
     def _gmres(self, b, x, bn, tol, m=30, restarts=10):
         """Restarted GMRES(m) in FP64 for A x = b from x, right-preconditioned
-        by the band solve; stops at ||b - A x|| <= tol ||b||."""
+        by the factorisation's solve; stops at ||b - A x|| <= tol ||b||."""
         torch = self.torch
         A = torch.sparse_csr_tensor(self.ptrA, self.indA, self._A, size=(self.n, self.n))
         for _ in range(restarts):
@@ -835,7 +765,7 @@ class BandLU:
         return x, float(torch.linalg.vector_norm(r))
 
     def _precondition(self, b):
-        """The band solve with the border (FP64 vectors)."""
+        """The node-block solve with the border (FP64 vectors)."""
         torch = self.torch
         n1 = self.n1
         b1 = torch.empty(n1, dtype=torch.float64, device=self.device)
@@ -847,8 +777,10 @@ class BandLU:
         x[n1] = x2
         return x
 
-    def solve(self, b):
-        """x with A x = b for the last factorised matrix (b is not modified)."""
+    def solve(self, b, check: bool = False):
+        """x with A x = b for the last factorised matrix (b is not modified).
+        check=True forms this solve's FP64 residual whatever the policy
+        (refined or handed to the host factorisation like a first solve)."""
         torch = self.torch
         if self._A is None:
             raise RfLUError("factor() first")
@@ -856,7 +788,7 @@ class BandLU:
             return torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
         if self.precision == "f32":
             x = self._precondition(b)
-            if self._checked and not self._refine_all:
+            if self._checked and not self._refine_all and not check:
                 return x
             rn, bn = self._residual(self._A, x, b)
             if rn > self.refine_tol * bn:
@@ -867,15 +799,8 @@ class BandLU:
                 self._checked = True
                 return x
             self._checked = False  # the host factorisation below takes over
-        n1 = self.n1
-        b1 = torch.empty(n1, dtype=torch.float64, device=self.device)
-        b1[self.pu] = b[:n1]
-        y = self._solve11(b1)
-        x2 = (b[n1] - torch.dot(self.a21, y)) / self.s
-        x = torch.empty_like(b)
-        x[:n1] = (y - self.z * x2)[self.pu]
-        x[n1] = x2
-        if not self._checked:
+        x = self._precondition(b)
+        if not self._checked or check:
             self._checked = True
             rn, bn = self._residual(self._A, x, b)
             if not (rn <= self.check_tol * max(bn, 1e-300)):
@@ -887,3 +812,101 @@ class BandLU:
                 self.host_fallbacks += 1
                 return torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
         return x
+
+
+class BandLU(_BorderedLU):
+    """Banded LU without pivoting on the GPU (libvef_band.so).
+
+    The node unknowns are ordered by reverse Cuthill-McKee on the node graph
+    (6 unknowns per node, consecutive), which on the cylinder meshes gives a
+    half-bandwidth of about 6 x 254 unknowns at 1,600 elements; the dense
+    arclength row and column form a border.  Each factorisation scatters the
+    node block of the matrix into the band and factorises it in 64-column
+    steps (a one-wavefront diagonal LU, thread-parallel triangular solves for
+    the panel and the diagonal blocks' inverses, a rocBLAS dgemm for the
+    trailing update — a dense, dependency-light schedule where the sparse
+    re-factorisation is bound by its 3,097-level elimination chain), then
+    solves A11 z = a12 and keeps the Schur complement s = a22 - a21.z.  A
+    solve is x2 = (b2 - a21.A11^-1 b1)/s, x1 = A11^-1 b1 - z x2.  The first
+    solve after a factorisation is checked as in RfLU (host fallback with
+    threshold pivoting).  precision="f32" stores and factorises the band in
+    FP32 (sgemm panels and updates) with FP64 solve vectors and border: the
+    single-precision factorisation of an inexact Newton iteration.  With
+    FP32 storage the first solve after a factorisation also forms its FP64
+    residual (one sparse product), and so does every later one when that
+    first solve needed refining; above refine_tol (relative) a solve is
+    refined by restarted GMRES in FP64 with the FP32 factors as right
+    preconditioner — what keeps the FP32
+    factorisation usable where its plain solves lose too much accuracy (large
+    meshes, whose condition number grows)."""
+
+    NB = 64
+
+    def __init__(self, rowptr: np.ndarray, col: np.ndarray, device="cuda:0", check_tol: float = None,
+                 precision: str = "f64", refine_tol: float = 1.0e-3):
+        import scipy.sparse as sp
+        import torch
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+        n1 = len(rowptr) - 2
+        if n1 % 6 != 0:
+            raise RfLUError("BandLU expects 6 unknowns per node and one arclength unknown")
+        nn = n1 // 6
+        rowptr = np.asarray(rowptr, dtype=np.int64)
+        rows = np.repeat(np.arange(n1 + 1, dtype=np.int64), np.diff(rowptr))
+        cols = np.asarray(col, dtype=np.int64)
+        inner = (rows < n1) & (cols < n1)
+        G = sp.csr_matrix((np.ones(int(inner.sum())), (rows[inner] // 6, cols[inner] // 6)), shape=(nn, nn))
+        order = reverse_cuthill_mckee(G, symmetric_mode=True).astype(np.int64)
+        pos = np.empty(nn, dtype=np.int64)
+        pos[order] = np.arange(nn)
+        pu = pos[np.arange(n1) // 6] * 6 + np.arange(n1) % 6   # new index of each node unknown
+        self._border_setup(rowptr, col, device, precision, check_tol, refine_tol, pu)
+        pi, pj = pu[rows[inner]], pu[cols[inner]]
+        self.w = int(np.abs(pi - pj).max()) if len(pi) else 0
+        L = _band_lib()
+        self.h = ctypes.c_void_p()
+        bits = 64 if precision == "f64" else 32
+        if L.vef_band_create_prec(n1, self.w, self.NB, bits, ctypes.byref(self.h)) != 0:
+            raise RfLUError("vef_band_create_prec failed (device memory?)")
+        ld, K = ctypes.c_int64(), ctypes.c_int32()
+        self._err(L.vef_band_storage(self.h, None, ctypes.byref(ld), ctypes.byref(K)), "vef_band_storage")
+        self.ld, self.K = ld.value, K.value
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int64)).to(device)  # noqa: E731
+        self.src_in = dev(np.nonzero(inner)[0])
+        self.dst_in = dev((self.K + pi - pj) + pj * self.ld)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _band_lib().vef_band_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, rc, what):
+        if rc != 0:
+            raise RfLUError(f"{what}: {_band_lib().vef_band_last_error(self.h).decode()}")
+
+    @property
+    def uses_graph(self) -> bool:
+        """True once the factorisation replays its captured HIP graph."""
+        return bool(_band_lib().vef_band_uses_graph(self.h))
+
+    def _solve11(self, y):
+        """A11^-1 y in the band order (y is consumed)."""
+        x = self.torch.empty_like(y)
+        self._err(_band_lib().vef_band_solve(self.h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                             self._stream()), "vef_band_solve")
+        return x
+
+    def _factor11(self, values):
+        L = _band_lib()
+        st = self._stream()
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        self._err(L.vef_band_set(self.h, int(self.src_in.numel()), p(self.src_in), p(self.dst_in), p(values), st),
+                  "vef_band_set")
+        self._err(L.vef_band_factor(self.h, st), "vef_band_factor")

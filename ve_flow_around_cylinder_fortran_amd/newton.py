@@ -150,16 +150,23 @@ class NewtonSolver:
         self.device = device
         self.asm = Assembler(mesh, params)
         rowptr, col = self.asm.pattern()
-        classes = {"rf": RfLU, "block": BlockLU, "hybrid": HybridLU, "band": BandLU}
+        from .multifrontal import MultifrontalLU
+
+        classes = {"rf": RfLU, "block": BlockLU, "hybrid": HybridLU, "band": BandLU, "mf": MultifrontalLU}
         if factorization is None:  # the fastest for both precisions
             factorization = "band"
         if factorization not in classes:
             raise NewtonError("factorization must be 'rf' (scalar re-factorisation), 'block' (6x6 node blocks), "
-                              "'hybrid' (scalar re-factorisation in node-block order, block solves) or 'band' "
-                              "(banded LU in reverse Cuthill-McKee node order)")
-        lu_class = classes[factorization]
-        self.lu = lu_class(rowptr, col, device=device, precision=linear_precision,
-                           check_tol=1e-6 if (reuse_factors and linear_precision == "f64") else None)
+                              "'hybrid' (scalar re-factorisation in node-block order, block solves), 'band' "
+                              "(banded LU in reverse Cuthill-McKee node order) or 'mf' (multifrontal LU in "
+                              "nested-dissection order)")
+        check_tol = 1e-6 if (reuse_factors and linear_precision == "f64") else None
+        if factorization == "mf":
+            self.lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, device=device, precision=linear_precision,
+                                     check_tol=check_tol)
+        else:
+            self.lu = classes[factorization](rowptr, col, device=device, precision=linear_precision,
+                                             check_tol=check_tol)
         self.N = self.asm.N
         self.b = torch.empty(self.N, dtype=torch.float64, device=device)
         self.values = torch.empty(self.asm.nnz, dtype=torch.float64, device=device)
@@ -171,19 +178,31 @@ class NewtonSolver:
         self.factored = False
 
     def solve(self, x0, LN0: float, xF: float = 1.0, max_iter: int = NITER, increment: int = 1, dArN: float = 0.0,
-              x_old=None, tangent=None, dS: float = 0.0):
+              x_old=None, tangent=None, dS: float = 0.0, cor_norm_div: float = COR_NORM_DIV):
         """Newton from x0 (numpy or device tensor); returns (x as a device
         tensor, list of NewtonRecord).  Increments 1-2 pin EU at
         LN0 + (increment-1)*dArN; from increment 3 the pseudo-arclength row
         tangent.(x - x_old) = dS closes the system (x_old, tangent: device
         tensors).  The inflow values follow the current EU, as the reference
-        re-solves its inflow problem inside every arclength assembly."""
+        re-solves its inflow problem inside every arclength assembly.
+
+        Recovery (checkNewtonConvergence and emergencyNewtonRaphson,
+        src/Newton_Raphson_mod.F90:453-584): when ||s|| exceeds cor_norm_div
+        or the iteration count passes its cap, the solve restarts from x0 (the
+        step's prediction) with the relaxation xF halved, the cap raised to
+        50*max_iter, dS halved and a fresh Jacobian; below xF = 1e-2 it stops
+        with NewtonError.  A NaN correction stops at once, as the reference's
+        general stop does.  `self.dS_used` and `self.emergencies` report the
+        step length and the restarts of the last solve."""
         torch = self.torch
-        x = x0.clone() if torch.is_tensor(x0) else torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
+        x0 = x0.clone() if torch.is_tensor(x0) else torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
+        x = x0.clone()
         flag = "MNR" if (self.reuse_factors and self.factored) else "NRP"
         res_norm = cor_new = cor_old = 1.0
         it = 0
+        cap = max_iter
         hist: List[NewtonRecord] = []
+        self.emergencies = []
         last_EU = None
         while True:
             if cor_new < COR_NORM_EPS and res_norm < COR_NORM_EPS:
@@ -223,12 +242,29 @@ class NewtonSolver:
                         flag = "NRP"
             else:
                 flag = "NRP" if cor_new > cor_old else "MNR"
-            if cor_new > COR_NORM_DIV or it > max_iter:
-                raise NewtonError(f"no convergence after {it} iterations (||s|| = {cor_new:.3e})")
+            if cor_new > cor_norm_div or it > cap:
+                # emergencyNewtonRaphson: restart from the prediction, relaxed
+                why = "divergence" if cor_new > cor_norm_div else "iterations"
+                xF = 0.5 * xF
+                self.emergencies.append((len(hist), why, xF))
+                if xF < 1.0e-2:
+                    raise NewtonError(f"no convergence: relaxation factor {xF:g} below 1e-2 after "
+                                      f"{len(self.emergencies)} restarts (last: {why}, ||s|| = {cor_new:.3e})")
+                cap = 50 * max_iter
+                dS = 0.5 * dS
+                x = x0.clone()
+                it = 0
+                res_norm = cor_new = cor_old = 1.0
+                flag = "NRP"
+                last_EU = None
+                torch.cuda.synchronize()
+                hist[-1].seconds = time.perf_counter() - t0
+                continue
             cor_old = cor_new
             x = x - xF * s
             torch.cuda.synchronize()
             hist[-1].seconds = time.perf_counter() - t0
+        self.dS_used = dS
         return x, hist
 
     def residual_norm(self, x, LN0: float, increment: int, dArN: float = 0.0, x_old=None, tangent=None,
@@ -264,6 +300,8 @@ class ArclengthContinuation:
         self.x_old = None   # last converged state
         self.tangent = None
         self.predictor = predictor  # optional learned predictor (learned.LearnedPredictor)
+        self.max_iter = NITER
+        self.cor_norm_div = COR_NORM_DIV
         self.choices = []   # per step: "tangent" or "learned" for the next prediction
         self.residuals = []  # per learned proposal: (||b|| at the tangent prediction, at the proposal)
         self._pending = None  # (features, tangent prediction) of the last step
@@ -289,7 +327,9 @@ class ArclengthContinuation:
         k = self.increment
         predicted = self.x.clone()
         x, hist = self.s.solve(self.x, self.LN0, increment=k, dArN=self.dArN, x_old=self.x_old,
-                               tangent=self.tangent, dS=self.dS)
+                               tangent=self.tangent, dS=self.dS, max_iter=self.max_iter,
+                               cor_norm_div=self.cor_norm_div)
+        self.dS = self.s.dS_used  # halved by each emergency restart (dSo is global in the reference)
         if k > 1:
             if k == 2:
                 d = x - self.x_old
