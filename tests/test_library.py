@@ -64,8 +64,65 @@ def test_band_library_rejects_bad_arguments_without_a_gpu():
     assert lib.vef_band_set(None, 0, None, None, None, None) != 0
 
 
+def test_mf_library_exports_every_declared_symbol():
+    from ve_flow_around_cylinder_fortran_amd import multifrontal
+
+    lib = multifrontal._mf_lib()
+    names = declared_functions("vef_mf.h")
+    assert len(names) == 9
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_mf_library_rejects_bad_plans_without_a_gpu():
+    """The plan is checked on the host before any device allocation: front
+    sizes against their bucket, parents above their children, rows in range,
+    every unknown the pivot of exactly one front; NULL handles."""
+    from ve_flow_around_cylinder_fortran_amd import multifrontal
+
+    lib = multifrontal._mf_lib()
+    i32 = lambda *a: np.array(a, dtype=np.int32)  # noqa: E731
+    i64 = lambda *a: np.array(a, dtype=np.int64)  # noqa: E731
+
+    def plan(front_s=(6, 6), rows=None, parent=(1, -1), slot=(0, 0), bucket_h=(0, 1), n=12):
+        # two fronts in two buckets: a leaf (6 pivots, 6 updates) under a root (6 pivots)
+        arrs = dict(bh=i32(*bucket_h), bc=i32(1, 1), bs=i32(6, 6), bu=i32(6, 0), fs=i32(*front_s),
+                    fu=i32(6, 0), ro=i64(0, 12, 18), rows=i32(*(rows or list(range(12)) + list(range(6, 12)))),
+                    par=i32(*parent), slot=i32(*slot), eo=i64(0, 6, 6), em=i32(*range(6)))
+        vp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+        p = multifrontal._Plan(n, 2, vp(arrs["bh"]), vp(arrs["bc"]), vp(arrs["bs"]), vp(arrs["bu"]), 2,
+                               vp(arrs["fs"]), vp(arrs["fu"]), vp(arrs["ro"]), vp(arrs["rows"]), vp(arrs["par"]),
+                               vp(arrs["slot"]), vp(arrs["eo"]), vp(arrs["em"]))
+        return p, arrs
+
+    def create(p):
+        h = ctypes.c_void_p()
+        rc = lib.vef_mf_create(ctypes.byref(p), 64, ctypes.byref(h))
+        msg = lib.vef_mf_last_error(h).decode() if h.value else ""
+        lib.vef_mf_destroy(h)
+        return rc, msg
+
+    p, keep = plan(bucket_h=(1, 0))
+    assert create(p) == (-1, "plan: buckets not ordered by height")
+    p, keep = plan(front_s=(12, 6))
+    assert create(p) == (-1, "plan: front larger than its bucket")
+    p, keep = plan(rows=list(range(11)) + [12] + list(range(6, 12)))
+    assert create(p) == (-1, "plan: row out of range")
+    p, keep = plan(rows=list(range(12)) + list(range(0, 6)))
+    assert create(p) == (-1, "plan: an unknown is a pivot of two fronts")
+    p, keep = plan(parent=(0, -1))
+    assert create(p) == (-1, "plan: parent not above its child")
+    p, keep = plan()
+    h = ctypes.c_void_p()
+    assert lib.vef_mf_create(ctypes.byref(p), 16, ctypes.byref(h)) != 0 and not h.value
+    assert lib.vef_mf_factor(None, None) != 0
+    assert lib.vef_mf_solve(None, None, None, None) != 0
+    assert lib.vef_mf_set(None, 0, None, None, None) != 0
+    assert lib.vef_mf_zero_pivots(None, None) != 0
+
+
 def test_every_header_is_checked():
-    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["vef.h", "vef_band.h", "vef_mlp.h"]
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["vef.h", "vef_band.h", "vef_mf.h", "vef_mlp.h"]
 
 
 def test_fortran_binding_names_match_header():
