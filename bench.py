@@ -86,6 +86,22 @@ def parse():
     ap.add_argument("--newton-ven", type=float, default=0.0,
                     help="L-PTT epsilon or Giesekus alpha of --newton; other than Oldroyd-B, the inlet values come "
                          "from the fully developed inflow problem at each Wi")
+    ap.add_argument("--continuation", action="store_true",
+                    help="arclength continuation in Wi at full size (40,000 elements unless --elements) up to "
+                         "--to-wi, for the modes in --modes")
+    ap.add_argument("--to-wi", type=float, default=0.9)
+    ap.add_argument("--max-steps", type=int, default=80)
+    ap.add_argument("--modes", default="reference,fp32",
+                    help="comma-separated: reference (FP64 reference policy), fp32 (inexact Newton, FP32 "
+                         "Jacobian and factorisation), fp32_learned (and the learned predictor)")
+    ap.add_argument("--ds-max", type=float, default=50.0,
+                    help="largest arclength step (the reference's dS_max, src/Parameters_mod.F90:27: an absolute "
+                         "length in the state's 2-norm, which grows as the square root of the unknowns)")
+    ap.add_argument("--factorization", default="mf", choices=["mf", "band"],
+                    help="linear solver of --continuation: multifrontal (nested dissection) or banded LU")
+    ap.add_argument("--learned-epochs", type=int, default=50)
+    ap.add_argument("--learned-nodes", type=int, default=0,
+                    help="nodes sampled per training step of the learned predictor (0: all)")
     ap.add_argument("--stability", action="store_true",
                     help="linear stability assembly (A and B) of the 40,000-element mesh instead of the Newton "
                          "assembly")
@@ -240,6 +256,152 @@ def newton_bench(args):
         out["config"]["EU_cpu_last"] = float(states[-1][-1])
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def continuation_bench(args):
+    """Arclength continuation in Wi at the configuration's full size (the
+    40,000-element mesh by default), as loopArclength runs it
+    (src/Continuation_Arclength_mod.F90:62-128): two pinned steps (Wi = 0.1,
+    0.12), then pseudo-arclength steps until Wi reaches --to-wi (or
+    --max-steps).  Modes: "reference" = the reference's Newton policy in FP64
+    (a fresh forward-difference Jacobian at the start of every step, MNR
+    iterations on its factorisation), "fp32" = the inexact Newton with the
+    FP32 analytic Jacobian and FP32 factorisation carried across steps (FP64
+    residual, update and convergence test), "fp32_learned" = the same with the
+    learned predictor.  Linear solves: the multifrontal LU in nested-
+    dissection order.  Per step: Wi, Newton flags, seconds, drag Fd and peak
+    wall Txx; per mode, an FP64 reference-policy Newton solve at the mode's
+    final Wi from its final state gives the drag and wall-stress errors.
+    value = mean seconds per pseudo-arclength step of the first mode."""
+    import numpy as np
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd import Params, workload_mesh
+    from ve_flow_around_cylinder_fortran_amd.mesh import boundary_edges
+    from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation, NewtonSolver, initial_state
+    from ve_flow_around_cylinder_fortran_amd.post import peak_wall_stress
+
+    pset = dict(ReN=0.0, BvN=0.59, VeN=args.newton_ven, DfN=1.0e-3, WiN0=1.0, model=args.newton_model)
+    oldroyd_b = pset["VeN"] == 0.0 and pset["model"] == "L-PTT"
+    model_name = "Oldroyd-B" if oldroyd_b else (f"linear PTT eps={pset['VeN']}" if pset["model"] == "L-PTT"
+                                                else f"Giesekus alpha={pset['VeN']}")
+    n_el = 40000 if args.elements == N_ELEMENTS else args.elements
+    LN0, dArN = 0.1, 0.02
+    t_mesh = time.perf_counter()
+    mesh = workload_mesh(n_el)
+    x0 = initial_state(mesh, LN0)
+    edges = boundary_edges(mesh, "Cylinder")
+    print(f"[continuation] {n_el} elements, {model_name}: mesh {time.perf_counter() - t_mesh:.1f} s", flush=True)
+
+    def drag_and_wall(solver, x):
+        solver.asm.set_drag_edges(edges)
+        Fd = float(solver.asm.drag(x).item())
+        Pw = peak_wall_stress(mesh, solver.asm.nodal_stress(x, pset["WiN0"] * float(x[-1])).cpu().numpy())
+        return Fd, Pw
+
+    def run(mode):
+        learned = mode.endswith("learned")
+        fp32 = mode.startswith("fp32")
+        t_setup = time.perf_counter()
+        solver = NewtonSolver(mesh, Params(**pset), factorization=args.factorization, reuse_factors=fp32,
+                              linear_precision="f32" if fp32 else "f64", jacobian="ad32" if fp32 else "fd64")
+        predictor = None
+        if learned:
+            from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
+
+            predictor = LearnedPredictor(mesh, seed=0, epochs=args.learned_epochs, overlap=True,
+                                         sample_nodes=args.learned_nodes)
+        def progress(rec, n, xF):  # long solves (restarts) report as they go
+            if n > 12 and n % 5 == 0:
+                print(f"[continuation] {mode}   iteration {n}: {rec.flag} ||b|| {rec.res_norm:.2e} "
+                      f"||s|| {rec.cor_norm:.2e} xF {xF:g}", flush=True)
+
+        solver.on_iteration = progress
+        cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN, predictor=predictor, dS_max=args.ds_max)
+        cont.start(x0)
+        setup = time.perf_counter() - t_setup
+        steps = []
+        failure = None
+        x = None
+        for k in range(args.max_steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            try:
+                x, hist = cont.step()
+            except Exception as e:  # recorded with what the solver saw; the mode ends here
+                h = getattr(solver, "last_history", [])
+                vals = solver.values
+                failure = {"step": k + 1, "error": f"{type(e).__name__}: {e}",
+                           "newton": [(r.flag, r.res_norm, r.cor_norm) for r in h],
+                           "values_finite": bool(torch.isfinite(vals).all()),
+                           "b_finite": bool(torch.isfinite(solver.b).all()),
+                           "predictor_Wi": pset["WiN0"] * float(cont.x[-1]), "dS": cont.dS}
+                print(f"[continuation] {mode} step {k + 1} failed: {failure}", flush=True)
+                break
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            Fd, Pw = drag_and_wall(solver, x)
+            Wi = pset["WiN0"] * float(x[-1])
+            steps.append({"Wi": Wi, "seconds": dt, "flags": "".join(h.flag[0] for h in hist),
+                          "cor_norms": [h.cor_norm for h in hist],
+                          "newton_iterations": len(hist), "Fd": Fd, "peak_wall_Txx": Pw, "dS": cont.dS,
+                          "restarts": len(solver.emergencies), "lu_check_residual": solver.lu.last_residual,
+                          "refined_solves": solver.lu.refinements})
+            print(f"[continuation] {mode} step {k + 1}: Wi {Wi:.5f} {dt:.3f} s flags {steps[-1]['flags']} "
+                  f"Fd {Fd:.10f} lu residual {solver.lu.last_residual} refined {solver.lu.refinements}", flush=True)
+            if Wi >= args.to_wi:
+                break
+        # FP64 reference policy at the final Wi, from the final state
+        x = cont.x_old
+        ref = NewtonSolver(mesh, Params(**pset), factorization="mf")
+        xr, hr = ref.solve(x, float(x[-1]))
+        Fr, Pr = drag_and_wall(ref, xr)
+        arc = [s["seconds"] for s in steps[2:]]
+        out = {"s_per_step": float(np.mean(arc)) if arc else None, "steps": len(steps), "setup_seconds": setup,
+               "Wi_final": steps[-1]["Wi"], "total_seconds": float(sum(s["seconds"] for s in steps)),
+               "newton_iterations": int(sum(s["newton_iterations"] for s in steps)),
+               "factorizations": solver.lu.refactorizations, "refined_solves": solver.lu.refinements,
+               "host_fallbacks": solver.lu.host_fallbacks,
+               "Fd_final": steps[-1]["Fd"], "fp64_newton_iterations_at_final_Wi": len(hr),
+               "drag_rel_diff_vs_fp64_newton": abs(steps[-1]["Fd"] - Fr) / abs(Fr),
+               "peak_wall_Txx_rel_diff_vs_fp64_newton": abs(steps[-1]["peak_wall_Txx"] - Pr) / abs(Pr),
+               "per_step": steps, "failure": failure}
+        if learned:
+            out["choices"] = cont.choices
+        del solver, ref, cont
+        torch.cuda.empty_cache()
+        return out
+
+    modes = {m: run(m) for m in args.modes.split(",")}
+    first = next(iter(modes.values()))
+    for m in modes.values():
+        m["speedup_per_step_vs_first"] = first["s_per_step"] / m["s_per_step"] if m["s_per_step"] else None
+        m["speedup_total_vs_first"] = first["total_seconds"] / m["total_seconds"]
+    out = {
+        "metric": "s per converged arclength step",
+        "value": first["s_per_step"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": first["steps"] - 2,
+        "warmup": 2,
+        "ms_per_step": first["s_per_step"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cylinder-in-channel Q2 mesh, {mesh.n_elements} elements, {model_name} beta=0.59 Re=0 "
+                        "DfN=1e-3, " + ("" if oldroyd_b else "fully developed inflow solved at each Wi, ")
+                        + f"arclength continuation in Wi from {LN0} to {args.to_wi} (pinned steps {LN0}, "
+                        f"{LN0 + dArN}; dS0 = 1, dS_max = {args.ds_max:g}, eps_pr = 1e-3), Newton tolerance 1e-8 "
+                        f"on ||b|| and ||s||, {'multifrontal' if args.factorization == 'mf' else 'banded'} LU",
+            "elements": mesh.n_elements, "unknowns": mesh.n_unknowns,
+            "modes": modes,
+        },
+        "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -408,6 +570,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.continuation:
+        continuation_bench(args)
         return
     if args.newton:
         newton_bench(args)

@@ -162,10 +162,15 @@ class LearnedPredictor:
     proposal, the next proposal waits for the training)."""
 
     def __init__(self, mesh: Mesh, seed: int = 0, epochs: int = 200, lr: float = 2e-3, buffer: int = 4,
-                 device: str = "cuda:0", overlap: bool = False):
+                 device: str = "cuda:0", overlap: bool = False, sample_nodes: int = 0):
         import torch
 
         self.torch = torch
+        # sample_nodes > 0: each Adam step takes that many rows of the buffer,
+        # drawn uniformly (a seeded device generator), instead of all of them
+        self.sample_nodes = int(sample_nodes)
+        self._gen = torch.Generator(device=device)
+        self._gen.manual_seed(seed)
         self.mesh = mesh
         self.net = NodeMLP(N_FEATURES, 6, seed=seed, device=device)
         self.device = device
@@ -213,13 +218,24 @@ class LearnedPredictor:
         if self.overlap:
             self._untrained = True
             return
+        loss = self._train_round()
+        self._rounds += 1
+        self.losses.append(float(loss.item()))
+
+    def _train_round(self):
+        """`epochs` Adam steps on the buffer (or on row samples of it)."""
+        torch = self.torch
         Xa = torch.cat([d[0] for d in self.data]).contiguous()
         Ta = torch.cat([d[1] for d in self.data]).contiguous()
         loss = None
+        n = Xa.shape[0]
         for _ in range(self.epochs):
-            loss = self.net.adam_step(Xa, Ta, self.lr)
-        self._rounds += 1
-        self.losses.append(float(loss.item()))
+            if 0 < self.sample_nodes < n:
+                idx = torch.randint(0, n, (self.sample_nodes,), generator=self._gen, device=self.device)
+                loss = self.net.adam_step(Xa[idx].contiguous(), Ta[idx].contiguous(), self.lr)
+            else:
+                loss = self.net.adam_step(Xa, Ta, self.lr)
+        return loss
 
     def ready(self) -> bool:
         return self._rounds >= 1
@@ -243,12 +259,7 @@ class LearnedPredictor:
             for X, T in self.data:
                 X.record_stream(self._side)
                 T.record_stream(self._side)
-            Xa = torch.cat([d[0] for d in self.data]).contiguous()
-            Ta = torch.cat([d[1] for d in self.data]).contiguous()
-            loss = None
-            for _ in range(self.epochs):
-                loss = self.net.adam_step(Xa, Ta, self.lr)
-            self._loss = loss
+            self._loss = self._train_round()
             self._trained = torch.cuda.Event()
             self._trained.record(self._side)
         self._rounds += 1

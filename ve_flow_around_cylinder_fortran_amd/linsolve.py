@@ -207,7 +207,7 @@ class RfLU:
         r = torch.mv(A, x) - b
         return float(torch.linalg.vector_norm(r)), float(torch.linalg.vector_norm(b))
 
-    def solve(self, b):
+    def solve(self, b, check: bool = False):
         """x with A x = b for the last factorised matrix (b is not modified).
         The first solve after a factorisation checks ||A x - b|| <= check_tol
         ||b||; a failed check moves that matrix's solves to the host."""
@@ -470,7 +470,7 @@ class BlockLU:
         r = torch.mv(A, x) - b
         return float(torch.linalg.vector_norm(r)), float(torch.linalg.vector_norm(b))
 
-    def solve(self, b):
+    def solve(self, b, check: bool = False):
         """x with A x = b for the last factorised matrix (b is not modified)."""
         L = _sparse_lib()
         torch = self.torch
@@ -661,7 +661,17 @@ class _BorderedLU:
     the residual check of the first solve after a factorisation (a host
     factorisation with threshold pivoting takes over if it fails) and, for
     FP32 storage, FP64 residuals and GMRES refinement with the FP32 factors
-    as right preconditioner."""
+    as right preconditioner.  FP64 solves are checked by their normwise
+    backward error; one that fails is refined the same way with the FP64
+    factors before the host factorisation (only on systems up to
+    HOST_FALLBACK_MAX unknowns) is considered."""
+
+    # the host factorisation (threshold pivoting) only takes over on systems
+    # up to this size; larger ones raise when refinement cannot pass the check
+    HOST_FALLBACK_MAX = 200000
+    # GMRES(30) restarts per refined solve: a factorisation that needs more is
+    # too far from the matrix (the caller refactorises or restarts)
+    gmres_restarts = 3
 
     def _border_setup(self, rowptr, col, device, precision, check_tol, refine_tol, pu):
         import torch
@@ -671,7 +681,8 @@ class _BorderedLU:
         self.torch = torch
         self.device = device
         self.precision = precision
-        self.check_tol = (1e-10 if precision == "f64" else 1e-3) if check_tol is None else check_tol
+        # FP64: a normwise backward error bound; FP32: a relative residual bound
+        self.check_tol = (1e-12 if precision == "f64" else 1e-3) if check_tol is None else check_tol
         self.refine_tol = refine_tol
         self.refinements = 0      # solves refined by GMRES
         self.gmres_iterations = 0
@@ -702,10 +713,15 @@ class _BorderedLU:
         self._checked = False
         self._refine_all = False
         self._host_lu = None
+        self.last_residual = None
         return rows, cols
 
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def zero_pivot_note(self) -> str:
+        zp = getattr(self, "zero_pivots", None)
+        return f"{zp()} fronts with a zero pivot" if zp else "factorisation"
 
     def factor(self, values):
         """Factorises the matrix with the given device values (same pattern)."""
@@ -715,6 +731,7 @@ class _BorderedLU:
         self._refine_all = False
         self._host_lu = None
         self._factor11(values)
+        self._Anorm = float(torch.linalg.vector_norm(values))  # Frobenius norm, for the backward error
         self.a12.zero_()
         self.a12[self.dst_c] = values[self.src_c]
         self.a21.zero_()
@@ -737,7 +754,7 @@ class _BorderedLU:
         for _ in range(restarts):
             r = b - torch.mv(A, x)
             beta = float(torch.linalg.vector_norm(r))
-            if beta <= tol * bn:
+            if beta <= tol * bn or not np.isfinite(beta):
                 return x, beta
             V = [r / beta]
             Z = []
@@ -754,13 +771,20 @@ class _BorderedLU:
                     w = w - H[i, j] * V[i]
                 H[j + 1, j] = float(torch.linalg.vector_norm(w))
                 self.gmres_iterations += 1
+                if not np.all(np.isfinite(H[:j + 2, :j + 1])):
+                    return x, beta
                 y = np.linalg.lstsq(H[:j + 2, :j + 1], g[:j + 2], rcond=None)[0]
                 est = float(np.linalg.norm(g[:j + 2] - H[:j + 2, :j + 1] @ y))
                 if est <= tol * bn or H[j + 1, j] == 0.0:
                     break
                 V.append(w / H[j + 1, j])
+            xn = x
             for i, yi in enumerate(y):
-                x = x + float(yi) * Z[i]
+                xn = xn + float(yi) * Z[i]
+            rn = float(torch.linalg.vector_norm(b - torch.mv(A, xn)))
+            if rn >= beta:  # no progress (the residual is at rounding level): keep the better iterate
+                return x, beta
+            x = xn
         r = b - torch.mv(A, x)
         return x, float(torch.linalg.vector_norm(r))
 
@@ -791,27 +815,56 @@ class _BorderedLU:
             if self._checked and not self._refine_all and not check:
                 return x
             rn, bn = self._residual(self._A, x, b)
+            self.last_residual = rn / max(bn, 1e-300)
+            if not np.isfinite(rn):
+                raise RfLUError(f"non-finite solve ({self.zero_pivot_note()})")
             if rn > self.refine_tol * bn:
                 self.refinements += 1
                 self._refine_all = True  # this factorisation's solves are checked from now on
-                x, rn = self._gmres(b, x, bn, self.refine_tol)
-            if rn <= self.check_tol * max(bn, 1e-300):
+                x, rn = self._gmres(b, x, bn, self.refine_tol, restarts=self.gmres_restarts)
+            if rn <= self.check_tol * max(bn, 1e-300) or self.n > self.HOST_FALLBACK_MAX:
+                # above the bound on a large system: the best iterate (the
+                # inexact Newton refactorises when its corrections grow)
                 self._checked = True
                 return x
-            self._checked = False  # the host factorisation below takes over
+            return self._host_solve(b)
         x = self._precondition(b)
-        if not self._checked or check:
+        if not self._checked or check or self._refine_all:
+            # FP64: the normwise backward error ||r|| / (||A||_F ||x|| + ||b||)
+            # (a backward-stable solve reaches eps times the factors' growth,
+            # whatever the conditioning); above check_tol the solve is
+            # refined by FP64 GMRES with this factorisation as preconditioner
+            # and the factorisation's later solves are checked too
             self._checked = True
             rn, bn = self._residual(self._A, x, b)
-            if not (rn <= self.check_tol * max(bn, 1e-300)):
-                import scipy.sparse as sp
-                import scipy.sparse.linalg as sla
-
-                A = sp.csr_matrix((self._A.cpu().numpy(), self.col_h, self.rowptr_h), shape=(self.n, self.n))
-                self._host_lu = sla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A", options=dict(DiagPivotThresh=0.01))
-                self.host_fallbacks += 1
-                return torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
+            scale = self._Anorm * float(torch.linalg.vector_norm(x)) + bn
+            self.last_residual = rn / max(scale, 1e-300)
+            if not np.isfinite(rn):
+                raise RfLUError(f"non-finite solve ({self.zero_pivot_note()})")
+            if not (rn <= self.check_tol * scale):
+                self.refinements += 1
+                self._refine_all = True
+                xr, rr = self._gmres(b, x, bn, self.check_tol * scale / max(bn, 1e-300), restarts=self.gmres_restarts)
+                if rr < rn:
+                    x, rn = xr, rr
+                if rn <= self.check_tol * scale:
+                    return x
+                if self.n > self.HOST_FALLBACK_MAX:
+                    raise RfLUError(f"solve backward error {rn / max(scale, 1e-300):.3e} above {self.check_tol:.1e} "
+                                    f"after refinement (first solve: {self.last_residual:.3e})")
+                return self._host_solve(b)
         return x
+
+    def _host_solve(self, b):
+        """Threshold-pivoting LU on the host (SuperLU) for this factorisation's
+        matrix, used for the rest of its solves."""
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as sla
+
+        A = sp.csr_matrix((self._A.cpu().numpy(), self.col_h, self.rowptr_h), shape=(self.n, self.n))
+        self._host_lu = sla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A", options=dict(DiagPivotThresh=0.01))
+        self.host_fallbacks += 1
+        return self.torch.from_numpy(self._host_lu.solve(b.cpu().numpy())).to(self.device)
 
 
 class BandLU(_BorderedLU):

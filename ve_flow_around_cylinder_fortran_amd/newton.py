@@ -28,6 +28,7 @@ from typing import List
 import numpy as np
 
 from .assembly import Assembler, Params
+from .linsolve import RfLUError
 from .mesh import MESH_TOLERANCE, Mesh
 
 COR_NORM_EPS = 1.0e-8
@@ -176,6 +177,8 @@ class NewtonSolver:
         self.reuse_factors = reuse_factors
         self.reuse_max_iters = reuse_max_iters
         self.factored = False
+        self.emergencies = []
+        self.on_iteration = None  # optional callback(record, records in this solve, xF) per iteration
 
     def solve(self, x0, LN0: float, xF: float = 1.0, max_iter: int = NITER, increment: int = 1, dArN: float = 0.0,
               x_old=None, tangent=None, dS: float = 0.0, cor_norm_div: float = COR_NORM_DIV):
@@ -191,8 +194,9 @@ class NewtonSolver:
         or the iteration count passes its cap, the solve restarts from x0 (the
         step's prediction) with the relaxation xF halved, the cap raised to
         50*max_iter, dS halved and a fresh Jacobian; below xF = 1e-2 it stops
-        with NewtonError.  A NaN correction stops at once, as the reference's
-        general stop does.  `self.dS_used` and `self.emergencies` report the
+        with NewtonError.  A factorisation that cannot solve the system (zero
+        pivots) or a non-finite correction counts as divergence; a non-finite
+        residual stops at once, as the reference's general stop on NaN does.  `self.dS_used` and `self.emergencies` report the
         step length and the restarts of the last solve."""
         torch = self.torch
         x0 = x0.clone() if torch.is_tensor(x0) else torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
@@ -222,14 +226,25 @@ class NewtonSolver:
                                x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
             self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
             res_norm = float(torch.linalg.vector_norm(self.b))
-            if jac:
-                self.lu.factor(self.values)
-                self.factored = True
-            s = self.lu.solve(self.b)
-            cor_new = float(torch.linalg.vector_norm(s))
-            if cor_new != cor_new:
-                raise NewtonError("NaN correction")
+            if not math.isfinite(res_norm):
+                raise NewtonError("non-finite residual")  # the reference's general stop on NaN
+            try:
+                if jac:
+                    self.lu.factor(self.values)
+                    self.factored = True
+                s = self.lu.solve(self.b)
+                cor_new = float(torch.linalg.vector_norm(s))
+            except RfLUError:
+                # a Jacobian this factorisation cannot solve (zero pivots at a
+                # wandering iterate): under MUMPS's static pivoting the
+                # reference sees a huge correction, i.e. divergence
+                s, cor_new = None, math.inf
+            if not math.isfinite(cor_new):
+                cor_new = math.inf
             hist.append(NewtonRecord(it, flag, res_norm, cor_new, 0.0))
+            self.last_history = hist
+            if self.on_iteration is not None:
+                self.on_iteration(hist[-1], len(hist), xF)
             if flag == "NRP":
                 flag = "MNR" if cor_new < MNR_EPS else "NRP"
             elif self.reuse_factors:
@@ -243,6 +258,8 @@ class NewtonSolver:
             else:
                 flag = "NRP" if cor_new > cor_old else "MNR"
             if cor_new > cor_norm_div or it > cap:
+                if s is None or not math.isfinite(cor_new):
+                    self.factored = False  # no usable factorisation carried over
                 # emergencyNewtonRaphson: restart from the prediction, relaxed
                 why = "divergence" if cor_new > cor_norm_div else "iterations"
                 xF = 0.5 * xF
@@ -337,7 +354,9 @@ class ArclengthContinuation:
             else:
                 e = torch.zeros(self.s.N, dtype=torch.float64, device=self.s.device)
                 e[-1] = 1.0
-                t = self.s.lu.solve(e)
+                # checked (and refined) whatever the factorisation's policy: a
+                # carried FP32 factorisation is not otherwise checked here
+                t = self.s.lu.solve(e, check=True)
                 self.tangent = t / torch.linalg.vector_norm(t)
             err = self._field_errors(x, predicted)
             trns = math.sqrt(self.eps_pr / err) if err > 0 else 2.0
