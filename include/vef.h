@@ -127,6 +127,18 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
                   int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
                   void* hip_stream);
 
+/* As vef_arclength, allocating nothing and using no scratch of the handle:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The partial
+ * sums of the pseudo-arclength dot product go to a file-scope device array of
+ * 6 x 4096 doubles (192 KiB per GPU, kept until the process ends), so meshes
+ * of at most 16,777,216 nodes (larger ones: VEF_ERR_UNSUPPORTED).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads) race.  Results equal vef_arclength's bit for bit. */
+int vef_arclength_noalloc(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
+                          int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b,
+                          double* values, void* hip_stream);
+
 /* Dirichlet rows, applied after the assembly and the arclength row
  * (applyDirichletBCFullyDeveloped and applyDirichletBC in assemblePetscObjects,
  * src/Dirichlet_mod.F90:104-190, src/Newton_Raphson_mod.F90:384-394).

@@ -45,6 +45,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1671,6 +1672,8 @@ __global__ void vef_arc_pin_rhs_kernel(int64_t N, const double* __restrict__ x, 
 }
 
 constexpr int kArcChunk = 4096;  // nodes per partial sum
+// Largest number of node chunks vef_arclength_noalloc handles (16,777,216 nodes).
+constexpr int kArcScratchBlocks = 4096;
 
 // Per-field partial sums of t*(x - xo) over fixed node chunks; fixed
 // reduction tree, so the result is deterministic.
@@ -1721,9 +1724,39 @@ __global__ void vef_fill_kernel(double* __restrict__ p, int64_t n) {
 
 }  // namespace
 
+// Scratch of vef_arclength_noalloc: the per-field partial sums of the
+// pseudo-arclength dot product (6 * kArcScratchBlocks doubles = 192 KiB), one
+// copy per GPU the library runs on, held until the process ends.  Host code
+// never passes `scratch` itself to a kernel: it uses the device address that
+// vef_create resolves once per device (g_scratch below).
+static __device__ __align__(256) double scratch[6 * kArcScratchBlocks];
+
 // =============================================================================
 // Host side
 // =============================================================================
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};   // device address of `scratch`, per device
+bool g_scratch_done[kMaxDevices] = {};
+
+// Resolves the current device's address of `scratch` once (the caller has
+// that device current).  nullptr if it cannot.
+double* scratch_address(int* device) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  *device = dev;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_done[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch[dev] = (double*)ptr;
+    else (void)hipGetLastError();
+    g_scratch_done[dev] = true;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
 struct vef_handle {
   int32_t n_nodes = 0, n_el = 0, n_edges = 0;
   int64_t N = 0, nnz = 0;
@@ -1779,6 +1812,8 @@ struct vef_handle {
   double* d_drag_terms = nullptr;   // term_el per edge
   int32_t n_drag_edges = 0;
   int arc_blocks = 0;
+  int device = -1;                   // the device vef_create ran on
+  double* arc_scratch = nullptr;     // that device's address of `scratch` (vef_arclength_noalloc)
   double* d_tmp_x = nullptr;  // host-assemble staging
   double* d_tmp_b = nullptr;
   double* d_tmp_v = nullptr;
@@ -2119,6 +2154,7 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   UP(h->d_ebank, (const double*)nullptr, (size_t)std::max(n_edges, 1) * 18 * kNeval);
   h->arc_blocks = (int)((n_nodes + kArcChunk - 1) / kArcChunk);
   UP(h->d_arc, (const double*)nullptr, (size_t)6 * h->arc_blocks);
+  h->arc_scratch = scratch_address(&h->device);
 #undef UP
   if (upload_tables(h)) {
     g_create_err = h->err;
@@ -2249,10 +2285,9 @@ int vef_profile_read(vef_handle* h, double* bulk_ms, double* scatter_ms, int64_t
   return VEF_OK;
 }
 
-int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
-                  int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
-                  void* stream) {
-  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+static int arclength_impl(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
+                          int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b,
+                          double* values, void* stream, double* partials) {
   if (!x || !b) return set_err(h, VEF_ERR_ARG, "x and b are required");
   if (flag != VEF_RES_ONLY && flag != VEF_RES_JAC && flag != VEF_RES_JAC_FP32)
     return set_err(h, VEF_ERR_ARG, "bad flag");
@@ -2272,9 +2307,9 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
   if (mode != VEF_ARC_PSEUDO) return set_err(h, VEF_ERR_ARG, "mode must be VEF_ARC_PIN or VEF_ARC_PSEUDO");
   if (!x_old || !tangent) return set_err(h, VEF_ERR_ARG, "x_old and tangent are required for VEF_ARC_PSEUDO");
   hipLaunchKernelGGL(vef_arc_dot_kernel, dim3(h->arc_blocks), dim3(256), 0, st, (int64_t)h->n_nodes, x, x_old,
-                     tangent, h->d_arc);
+                     tangent, partials);
   VEF_HIP(h, hipGetLastError());
-  hipLaunchKernelGGL(vef_arc_rhs_kernel, dim3(1), dim3(64), 0, st, N, h->arc_blocks, h->d_arc, x, x_old, tangent, dS,
+  hipLaunchKernelGGL(vef_arc_rhs_kernel, dim3(1), dim3(64), 0, st, N, h->arc_blocks, partials, x, x_old, tangent, dS,
                      b);
   VEF_HIP(h, hipGetLastError());
   if (flag == VEF_RES_JAC) {
@@ -2282,6 +2317,26 @@ int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_
     VEF_HIP(h, hipGetLastError());
   }
   return VEF_OK;
+}
+
+int vef_arclength(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
+                  int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b, double* values,
+                  void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  return arclength_impl(h, mode, x, x_old, tangent, increment, LN0, dArN, dS, flag, b, values, stream, h->d_arc);
+}
+
+int vef_arclength_noalloc(vef_handle* h, int32_t mode, const double* x, const double* x_old, const double* tangent,
+                          int32_t increment, double LN0, double dArN, double dS, int32_t flag, double* b,
+                          double* values, void* stream) {
+  if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
+  if (h->arc_blocks > kArcScratchBlocks)
+    return set_err(h, VEF_ERR_UNSUPPORTED, "vef_arclength_noalloc: more than 16,777,216 nodes (its scratch holds "
+                                           "4096 node chunks of 4096)");
+  if (!h->arc_scratch)
+    return set_err(h, VEF_ERR_HIP, "vef_arclength_noalloc: the scratch array's device address is unknown");
+  return arclength_impl(h, mode, x, x_old, tangent, increment, LN0, dArN, dS, flag, b, values, stream,
+                        h->arc_scratch);
 }
 
 int vef_set_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows, const double* values) {
