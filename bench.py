@@ -99,6 +99,11 @@ def parse():
                          "length in the state's 2-norm, which grows as the square root of the unknowns)")
     ap.add_argument("--factorization", default="mf", choices=["mf", "band"],
                     help="linear solver of --continuation: multifrontal (nested dissection) or banded LU")
+    ap.add_argument("--mode-seconds", type=float, default=600.0,
+                    help="wall-time budget of each --continuation mode (the mode stops at the next step)")
+    ap.add_argument("--check-every", type=int, default=10,
+                    help="--continuation: every this many steps, an FP64 Newton solve at the step's Wi checks the "
+                         "drag of the non-reference modes")
     ap.add_argument("--learned-epochs", type=int, default=50)
     ap.add_argument("--learned-nodes", type=int, default=0,
                     help="nodes sampled per training step of the learned predictor (0: all)")
@@ -324,7 +329,13 @@ def continuation_bench(args):
         steps = []
         failure = None
         x = None
+        checks = []
+        t_mode = time.perf_counter()
         for k in range(args.max_steps):
+            if time.perf_counter() - t_mode > args.mode_seconds:
+                failure = {"step": k + 1, "error": f"time budget of {args.mode_seconds:g} s spent"}
+                print(f"[continuation] {mode}: time budget spent at step {k + 1}", flush=True)
+                break
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             try:
@@ -350,6 +361,17 @@ def continuation_bench(args):
                           "refined_solves": solver.lu.refinements})
             print(f"[continuation] {mode} step {k + 1}: Wi {Wi:.5f} {dt:.3f} s flags {steps[-1]['flags']} "
                   f"Fd {Fd:.10f} lu residual {solver.lu.last_residual} refined {solver.lu.refinements}", flush=True)
+            if mode != "reference" and (k + 1) % args.check_every == 0:
+                # FP64 reference-policy Newton at this step's Wi from its state
+                if check_solver[0] is None:
+                    check_solver[0] = NewtonSolver(mesh, Params(**pset), factorization=args.factorization)
+                xr, hr = check_solver[0].solve(x, float(x[-1]))
+                Fr, Pr = drag_and_wall(check_solver[0], xr)
+                checks.append({"step": k + 1, "Wi": Wi, "Fd": Fd, "Fd_fp64": Fr,
+                               "drag_rel_diff": abs(Fd - Fr) / abs(Fr),
+                               "peak_wall_Txx_rel_diff": abs(Pw - Pr) / abs(Pr), "fp64_iterations": len(hr)})
+                print(f"[continuation] {mode} step {k + 1}: FP64 check |dFd|/Fd {checks[-1]['drag_rel_diff']:.2e} "
+                      f"({len(hr)} iterations)", flush=True)
             if Wi >= args.to_wi:
                 break
         # FP64 reference policy at the final Wi, from the final state
@@ -366,13 +388,15 @@ def continuation_bench(args):
                "Fd_final": steps[-1]["Fd"], "fp64_newton_iterations_at_final_Wi": len(hr),
                "drag_rel_diff_vs_fp64_newton": abs(steps[-1]["Fd"] - Fr) / abs(Fr),
                "peak_wall_Txx_rel_diff_vs_fp64_newton": abs(steps[-1]["peak_wall_Txx"] - Pr) / abs(Pr),
-               "per_step": steps, "failure": failure}
+               "per_step": steps, "failure": failure, "fp64_checks": checks,
+               "max_drag_rel_diff_fp64_checks": max([c["drag_rel_diff"] for c in checks], default=None)}
         if learned:
             out["choices"] = cont.choices
         del solver, ref, cont
         torch.cuda.empty_cache()
         return out
 
+    check_solver = [None]  # FP64 Newton for the periodic checks (built on first use, shared by the modes)
     modes = {m: run(m) for m in args.modes.split(",")}
     first = next(iter(modes.values()))
     for m in modes.values():
@@ -478,7 +502,7 @@ def stability_bench(args):
         "roofline": {
             "kernel": "vef_stab_prep_kernel + vef_stab_kernel (base state and test-node coefficients once per "
                       "element, then 54 forward differences per element, A and B)",
-            "bound": "mfma",
+            "bound": "fp64 valu",
             "ceiling": "FP64 compute (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; no FMA "
                        "contraction, so 0.5 of peak is its ceiling)",
             "achieved": achieved,
@@ -550,7 +574,10 @@ def cpu_baseline(mesh, x, target_s):
                       f"FD Jacobian + serial ADD_VALUES scatter, {dt:.1f} s"}
 
 
-def traffic_from_profiles(n_elements):
+def traffic_from_profiles(n_elements, key="hbm_bytes_per_launch"):
+    """HBM bytes from the committed counter profile of this workload
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's): the Jacobian
+    kernel's per launch, or with key="step_hbm_bytes" a whole NRP assembly's."""
     path = os.path.join(ROOT, "profiles", "vef_bulk_kernel_pmc_100k.json")
     if not os.path.exists(path):
         return None
@@ -558,7 +585,7 @@ def traffic_from_profiles(n_elements):
         d = json.load(f)
     if d.get("elements") != n_elements:
         return None
-    return d.get("hbm_bytes_per_launch")
+    return d.get(key)
 
 
 def main():
@@ -597,7 +624,6 @@ def main():
         asm.assemble(xd, "NRP", b=b, values=v)
     torch.cuda.synchronize()
 
-    asm.profile(True)
     stream = torch.cuda.current_stream()
     marks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
@@ -608,9 +634,14 @@ def main():
         marks[i + 1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    step_ms = sorted(marks[i].elapsed_time(marks[i + 1]) for i in range(args.steps))
+    # kernel durations from a separate, untimed pass (the library records HIP
+    # events around its kernels on the launch stream while profiling is on)
+    asm.profile(True)
+    for _ in range(args.steps):
+        asm.assemble(xd, "NRP", b=b, values=v)
     bulk_ms, scatter_ms, launches = asm.profile_read()
     asm.profile(False)
-    step_ms = sorted(marks[i].elapsed_time(marks[i + 1]) for i in range(args.steps))
     pct = lambda q: step_ms[min(len(step_ms) - 1, int(round(q * (len(step_ms) - 1))))]  # noqa: E731
 
     # residual-only assemblies (Flag_NR = 'MNR'), reported beside
@@ -625,13 +656,15 @@ def main():
     v32 = torch.empty_like(v)
     for _ in range(args.warmup):
         asm.assemble(xd, "NRP32", b=b, values=v32)
-    asm.profile(True)
     torch.cuda.synchronize()
     t4 = time.perf_counter()
     for _ in range(args.steps):
         asm.assemble(xd, "NRP32", b=b, values=v32)
     torch.cuda.synchronize()
     t5 = time.perf_counter()
+    asm.profile(True)
+    for _ in range(args.steps):
+        asm.assemble(xd, "NRP32", b=b, values=v32)
     jac32_ms, scatter32_ms, launches32 = asm.profile_read()
     asm.profile(False)
     rel32 = float(torch.linalg.vector_norm(v32 - v) / torch.linalg.vector_norm(v))
@@ -686,7 +719,7 @@ def main():
         "roofline": {
             "kernel": "vef_bulk_kernel<1,7,56> (the 55 perturbed element residuals and their quotients; "
                       "geometry from the residual pass)",
-            "bound": "mfma",
+            "bound": "fp64 valu",
             "ceiling": "FP64 compute (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s; "
                        "the kernel issues FP64 VALU ops without FMA contraction, so 0.5 of peak is its ceiling)",
             "achieved": achieved,
@@ -697,6 +730,10 @@ def main():
             "flops_per_launch": bulk_flops,
         },
     }
+    step_traffic = traffic_from_profiles(ne, "step_hbm_bytes")
+    if step_traffic:
+        out["config"]["step_hbm_bytes"] = step_traffic
+        out["config"]["step_hbm_tb_per_s"] = step_traffic / (ms_step * 1e-3) / 1e12
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, x, args.cpu_seconds)
     else:

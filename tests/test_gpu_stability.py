@@ -121,3 +121,48 @@ def test_leading_eigenvalues_of_a_converged_base_state(oracle):
     assert np.all(np.isfinite(lg))
     assert np.abs(lg - lo).max() <= 1e-9 * np.abs(lo).max()
     assert lg[0].real < 0.0
+
+
+def test_stability_40k_bitwise_and_leading_eigenvalues(oracle):
+    """The full-size stability case: the 40,000-element mesh, linear PTT
+    (eps = 0.25), base state at Wi = 0.6 (the smooth synthetic state), the
+    Dirichlet rows of Cylinder_Stability_2D.  A and B bit for bit against the
+    oracle on every row; the 10 eigenvalues nearest 0 (shift-invert Arnoldi on
+    the GPU multifrontal LU of A) of the GPU matrices against those of the
+    oracle's, within 1e-9 relative, and each GPU eigenpair's residual
+    ||A q - lambda B q|| small against ||A|| ||q||."""
+    from ve_flow_around_cylinder_fortran_amd.stability import leading_eigenvalues_gpu
+
+    import time
+
+    mesh = workload_mesh(40000)
+    pset = PARAM_SETS["lptt"]
+    x = smooth_state(mesh, EU=0.6)
+    t0 = time.perf_counter()
+    A, B = check_bitwise(oracle, mesh, pset, x, "40k", dirichlet=True)
+    print(f"[gpu stability] 40k: bitwise in {time.perf_counter() - t0:.1f} s", flush=True)
+    rowptr, col = oracle.stab_pattern(mesh)
+    Ad, Bd = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    t0 = time.perf_counter()
+    lam, V, lu, napply = leading_eigenvalues_gpu(Ad, Bd, rowptr, col, mesh, k=10, vectors=True, tol=1e-10)
+    print(f"[gpu stability] 40k: eigenvalues {lam[:4]} ... in {time.perf_counter() - t0:.1f} s, {napply} operator "
+          f"applications, backward error {max(lu.backward_errors):.1e}", flush=True)
+    # the oracle's matrices (uploaded: the same solver path on the oracle-assembled values)
+    oA, oB = oracle.stab_assemble(oracle.params(**pset), mesh, x, rowptr, col, nthreads=8)
+    oracle.stab_dirichlet(rowptr, col, oA, oB, stability_dirichlet_rows(mesh))
+    lam_o, _, _, _ = leading_eigenvalues_gpu(torch.from_numpy(oA).cuda(), torch.from_numpy(oB).cuda(), rowptr, col,
+                                             mesh, k=10, lu=lu, tol=1e-10)
+    # as sets (nearest match), within 1e-9 relative
+    for lo in lam_o:
+        assert np.abs(lam - lo).min() <= 1e-9 * abs(lo), (lo, lam)
+    import scipy.sparse as sp
+
+    n = len(rowptr) - 1
+    # the GPU eigenpairs are eigenpairs of the oracle-assembled problem
+    mA = sp.csr_matrix((oA, col, rowptr), shape=(n, n))
+    mB = sp.csr_matrix((oB, col, rowptr), shape=(n, n))
+    normA = np.sqrt(np.sum(oA * oA))
+    for j in range(10):
+        q = V[:, j]
+        r = mA @ q - lam[j] * (mB @ q)
+        assert np.linalg.norm(r) <= 1e-8 * normA * np.linalg.norm(q), (j, lam[j])

@@ -440,6 +440,7 @@ int vef_band_create_prec(int64_t n, int32_t w, int32_t nb, int32_t bits, vef_ban
     rc = 1;
   if (!rc && hipMemset(h->AB, 0, band_bytes) != hipSuccess) rc = 1;
   if (!rc && rocblas_create_handle(&h->rb) != rocblas_status_success) rc = 1;
+  if (!rc && rocblas_set_atomics_mode(h->rb, rocblas_atomics_not_allowed) != rocblas_status_success) rc = 1;
   if (rc) {
     vef_band_destroy(h);
     return 1;

@@ -399,6 +399,10 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
       upload(h, &h->d_fs_tgt, fs_tgt) || upload(h, &h->d_fs_ptr, fs_ptr) || upload(h, &h->d_fs_v, fs_v))
     return bad(h->err);
   if (rocblas_create_handle(&h->rb) != rocblas_status_success) return bad("rocblas_create_handle failed");
+  // no atomic reductions inside rocBLAS/rocSOLVER: repeated factorisations and
+  // solves of the same values give the same bits
+  if (rocblas_set_atomics_mode(h->rb, rocblas_atomics_not_allowed) != rocblas_status_success)
+    return bad("rocblas_set_atomics_mode failed");
   *out = h;
   return 0;
 }

@@ -291,12 +291,13 @@ def build_plan(coords, conn, leaf: int = 4, slack: float = 1.3) -> MultifrontalP
 
 def value_map(plan: MultifrontalPlan, rowptr: np.ndarray, col: np.ndarray) -> np.ndarray:
     """dst[k] = position in the front storage of stored entry k of the CSR
-    pattern (node block), -1 for the arclength row and column.  The pattern
-    must hold full 6x6 node blocks (preAllocatePetsc's)."""
+    pattern (node block), -1 for the arclength row and column (a pattern of
+    plan.n rows has none).  The pattern must hold full 6x6 node blocks
+    (preAllocatePetsc's)."""
     rowptr = np.asarray(rowptr, dtype=np.int64)
     col = np.asarray(col, dtype=np.int64)
     n = len(rowptr) - 1
-    n1 = n - 1
+    n1 = n - 1 if n == plan.n + 1 else n   # with or without the arclength border
     nn = n1 // 6
     if n1 != plan.n:
         raise ValueError("pattern and plan sizes differ")
@@ -538,3 +539,78 @@ class MultifrontalLU(_border_base()):
         c = ctypes.c_int32()
         self._err(_mf_lib().vef_mf_zero_pivots(self.h, ctypes.byref(c)), "vef_mf_zero_pivots")
         return int(c.value)
+
+
+class NodeBlockLU:
+    """Multifrontal LU (FP64) of a square matrix on the node unknowns alone
+    (no arclength border), e.g. the shifted operator A - sigma B of the
+    linear stability problem.  factor(values), solve(b) with device vectors
+    in the pattern's order; the first solve after a factorisation is checked
+    by its normwise backward error."""
+
+    def __init__(self, rowptr, col, coords, conn, device="cuda:0", leaf: int = 4, plan: MultifrontalPlan = None):
+        import torch
+
+        from .linsolve import RfLUError
+
+        self.torch = torch
+        self.device = device
+        self.plan = plan if plan is not None else build_plan(coords, conn, leaf)
+        P = self.plan
+        self.n = len(rowptr) - 1
+        if self.n != P.n:
+            raise RfLUError("NodeBlockLU: the pattern and the mesh differ")
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)  # noqa: E731
+        keep = [i32(P.bucket_height), i32(P.bucket_count), i32(P.bucket_s), i32(P.bucket_u), i32(P.front_s),
+                i32(P.front_u), i64(P.rows_off), i32(P.rows), i32(P.parent), i32(P.slot), i64(P.ea_off),
+                i32(P.ea_map)]
+        ptr = [ctypes.c_void_p(a.ctypes.data) for a in keep]
+        cp = _Plan(P.n, len(P.bucket_count), ptr[0], ptr[1], ptr[2], ptr[3], P.n_fronts, ptr[4], ptr[5], ptr[6],
+                   ptr[7], ptr[8], ptr[9], ptr[10], ptr[11])
+        L = _mf_lib()
+        self.h = ctypes.c_void_p()
+        if L.vef_mf_create(ctypes.byref(cp), 64, ctypes.byref(self.h)) != 0:
+            msg = L.vef_mf_last_error(self.h).decode() if self.h else "bad arguments"
+            self.close()
+            raise RfLUError(f"vef_mf_create: {msg}")
+        self.dst = torch.from_numpy(value_map(P, rowptr, col)).to(device)
+        self.pu = torch.from_numpy(P.unknown_pos).to(device)
+        self.ptrA = torch.from_numpy(np.asarray(rowptr, dtype=np.int64)).to(device)
+        self.indA = torch.from_numpy(np.asarray(col, dtype=np.int64)).to(device)
+        self._A = None
+        self.backward_errors = []
+
+    close = MultifrontalLU.close
+    __del__ = MultifrontalLU.__del__
+    _err = MultifrontalLU._err
+    zero_pivots = MultifrontalLU.zero_pivots
+
+    def _stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def factor(self, values):
+        L = _mf_lib()
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        st = self._stream()
+        self._err(L.vef_mf_set(self.h, int(self.dst.numel()), p(self.dst), p(values), st), "vef_mf_set")
+        self._err(L.vef_mf_factor(self.h, st), "vef_mf_factor")
+        self._A = values
+        self._Anorm = float(self.torch.linalg.vector_norm(values))
+        self._checked = False
+
+    def solve(self, b):
+        torch = self.torch
+        y = torch.empty_like(b)
+        y[self.pu] = b
+        x = torch.empty_like(b)
+        self._err(_mf_lib().vef_mf_solve(self.h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                         self._stream()), "vef_mf_solve")
+        x = x[self.pu]
+        if not self._checked:
+            A = torch.sparse_csr_tensor(self.ptrA, self.indA, self._A, size=(self.n, self.n))
+            rn = float(torch.linalg.vector_norm(torch.mv(A, x) - b))
+            scale = self._Anorm * float(torch.linalg.vector_norm(x)) + float(torch.linalg.vector_norm(b))
+            self.backward_errors.append(rn / max(scale, 1e-300))
+            self._checked = True
+        return x

@@ -28,6 +28,7 @@ from typing import List
 import numpy as np
 
 from .assembly import Assembler, Params
+from .inflow import InflowError
 from .linsolve import RfLUError
 from .mesh import MESH_TOLERANCE, Mesh
 
@@ -196,8 +197,12 @@ class NewtonSolver:
         50*max_iter, dS halved and a fresh Jacobian; below xF = 1e-2 it stops
         with NewtonError.  A factorisation that cannot solve the system (zero
         pivots) or a non-finite correction counts as divergence; a non-finite
-        residual stops at once, as the reference's general stop on NaN does.  `self.dS_used` and `self.emergencies` report the
-        step length and the restarts of the last solve."""
+        residual stops at once, as the reference's general stop on NaN does,
+        and an iterate whose Wi has no fully developed inflow counts as
+        divergence.  With reuse_factors, a modified-Newton correction that
+        grows is not applied (the next iteration refactorises at the same
+        iterate).  `self.dS_used` and `self.emergencies` report the step
+        length and the restarts of the last solve."""
         torch = self.torch
         x0 = x0.clone() if torch.is_tensor(x0) else torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
         x = x0.clone()
@@ -217,46 +222,67 @@ class NewtonSolver:
                 flag = "NRP"
             jac = flag == "NRP"
             EU = float(x[-1])
+            inflow_failed = False
+            reject = False
             if EU != last_EU:
-                rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * EU, self.inflow)
-                self.asm.set_dirichlet(rows, vals)
-                last_EU = EU
-            self.asm.assemble(x, self.jac_flag if jac else flag, b=self.b, values=self.values if jac else None)
-            self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
-                               x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
-            self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
-            res_norm = float(torch.linalg.vector_norm(self.b))
-            if not math.isfinite(res_norm):
-                raise NewtonError("non-finite residual")  # the reference's general stop on NaN
-            try:
-                if jac:
-                    self.lu.factor(self.values)
-                    self.factored = True
-                s = self.lu.solve(self.b)
-                cor_new = float(torch.linalg.vector_norm(s))
-            except RfLUError:
-                # a Jacobian this factorisation cannot solve (zero pivots at a
-                # wandering iterate): under MUMPS's static pivoting the
-                # reference sees a huge correction, i.e. divergence
-                s, cor_new = None, math.inf
-            if not math.isfinite(cor_new):
+                try:
+                    rows, vals = cylinder_dirichlet(self.mesh, self.params.WiN0 * EU, self.inflow)
+                except InflowError:
+                    # no fully developed inflow at this iterate's Wi (a wild
+                    # iterate): divergence of this solve
+                    inflow_failed = True
+                else:
+                    self.asm.set_dirichlet(rows, vals)
+                    last_EU = EU
+            if inflow_failed:
+                hist.append(NewtonRecord(it, flag, math.nan, math.inf, 0.0))
                 cor_new = math.inf
-            hist.append(NewtonRecord(it, flag, res_norm, cor_new, 0.0))
-            self.last_history = hist
-            if self.on_iteration is not None:
-                self.on_iteration(hist[-1], len(hist), xF)
-            if flag == "NRP":
-                flag = "MNR" if cor_new < MNR_EPS else "NRP"
-            elif self.reuse_factors:
-                flag = "MNR"
-                if cor_new > cor_old:
-                    flag = "NRP"
-                elif cor_new > COR_NORM_EPS and cor_old > 0.0:
-                    r = cor_new / cor_old
-                    if r > 0.0 and math.log(COR_NORM_EPS / cor_new) / math.log(r) > self.reuse_max_iters:
-                        flag = "NRP"
+                s = None
             else:
-                flag = "NRP" if cor_new > cor_old else "MNR"
+                self.asm.assemble(x, self.jac_flag if jac else flag, b=self.b, values=self.values if jac else None)
+                self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
+                                   x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
+                self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
+                res_norm = float(torch.linalg.vector_norm(self.b))
+                if not math.isfinite(res_norm):
+                    raise NewtonError("non-finite residual")  # the reference's general stop on NaN
+                try:
+                    if jac:
+                        self.lu.factor(self.values)
+                        self.factored = True
+                    s = self.lu.solve(self.b)
+                    cor_new = float(torch.linalg.vector_norm(s))
+                except RfLUError:
+                    # a Jacobian this factorisation cannot solve (zero pivots at a
+                    # wandering iterate): under MUMPS's static pivoting the
+                    # reference sees a huge correction, i.e. divergence
+                    s, cor_new = None, math.inf
+                if not math.isfinite(cor_new):
+                    cor_new = math.inf
+                hist.append(NewtonRecord(it, flag, res_norm, cor_new, 0.0))
+                self.last_history = hist
+                if self.on_iteration is not None:
+                    self.on_iteration(hist[-1], len(hist), xF)
+                if flag == "NRP":
+                    flag = "MNR" if cor_new < MNR_EPS else "NRP"
+                elif self.reuse_factors:
+                    flag = "MNR"
+                    if cor_new > cor_old:
+                        flag = "NRP"
+                        # inexact Newton only: a growing correction from a carried
+                        # factorisation is not applied; the next iteration
+                        # refactorises at the same iterate
+                        reject = math.isfinite(cor_new)
+                    elif cor_new > COR_NORM_EPS and cor_old > 0.0:
+                        r = cor_new / cor_old
+                        if r > 0.0 and math.log(COR_NORM_EPS / cor_new) / math.log(r) > self.reuse_max_iters:
+                            flag = "NRP"
+                else:
+                    flag = "NRP" if cor_new > cor_old else "MNR"
+            if reject and not (cor_new > cor_norm_div or it > cap):
+                torch.cuda.synchronize()
+                hist[-1].seconds = time.perf_counter() - t0
+                continue
             if cor_new > cor_norm_div or it > cap:
                 if s is None or not math.isfinite(cor_new):
                     self.factored = False  # no usable factorisation carried over
@@ -375,7 +401,10 @@ class ArclengthContinuation:
                 cand = self.predictor.propose(X, self.x, self.tangent)
                 nxt = k + 1
                 r_t = self.s.residual_norm(self.x, self.LN0, nxt, self.dArN, self.x_old, self.tangent, self.dS)
-                r_l = self.s.residual_norm(cand, self.LN0, nxt, self.dArN, self.x_old, self.tangent, self.dS)
+                try:
+                    r_l = self.s.residual_norm(cand, self.LN0, nxt, self.dArN, self.x_old, self.tangent, self.dS)
+                except InflowError:  # the proposal's Wi has no fully developed inflow
+                    r_l = math.inf
                 self.residuals.append((r_t, r_l))
                 if r_l < r_t:
                     self.x = cand

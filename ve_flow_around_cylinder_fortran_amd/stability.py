@@ -4,9 +4,12 @@ converged base state (include/vef.h, vef_stability_assemble), as
 assembleSlepcObjects (src/Stability/Stability_mod.F90:97-199) builds them for
 the unchanged SLEPc shift-invert solve (solveEVP, :271-425).
 
-The eigen solve itself is not part of this engine; `leading_eigenvalues` is a
-host-side stand-in (ARPACK shift-invert through scipy) for callers without
-SLEPc, used by the tests and the bench to look at the spectrum.
+The eigen solve itself is not part of this engine (SLEPc's Krylov-Schur with
+shift-invert in the reference); `leading_eigenvalues` (SuperLU, small
+meshes) and `leading_eigenvalues_gpu` (the operator (A - sigma B)^-1 B with
+the GPU multifrontal LU, for the full-size meshes) drive ARPACK's
+shift-invert Arnoldi from scipy as stand-ins for callers without SLEPc, used
+by the tests and the bench to look at the spectrum.
 """
 from __future__ import annotations
 
@@ -72,3 +75,39 @@ def leading_eigenvalues(A, B, k: int = 10, sigma: float = 0.0, seed: int = 0):
     nu = sla.eigs(op, k=k, which="LM", v0=v0, return_eigenvectors=False, tol=1e-12)
     lam = sigma + 1.0 / nu
     return lam[np.argsort(-lam.real)]
+
+
+def leading_eigenvalues_gpu(A, B, rowptr, col, mesh, k: int = 10, sigma: float = 0.0, seed: int = 0,
+                            lu=None, vectors: bool = False, tol: float = 1e-12, ncv: int = 64):
+    """The k eigenvalues of A q = lambda B q nearest sigma, sorted by
+    decreasing real part: ARPACK's shift-invert Arnoldi (scipy) on
+    (A - sigma B)^-1 B, the factorisation and solves of A - sigma B on the GPU
+    (multifrontal LU in nested-dissection order) and B q as a GPU sparse
+    product.  A, B: device value tensors in the pattern (rowptr, col).
+    Returns (eigenvalues, eigenvectors or None, the LU, operator applications)."""
+    import scipy.sparse.linalg as sla
+    import torch
+
+    from .multifrontal import NodeBlockLU
+
+    n = len(rowptr) - 1
+    dev = A.device
+    if lu is None:
+        lu = NodeBlockLU(rowptr, col, mesh.coords, mesh.conn, device=str(dev))
+    lu.factor(A - sigma * B)
+    Bt = torch.sparse_csr_tensor(torch.from_numpy(np.asarray(rowptr, dtype=np.int64)).to(dev),
+                                 torch.from_numpy(np.asarray(col, dtype=np.int64)).to(dev), B, size=(n, n))
+    count = [0]
+
+    def matvec(v):
+        count[0] += 1
+        vd = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
+        return lu.solve(torch.mv(Bt, vd)).cpu().numpy()
+
+    op = sla.LinearOperator((n, n), matvec=matvec, dtype=np.float64)
+    v0 = np.random.default_rng(seed).standard_normal(n)
+    out = sla.eigs(op, k=k, which="LM", v0=v0, return_eigenvectors=vectors, tol=tol, ncv=ncv)
+    nu, V = (out if vectors else (out, None))
+    lam = sigma + 1.0 / nu
+    o = np.lexsort((lam.imag, -lam.real))  # decreasing real part, conjugates by imaginary part
+    return lam[o], (V[:, o] if V is not None else None), lu, count[0]
