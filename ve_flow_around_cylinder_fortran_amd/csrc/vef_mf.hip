@@ -5,7 +5,8 @@
 // The symbolic analysis (nested dissection, fronts, buckets, maps) is the
 // caller's; this library checks the plan, lays the padded fronts out bucket
 // by bucket, and runs the numeric phases as height-ordered sequences of
-// kernels and strided-batched rocSOLVER/rocBLAS calls:
+// kernels and strided-batched rocSOLVER/rocBLAS calls (the buckets of one
+// height spread over eight internal streams):
 //   set:    clear, identity on the padded pivots, scatter of the CSR values;
 //   factor: per height, extend-add of the children's update blocks into the
 //           parents (slot 0 children, then slot 1: a fixed order), then per
@@ -169,6 +170,15 @@ struct vef_mf {
   int32_t *d_fg_src = nullptr, *d_bg_src = nullptr, *d_bs_dst = nullptr, *d_fs_tgt = nullptr;
   std::vector<int64_t> fg_off, bg_off, bs_off, fs_off, fs_ptr_off;  // per height
   rocblas_handle rb = nullptr;
+  // Buckets of one height are independent: they run on kSide internal
+  // streams (default priority), each with its own rocBLAS handle, forked from
+  // and joined back to the caller's stream by events around each height.
+  static constexpr int kSide = 8;
+  hipStream_t side[kSide] = {};
+  rocblas_handle rbs[kSide] = {};
+  hipEvent_t ev_main = nullptr;
+  hipEvent_t ev_side[kSide] = {};
+  std::vector<int> b_stream;  // side stream of each bucket (balanced by flops within its height)
   int64_t bytes = 0;
   std::string err;
 };
@@ -197,6 +207,33 @@ static void release(vef_mf* h) {
     if (p) (void)hipFree(p);
   if (h->rb) rocblas_destroy_handle(h->rb);
   h->rb = nullptr;
+  for (int k = 0; k < vef_mf::kSide; k++) {
+    if (h->rbs[k]) rocblas_destroy_handle(h->rbs[k]);
+    if (h->ev_side[k]) (void)hipEventDestroy(h->ev_side[k]);
+    if (h->side[k]) (void)hipStreamDestroy(h->side[k]);
+    h->rbs[k] = nullptr;
+    h->ev_side[k] = nullptr;
+    h->side[k] = nullptr;
+  }
+  if (h->ev_main) (void)hipEventDestroy(h->ev_main);
+  h->ev_main = nullptr;
+}
+
+// The caller's stream st -> every side stream (at the start of a height).
+static int fork_streams(vef_mf* h, hipStream_t st) {
+  if (hipEventRecord(h->ev_main, st) != hipSuccess) return fail(h, "hipEventRecord failed");
+  for (int k = 0; k < vef_mf::kSide; k++)
+    if (hipStreamWaitEvent(h->side[k], h->ev_main, 0) != hipSuccess) return fail(h, "hipStreamWaitEvent failed");
+  return 0;
+}
+
+// Every side stream -> the caller's stream (at the end of a height).
+static int join_streams(vef_mf* h, hipStream_t st) {
+  for (int k = 0; k < vef_mf::kSide; k++) {
+    if (hipEventRecord(h->ev_side[k], h->side[k]) != hipSuccess) return fail(h, "hipEventRecord failed");
+    if (hipStreamWaitEvent(st, h->ev_side[k], 0) != hipSuccess) return fail(h, "hipStreamWaitEvent failed");
+  }
+  return 0;
 }
 
 // The exported functions take C linkage from their declarations in vef_mf.h.
@@ -403,6 +440,39 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
   // solves of the same values give the same bits
   if (rocblas_set_atomics_mode(h->rb, rocblas_atomics_not_allowed) != rocblas_status_success)
     return bad("rocblas_set_atomics_mode failed");
+  if (hipEventCreateWithFlags(&h->ev_main, hipEventDisableTiming) != hipSuccess) return bad("hipEventCreate failed");
+  for (int k = 0; k < vef_mf::kSide; k++) {
+    if (hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) != hipSuccess) return bad("hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&h->ev_side[k], hipEventDisableTiming) != hipSuccess)
+      return bad("hipEventCreate failed");
+    if (rocblas_create_handle(&h->rbs[k]) != rocblas_status_success ||
+        rocblas_set_stream(h->rbs[k], h->side[k]) != rocblas_status_success ||
+        rocblas_set_atomics_mode(h->rbs[k], rocblas_atomics_not_allowed) != rocblas_status_success)
+      return bad("rocblas handle of a side stream failed");
+  }
+  // buckets of each height onto the side streams, largest first to the least loaded
+  h->b_stream.assign(h->nb, 0);
+  for (int b0 = 0; b0 < h->nb;) {
+    int b1 = b0;
+    while (b1 < h->nb && h->b_h[b1] == h->b_h[b0]) b1++;
+    std::vector<std::pair<double, int>> cost;
+    for (int b = b0; b < b1; b++) {
+      double s = h->b_s[b], u = h->b_u[b];
+      cost.push_back({h->b_c[b] * (2.0 / 3.0 * s * s * s + 2.0 * s * s * u + 2.0 * s * u * u + 1e5), b});
+    }
+    std::sort(cost.begin(), cost.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& c) {
+      return a.first > c.first || (a.first == c.first && a.second < c.second);
+    });
+    double load[vef_mf::kSide] = {};
+    for (auto& cb : cost) {
+      int k = 0;
+      for (int j = 1; j < vef_mf::kSide; j++)
+        if (load[j] < load[k]) k = j;
+      h->b_stream[cb.second] = k;
+      load[k] += cb.first;
+    }
+    b0 = b1;
+  }
   *out = h;
   return 0;
 }
@@ -435,7 +505,6 @@ int vef_mf_set(vef_mf* h, int64_t count, const int64_t* dst, const double* value
 template <typename T>
 static int factor_t(vef_mf* h, hipStream_t st) {
   T* buf = (T*)h->buf;
-  if (rocblas_set_stream(h->rb, st) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
   const T one = T(1), mone = T(-1);
   int b = 0;
   for (int hh = 0; hh < h->H; hh++) {
@@ -448,23 +517,25 @@ static int factor_t(vef_mf* h, hipStream_t st) {
       }
     }
     if (hipGetLastError() != hipSuccess) return fail(h, "kernel launch failed (extend-add)");
+    if (fork_streams(h, st)) return -1;
     for (; b < h->nb && h->b_h[b] == hh; b++) {
       int s = h->b_s[b], u = h->b_u[b], c = h->b_c[b];
       if (s == 0) continue;
+      rocblas_handle rb = h->rbs[h->b_stream[b]];
       int m = s + u;
       rocblas_stride sA = (rocblas_stride)m * m;
       T* A = buf + h->b_base[b];
-      if (getrf_sb(h->rb, s, A, m, sA, h->ipiv + h->b_ipiv[b], s, h->info + h->b_first[b], c) !=
-          rocblas_status_success)
+      if (getrf_sb(rb, s, A, m, sA, h->ipiv + h->b_ipiv[b], s, h->info + h->b_first[b], c) != rocblas_status_success)
         return fail(h, "getrf failed");
       if (u == 0) continue;
-      if (getrs_sb(h->rb, s, u, A, m, sA, h->ipiv + h->b_ipiv[b], s, A + (int64_t)s * m, m, sA, c) !=
+      if (getrs_sb(rb, s, u, A, m, sA, h->ipiv + h->b_ipiv[b], s, A + (int64_t)s * m, m, sA, c) !=
           rocblas_status_success)
         return fail(h, "getrs failed");
-      if (gemm_sb(h->rb, u, u, s, &mone, A + s, m, sA, A + (int64_t)s * m, m, sA, &one, A + (int64_t)s * m + s, m, sA,
-                  c) != rocblas_status_success)
+      if (gemm_sb(rb, u, u, s, &mone, A + s, m, sA, A + (int64_t)s * m, m, sA, &one, A + (int64_t)s * m + s, m, sA, c) !=
+          rocblas_status_success)
         return fail(h, "gemm failed");
     }
+    if (join_streams(h, st)) return -1;
   }
   return 0;
 }
@@ -493,7 +564,6 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
   T* W = (T*)h->W;
   T* V = (T*)h->V;
   T* XU = (T*)h->XU;
-  if (rocblas_set_stream(h->rb, st) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
   const T one = T(1), zero = T(0), mone = T(-1);
   // forward
   int b = 0;
@@ -501,21 +571,22 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
     int64_t n = h->fg_off[hh + 1] - h->fg_off[hh];
     if (n) mf_gather<T><<<blocks_for(n, 256), 256, 0, st>>>(W, h->d_fg_dst + h->fg_off[hh], h->d_fg_src + h->fg_off[hh],
                                                            r, n);
-    int b0 = b;
+    if (fork_streams(h, st)) return -1;
     for (; b < h->nb && h->b_h[b] == hh; b++) {
       int s = h->b_s[b], u = h->b_u[b], c = h->b_c[b];
       if (s == 0) continue;
+      rocblas_handle rb = h->rbs[h->b_stream[b]];
       int m = s + u;
       rocblas_stride sA = (rocblas_stride)m * m;
       T* A = buf + h->b_base[b];
-      if (getrs_sb(h->rb, s, 1, A, m, sA, h->ipiv + h->b_ipiv[b], s, W + h->b_w[b], s, s, c) != rocblas_status_success)
+      if (getrs_sb(rb, s, 1, A, m, sA, h->ipiv + h->b_ipiv[b], s, W + h->b_w[b], s, s, c) != rocblas_status_success)
         return fail(h, "getrs failed (solve)");
       if (u == 0) continue;
-      if (gemv_sb(h->rb, u, s, &one, A + s, m, sA, W + h->b_w[b], s, &zero, V + h->b_v[b], u, c) !=
+      if (gemv_sb(rb, u, s, &one, A + s, m, sA, W + h->b_w[b], s, &zero, V + h->b_v[b], u, c) !=
           rocblas_status_success)
         return fail(h, "gemv failed (solve)");
     }
-    (void)b0;
+    if (join_streams(h, st)) return -1;
     int64_t nt = h->fs_off[hh + 1] - h->fs_off[hh];
     if (nt)
       mf_subtract<T><<<blocks_for(nt, 256), 256, 0, st>>>(r, h->d_fs_tgt + h->fs_off[hh], h->d_fs_ptr + h->fs_ptr_off[hh],
@@ -529,16 +600,19 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
     if (n)
       mf_gather<T><<<blocks_for(n, 256), 256, 0, st>>>(XU, h->d_bg_dst + h->bg_off[hh], h->d_bg_src + h->bg_off[hh], x,
                                                       n);
+    if (fork_streams(h, st)) return -1;
     for (; b >= 0 && h->b_h[b] == hh; b--) {
       int s = h->b_s[b], u = h->b_u[b], c = h->b_c[b];
       if (s == 0 || u == 0) continue;
+      rocblas_handle rb = h->rbs[h->b_stream[b]];
       int m = s + u;
       rocblas_stride sA = (rocblas_stride)m * m;
       T* A = buf + h->b_base[b];
-      if (gemv_sb(h->rb, s, u, &mone, A + (int64_t)s * m, m, sA, XU + h->b_v[b], u, &one, W + h->b_w[b], s, c) !=
+      if (gemv_sb(rb, s, u, &mone, A + (int64_t)s * m, m, sA, XU + h->b_v[b], u, &one, W + h->b_w[b], s, c) !=
           rocblas_status_success)
         return fail(h, "gemv failed (backward solve)");
     }
+    if (join_streams(h, st)) return -1;
     int64_t np = h->bs_off[hh + 1] - h->bs_off[hh];
     if (np)
       mf_put<T><<<blocks_for(np, 256), 256, 0, st>>>(x, h->d_bs_dst + h->bs_off[hh], h->d_bs_src + h->bs_off[hh], W,
