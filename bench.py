@@ -56,13 +56,14 @@ STAB_FLOPS_PER_EVAL = 35289
 STAB_FLOPS_BASE = 9261
 STAB_COEF_FLOPS = 79
 STAB_QUOTIENT_FLOPS = 2 * 2916 * 3
-# The FP32 analytic Jacobian's work per element (VEF_RES_JAC_FP32): 207 dual-
-# number evaluations of the Gauss-point terms (9 Gauss points x 23
+# The FP32 analytic Jacobian's work per element (VEF_RES_JAC_FP32): 171 dual-
+# number evaluations of the Gauss-point terms (9 Gauss points x 19
 # directions, AD32_DUAL_FLOPS each), the field interpolations, the WET
-# products and the two contractions with the basis functions.  Pinned by
+# products, forming E (37 operations per (k, q, m) and Gauss point) and the
+# outer contraction with the test functions.  Pinned by
 # tests/test_ad_jacobian_host.py.
-AD32_DUAL_FLOPS = 1649
-AD32_FLOPS_PER_ELEMENT = 207 * AD32_DUAL_FLOPS + 9 * 22 * 9 * 2 + 207 * 18 + 224532
+AD32_DUAL_FLOPS = 1666
+AD32_FLOPS_PER_ELEMENT = 171 * AD32_DUAL_FLOPS + 9 * 22 * 9 * 2 + 171 * 18 + 9 * (162 * 37 + (2916 + 54) * 6)
 PEAK_FP64_TFLOPS = 78.6           # MI355X FP64 (vector = matrix), FMA counted as 2
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 (vector = matrix), FMA counted as 2
 PEAK_HBM_GBS = 8000.0

@@ -13,6 +13,7 @@ inline FlopReal fabs(FlopReal a) { return fabsr(a); }
 
 namespace {
 using vefad::Dual;
+using vefad::DualV;
 using vefad::kNumFields;
 
 void fields(const double* u, const double* bf, const double* dFx, const double* dFy, const double* d2x,
@@ -26,11 +27,48 @@ void fields(const double* u, const double* bf, const double* dFx, const double* 
   }
   F[vefad::kEU] = EU;
 }
+
+// dT/dF_f for one field direction f, with the dual type the scheme uses.
+template <class D>
+void direction(const double* F, int f, const double* fx, const double* fy, double H2,
+               const vefad::Params<double>& prm, double dT[3][6]);
+template <>
+void direction<Dual<double>>(const double* F, int f, const double* fx, const double* fy, double H2,
+                             const vefad::Params<double>& prm, double dT[3][6]) {
+  Dual<double> Fd[kNumFields];
+  for (int i = 0; i < kNumFields; i++) Fd[i] = Dual<double>(F[i], i == f ? 1.0 : 0.0);
+  Dual<double> Td[3][6];
+  vefad::gauss_terms(Fd, fx, fy, H2, prm, Td);
+  for (int k = 0; k < 3; k++)
+    for (int q = 0; q < 6; q++) dT[k][q] = Td[k][q].d;
+}
+template <>
+void direction<DualV<double, 1>>(const double* F, int f, const double* fx, const double* fy, double H2,
+                                 const vefad::Params<double>& prm, double dT[3][6]) {
+  DualV<double, 1> Fd[kNumFields];
+  for (int i = 0; i < kNumFields; i++) {
+    Fd[i].v = F[i];
+    Fd[i].d[0] = i == f ? 1.0 : 0.0;
+  }
+  DualV<double, 1> Td[3][6];
+  vefad::gauss_terms(Fd, fx, fy, H2, prm, Td);
+  for (int k = 0; k < 3; k++)
+    for (int q = 0; q < 6; q++) dT[k][q] = Td[k][q].d[0];
+}
+
+bool second_derivative(int f) {
+  return f == vefad::kG2x0 || f == vefad::kG2y0 || f == vefad::kG2x1 || f == vefad::kG2y1;
+}
 }  // namespace
 
+// scheme 0: every one of the 23 directions with Dual<double>.
+// scheme 1: the GPU kernel's (vef_jac32_kernel): DualV<double, 1>, 19
+//   directions; the second derivatives of U_k take -BvN times the dP/dx_k
+//   direction (they enter T only through BvN * (d2U_k/dx2 + d2U_k/dy2) beside
+//   -dP/dx_k).
 extern "C" int ad_host_element(const double* prm5, int model, const double* u54, double EU, const double* bf,
                                const double* dFx, const double* dFy, const double* d2x, const double* d2y,
-                               const double* H2, const double* WET, double* R, double* J, double* Jeu) {
+                               const double* H2, const double* WET, double* R, double* J, double* Jeu, int scheme) {
   vefad::Params<double> prm{prm5[0], prm5[1], prm5[2], prm5[3], prm5[4], 1.0 - prm5[1], model};
   for (int i = 0; i < 54; i++) R[i] = 0.0, Jeu[i] = 0.0;
   for (int i = 0; i < 54 * 54; i++) J[i] = 0.0;
@@ -43,14 +81,23 @@ extern "C" int ad_host_element(const double* prm5, int model, const double* u54,
     for (int n = 0; n < 9; n++)
       for (int q = 0; q < 6; q++) R[n * 6 + q] += WET[g] * (b[n] * T[0][q] + fx[n] * T[1][q] + fy[n] * T[2][q]);
     const double* basis[5] = {b, fx, fy, gx, gy};
+    double D[kNumFields][3][6];
     for (int f = 0; f < kNumFields; f++) {
-      Dual<double> Fd[kNumFields];
-      for (int i = 0; i < kNumFields; i++) Fd[i] = Dual<double>(F[i], i == f ? 1.0 : 0.0);
-      Dual<double> Td[3][6];
-      vefad::gauss_terms(Fd, fx, fy, H2[g], prm, Td);
+      if (scheme == 0)
+        direction<Dual<double>>(F, f, fx, fy, H2[g], prm, D[f]);
+      else if (!second_derivative(f))
+        direction<DualV<double, 1>>(F, f, fx, fy, H2[g], prm, D[f]);
+    }
+    if (scheme != 0)
+      for (int k = 0; k < 3; k++)
+        for (int q = 0; q < 6; q++) {
+          D[vefad::kG2x0][k][q] = D[vefad::kG2y0][k][q] = -prm.BvN * D[vefad::kPx][k][q];
+          D[vefad::kG2x1][k][q] = D[vefad::kG2y1][k][q] = -prm.BvN * D[vefad::kPy][k][q];
+        }
+    for (int f = 0; f < kNumFields; f++)
       for (int n = 0; n < 9; n++)
         for (int q = 0; q < 6; q++) {
-          const double dr = WET[g] * (b[n] * Td[0][q].d + fx[n] * Td[1][q].d + fy[n] * Td[2][q].d);
+          const double dr = WET[g] * (b[n] * D[f][0][q] + fx[n] * D[f][1][q] + fy[n] * D[f][2][q]);
           if (f == vefad::kEU) {
             Jeu[n * 6 + q] += dr;
           } else {
@@ -58,27 +105,30 @@ extern "C" int ad_host_element(const double* prm5, int model, const double* u54,
             for (int m = 0; m < 9; m++) J[(m * 6 + p) * 54 + n * 6 + q] += dr * basis[k][m];
           }
         }
-    }
   }
   return 0;
 }
 
 // Operations of one dual-number evaluation of gauss_terms (one seeded
-// direction), counted with the oracle's instrumented scalar: out = add, mul,
-// div, sqrt, total.  Every operation the kernel executes is counted, also
-// those on zero derivative parts.
+// direction) with the kernel's dual type DualV<., 1>, counted with the
+// oracle's instrumented scalar: out = add, mul, div, sqrt, total.  Every
+// operation the kernel executes is counted, also those on zero derivative
+// parts.
 extern "C" int ad_count_gauss_terms(const double* prm5, int model, const double* F, const double* dFx,
                                     const double* dFy, double H2, int direction, unsigned long long* out) {
   vefad::Params<FlopReal> prm{FlopReal(prm5[0]), FlopReal(prm5[1]), FlopReal(prm5[2]), FlopReal(prm5[3]),
                               FlopReal(prm5[4]), FlopReal(1.0 - prm5[1]), model};
-  Dual<FlopReal> Fd[kNumFields];
-  for (int i = 0; i < kNumFields; i++) Fd[i] = Dual<FlopReal>(FlopReal(F[i]), FlopReal(i == direction ? 1.0 : 0.0));
+  DualV<FlopReal, 1> Fd[kNumFields];
+  for (int i = 0; i < kNumFields; i++) {
+    Fd[i].v = FlopReal(F[i]);
+    Fd[i].d[0] = FlopReal(i == direction ? 1.0 : 0.0);
+  }
   FlopReal fx[9], fy[9];
   for (int n = 0; n < 9; n++) {
     fx[n] = FlopReal(dFx[n]);
     fy[n] = FlopReal(dFy[n]);
   }
-  Dual<FlopReal> T[3][6];
+  DualV<FlopReal, 1> T[3][6];
   FlopCounter::reset();
   vefad::gauss_terms(Fd, fx, fy, FlopReal(H2), prm, T);
   out[0] = FlopCounter::add;

@@ -53,20 +53,6 @@
 #include "vef_stability_math.h"
 #include "vef_ad_math.h"
 
-// Directions per thread in the FP32 Jacobian's dual-number pass.  One is the
-// fastest at 100,000 elements on MI355X (4.00 ms): two took 7.29 ms (217 VGPRs,
-// 2 waves per SIMD) or 5.60 ms (3 waves, spilling), four 7.8-15 ms; the
-// registers of the extra tangents cost more occupancy than the shared primal
-// evaluation saves.
-#ifndef JAC32_DIRS
-#define JAC32_DIRS 1
-#endif
-// Minimum waves per SIMD the FP32 Jacobian kernel is compiled for (bounds its
-// VGPRs: 4 gives 128).
-#ifndef JAC32_MIN_WAVES
-#define JAC32_MIN_WAVES 4
-#endif
-
 namespace {
 
 constexpr double kEpsJac = 1.0e-8;  // perturbVariable, src/Tools_mod.F90:831-840
@@ -866,56 +852,75 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
 }
 
 // ----------------------------------------------------------------------------
-// Analytic FP32 Jacobian (VEF_RES_JAC_FP32): one 256-thread workgroup per
-// element.  (1) The 22 fields at the 9 Gauss points in FP32; (2) 207 lanes =
-// 9 Gauss points x 23 directions (the fields and EU) evaluate the Gauss-point
-// coefficients T(F) of the test functions (vef_ad_math.h) with one-component
-// dual numbers, giving D = WET * dT/dF in LDS; (3) every matrix entry is the
-// contraction sum_g sum_k phi^n_k(g) E_g[k][q,m,p] with
-// E_g[k][q,m,p] = sum_{f in p} D[g][f][k,q] beta^m_f(g): per Gauss point E_g is
-// formed in LDS (VALU), then the outer sum runs on the matrix cores as
-// J += Phi_g^T E_g, a 16 x 4 by 4 x 336 product (test nodes padded to 16, the
-// three basis kinds and a zero row, the 324 entries of a row block and the 6
-// of dR/dEU padded to 336) in 21 v_mfma_f32_16x16x4f32 tiles whose FP32
-// accumulators stay in registers over the 9 Gauss points; stored through the
-// same scatter map as the forward-difference pass, dR/dEU into acbank.
+// Analytic FP32 Jacobian (VEF_RES_JAC_FP32): one 192-thread workgroup (three
+// wavefronts) per element.
+//  (1) The 22 fields at the 9 Gauss points in FP32.
+//  (2) 171 lanes = 9 Gauss points x 19 directions evaluate the Gauss-point
+//      coefficients T(F) of the test functions (vef_ad_math.h) with
+//      one-component dual numbers, giving D = WET * dT/dF in LDS.  The four
+//      second derivatives of U are not differentiated: they enter T only
+//      through BvN * (d2U_k/dx2 + d2U_k/dy2) beside -dP/dx_k in the strong
+//      momentum residual, so dT/d(d2U_k/dx2) = dT/d(d2U_k/dy2) =
+//      -BvN * dT/d(dP/dx_k) exactly; the 19 directions are the other 18
+//      fields and EU.
+//  (3) Every matrix entry is sum_g sum_k phi^n_k(g) E_g[k][q,m,p] with
+//      E_g[k][q,m,p] = sum_{f in p} D[g][f][k,q] beta^m_f(g).  Per Gauss point
+//      162 threads, one per (k, q, m), form the six E_g[k][q,m,0..5] from 18 D
+//      values and the node's 5 basis values (branch-free, FMAs), into one of
+//      two LDS buffers; then the outer sum runs on the matrix cores as
+//      J += Phi_g^T E_g, a 16 x 4 by 4 x 336 product (test nodes padded to 16,
+//      the three basis kinds and a zero row, the 324 entries of a row block and
+//      the 6 of dR/dEU padded to 336) in 21 v_mfma_f32_16x16x4f32 tiles, 7 per
+//      wavefront, whose FP32 accumulators stay in registers over the 9 Gauss
+//      points.  With E double-buffered one barrier per Gauss point suffices:
+//      a wavefront's products from buffer g % 2 precede its arrival at the
+//      barrier after which buffer g % 2 is written again.
+//  Stored through the same scatter map as the forward-difference pass, dR/dEU
+//  into acbank.
 // ----------------------------------------------------------------------------
-// Field index of basis kind j (0 value, 1 d/dx, 2 d/dy, 3 d2/dx2, 4 d2/dy2) of
-// unknown p (-1 where the unknown has no such field).
-__host__ __device__ constexpr int field_of(int p, int j) {
-  return p == 0 ? (j == 0 ? vefad::kUx : j == 1 ? vefad::kGU00 : j == 2 ? vefad::kGU10 : j == 3 ? vefad::kG2x0 : vefad::kG2y0)
-       : p == 1 ? (j == 0 ? vefad::kUy : j == 1 ? vefad::kGU01 : j == 2 ? vefad::kGU11 : j == 3 ? vefad::kG2x1 : vefad::kG2y1)
-       : j > 2  ? -1
-       : p == 2 ? (j == 0 ? vefad::kP : j == 1 ? vefad::kPx : vefad::kPy)
-       : p == 3 ? (j == 0 ? vefad::kS00 : j == 1 ? vefad::kS00x : vefad::kS00y)
-       : p == 4 ? (j == 0 ? vefad::kS01 : j == 1 ? vefad::kS01x : vefad::kS01y)
-                : (j == 0 ? vefad::kS11 : j == 1 ? vefad::kS11x : vefad::kS11y);
-}
+namespace jac32 {
+constexpr int kThreads = 192;
+constexpr int kDirs = 19;                 // differentiated directions per Gauss point
+constexpr int kECols = 336;               // E columns: 324 entries (q*54 + m*6 + p), 6 of dR/dEU, padding
+constexpr int kTiles = kECols / 16;       // 21
+constexpr int kTilesPerWave = kTiles / (kThreads / 64);  // 7
+static_assert(kTiles % (kThreads / 64) == 0, "the E tiles split evenly over the wavefronts");
+static_assert(9 * kDirs <= kThreads, "one lane per (Gauss point, direction)");
+static_assert(18 * 9 + 18 <= kThreads, "one thread per (k, q, m) and per dR/dEU entry");
+// Field of direction d: the fields in their order without the four second
+// derivatives of U (kG2x0 .. kG2y1), then EU.
+__host__ __device__ constexpr int dir_field(int d) { return d < vefad::kG2x0 ? d : d + 4; }
+// Directions of the fields E needs (dir_field inverse).
+constexpr int dUx = 0, dUy = 1, dGU00 = 2, dGU10 = 3, dGU01 = 4, dGU11 = 5;
+constexpr int dP = vefad::kP - 4, dPx = vefad::kPx - 4, dPy = vefad::kPy - 4;
+constexpr int dS00 = vefad::kS00 - 4, dS01 = vefad::kS01 - 4, dS11 = vefad::kS11 - 4;
+constexpr int dS00x = vefad::kS00x - 4, dS00y = vefad::kS00y - 4, dS01x = vefad::kS01x - 4;
+constexpr int dS01y = vefad::kS01y - 4, dS11x = vefad::kS11x - 4, dS11y = vefad::kS11y - 4;
+constexpr int dEU = vefad::kEU - 4;
+static_assert(dir_field(dP) == vefad::kP && dir_field(dS11y) == vefad::kS11y && dir_field(dEU) == vefad::kEU &&
+              dEU == kDirs - 1, "direction numbering");
+}  // namespace jac32
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JAC32_MIN_WAVES, 8)))
-vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ conn, const double* __restrict__ x,
-                 int64_t eu_index, int32_t n_el, vefad::Params<float> prm, FdOut fo) {
+__global__ void __launch_bounds__(jac32::kThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
+vef_jac32_kernel(const int32_t* __restrict__ conn, const double* __restrict__ x, int64_t eu_index, int32_t n_el,
+                 vefad::Params<float> prm, FdOut fo) {
 #pragma clang fp contract(fast)  // FP32 path: compared to FP32 accuracy, FMAs allowed
+  using namespace jac32;
   constexpr int NF = vefad::kNumFields;
-  __shared__ double sU[54];
-  __shared__ double sX[18];
-  __shared__ Geo sG[9];
+  __shared__ float sU[54];
   __shared__ float sBas[9][5][9];  // [g][basis kind][node]
   __shared__ float sWET[9], sH2[9];
   __shared__ float sF[9][NF];
-  __shared__ float sD[9][NF][18];  // WET * dT[k][q]/dF_f
-  constexpr int ECOL = 336;        // columns of E: 324 entries (q*54 + m*6 + p), 6 of dR/dEU, padding
-  constexpr int NTILE = ECOL / 16;
-  __shared__ float sM[3 * ECOL];   // one Gauss point's E: [k][column]
+  __shared__ float sD[9][kDirs][18];  // WET * dT[k][q]/dF for the direction's field
+  __shared__ __align__(16) float sM[2][3 * kECols];  // E of Gauss points g % 2: [k][column]
   __shared__ int32_t sCode[81];
   __shared__ int64_t sRb[9];
   __shared__ int32_t sRl[9];
   const int tid = threadIdx.x;
   const int64_t e = blockIdx.x;
   if (e >= n_el) return;  // uniform over the workgroup
-  const double EU = x[eu_index];
-  if (tid < 54) sU[tid] = x[(int64_t)conn[e * 9 + tid / 6] * 6 + tid % 6];
-  if (tid < 18) sX[tid] = coords[(int64_t)conn[e * 9 + tid / 2] * 2 + (tid & 1)];
+  const float EU = (float)x[eu_index];
+  if (tid < 54) sU[tid] = (float)x[(int64_t)conn[e * 9 + tid / 6] * 6 + tid % 6];
   if (tid < 81) sCode[tid] = fo.code[e * 81 + tid];
   if (tid < 9) {
     const int64_t r6 = (int64_t)conn[e * 9 + tid] * 6;
@@ -923,125 +928,136 @@ vef_jac32_kernel(const double* __restrict__ coords, const int32_t* __restrict__ 
     sRb[tid] = rb;
     sRl[tid] = (int32_t)(fo.rowptr[r6 + 1] - rb);
   }
+  // the Gauss points' basis derivatives, WET and Helem^2 from the geometry
+  // records the residual pass wrote (words 0..37 of each record), as FP32
   {
-    constexpr int NW = 9 * kGeoWords, NI = (NW + 255) / 256;
-    const double* src = fo.geo + e * 9 * kGeoWords;  // written by the residual pass
-    double* dst = reinterpret_cast<double*>(&sG[0]);
-    double gv[NI];
+    constexpr int NW = 9 * 38, NI = (NW + kThreads - 1) / kThreads;
+    const double* src = fo.geo + e * 9 * kGeoWords;
 #pragma unroll
-    for (int i = 0; i < NI; i++) gv[i] = src[min(tid + 256 * i, NW - 1)];
-#pragma unroll
-    for (int i = 0; i < NI; i++)
-      if (tid + 256 * i < NW) dst[tid + 256 * i] = gv[i];
-  }
-  __syncthreads();
-  if (tid < 9) {
-    const Geo& G = sG[tid];
-    for (int n = 0; n < 9; n++) {
-      sBas[tid][0][n] = (float)c_tab.bf[tid][n];
-      sBas[tid][1][n] = (float)G.dFx[n];
-      sBas[tid][2][n] = (float)G.dFy[n];
-      sBas[tid][3][n] = (float)G.d2x[n];
-      sBas[tid][4][n] = (float)G.d2y[n];
+    for (int i = 0; i < NI; i++) {
+      const int t = tid + kThreads * i;
+      if (t < NW) {
+        const int g = t / 38, w = t % 38;
+        const float v = (float)src[g * kGeoWords + w];
+        if (w < 36) sBas[g][1 + w / 9][w % 9] = v;
+        else if (w == 36) sWET[g] = v;
+        else sH2[g] = v;
+      }
     }
-    sWET[tid] = (float)G.WET;
-    sH2[tid] = (float)G.H2;
+    if (tid < 81) sBas[tid / 9][0][tid % 9] = (float)c_tab.bf[tid / 9][tid % 9];
+  }
+  // the padding columns of E stay zero
+  for (int t = tid; t < 2 * 3 * (kECols - 330); t += kThreads) {
+    const int b = t / (3 * (kECols - 330)), r = t % (3 * (kECols - 330));
+    sM[b][(r / (kECols - 330)) * kECols + 330 + r % (kECols - 330)] = 0.0f;
   }
   __syncthreads();
   // (1) the fields at the Gauss points
-  if (tid < 9 * vefad::kNumInterp) {
-    const int g = tid / vefad::kNumInterp, f = tid % vefad::kNumInterp;
-    const int q = vefad::field_unknown(f), k = vefad::field_basis(f);
-    float acc = 0.0f;
-    for (int n = 0; n < 9; n++) acc += (float)sU[n * 6 + q] * sBas[g][k][n];
-    sF[g][f] = acc;
-  } else if (tid < 9 * vefad::kNumInterp + 9) {
-    sF[tid - 9 * vefad::kNumInterp][vefad::kEU] = (float)EU;
+  for (int t = tid; t < 9 * vefad::kNumInterp + 9; t += kThreads) {
+    if (t < 9 * vefad::kNumInterp) {
+      const int g = t / vefad::kNumInterp, f = t % vefad::kNumInterp;
+      const int q = vefad::field_unknown(f), k = vefad::field_basis(f);
+      float acc = 0.0f;
+#pragma unroll
+      for (int n = 0; n < 9; n++) acc += sU[n * 6 + q] * sBas[g][k][n];
+      sF[g][f] = acc;
+    } else {
+      sF[t - 9 * vefad::kNumInterp][vefad::kEU] = EU;
+    }
   }
   __syncthreads();
-  // (2) D = WET * dT/dF, JAC32_DIRS directions per lane (directions f0 ..
-  //     f0 + JAC32_DIRS - 1 of Gauss point g; the last group's surplus
-  //     directions are computed and dropped)
-  constexpr int NDIR = JAC32_DIRS, NGRP = (NF + NDIR - 1) / NDIR;
-  if (tid < 9 * NGRP) {
-    const int g = tid / NGRP, f0 = (tid % NGRP) * NDIR;
-    vefad::DualV<float, NDIR> F[NF];
+  // (2) D = WET * dT/dF, one direction per lane
+  if (tid < 9 * kDirs) {
+    const int g = tid / kDirs, d = tid % kDirs, fd = dir_field(d);
+    vefad::DualV<float, 1> F[NF];
 #pragma unroll
     for (int i = 0; i < NF; i++) {
       F[i].v = sF[g][i];
-#pragma unroll
-      for (int j = 0; j < NDIR; j++) F[i].d[j] = i == f0 + j ? 1.0f : 0.0f;
+      F[i].d[0] = i == fd ? 1.0f : 0.0f;
     }
-    vefad::DualV<float, NDIR> T[3][6];
+    vefad::DualV<float, 1> T[3][6];
     vefad::gauss_terms(F, sBas[g][1], sBas[g][2], sH2[g], prm, T);
     const float w = sWET[g];
 #pragma unroll
-    for (int j = 0; j < NDIR; j++)
-      if (f0 + j < NF)
+    for (int k = 0; k < 3; k++)
 #pragma unroll
-        for (int k = 0; k < 3; k++)
-#pragma unroll
-          for (int q = 0; q < 6; q++) sD[g][f0 + j][k * 6 + q] = w * T[k][q].d[j];
+      for (int q = 0; q < 6; q++) sD[g][d][k * 6 + q] = w * T[k][q].d[0];
   }
-  // (3) J = sum_g Phi_g^T E_g on the matrix cores.  v_mfma_f32_16x16x4f32
-  //     lane layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j]
-  //     in register i % 4 of lane j + 16(i / 4).  Wavefront w owns the tiles
-  //     w, w + 4, ... (at most 6).
+  // (3) per Gauss point: E_g (VALU), then J += Phi_g^T E_g (matrix cores).
+  //     v_mfma_f32_16x16x4f32 lane layout: A[i][k] in lane i + 16k, B[k][j] in
+  //     lane j + 16k, D[i][j] in register i % 4 of lane j + 16(i / 4).
+  //     Wavefront w owns the tiles w, w + 3, ..., w + 18.
   const int wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;  // A row / B column, and k
+  // E-forming role: thread (kq, m) for tid < 162, dR/dEU entry kq for 162 <= tid < 180
+  const bool e_entry = tid < 162, e_eu = tid >= 162 && tid < 180;
+  const int kq = e_entry ? tid / 9 : tid - 162, em = tid % 9;
+  const int e_row = (kq / 6) * kECols;
+  const int e_col = e_entry ? (kq % 6) * 54 + em * 6 : 324 + kq % 6;
+  const float mBvN = -prm.BvN;
   typedef float floatx4 __attribute__((ext_vector_type(4)));
-  floatx4 accj[6];
+  floatx4 accj[kTilesPerWave];
 #pragma unroll
-  for (int t = 0; t < 6; t++) accj[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < kTilesPerWave; t++) accj[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  __syncthreads();  // D complete
   for (int g = 0; g < 9; g++) {
-    __syncthreads();  // D is complete / the previous E has been read
-    for (int t = tid; t < 3 * ECOL; t += 256) {
-      const int k = t / ECOL, col = t % ECOL;
-      float v = 0.0f;
-      if (col < 324) {
-        const int q = col / 54, m = (col % 54) / 6, p = col % 6;
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-          const int f = field_of(p, j);
-          if (f >= 0) v += sD[g][f][k * 6 + q] * sBas[g][j][m];
-        }
-      } else if (col < 330) {
-        v = sD[g][vefad::kEU][k * 6 + (col - 324)];
-      }
-      sM[t] = v;
+    float* E = sM[g & 1];
+    if (e_entry) {
+      const float* Dg = &sD[g][0][kq];
+#define DV(dd) Dg[(dd) * 18]
+      const float b0 = sBas[g][0][em], b1 = sBas[g][1][em], b2 = sBas[g][2][em];
+      const float b34 = sBas[g][3][em] + sBas[g][4][em];
+      const float2 v01{DV(dUx) * b0 + DV(dGU00) * b1 + DV(dGU10) * b2 + mBvN * DV(dPx) * b34,
+                       DV(dUy) * b0 + DV(dGU01) * b1 + DV(dGU11) * b2 + mBvN * DV(dPy) * b34};
+      const float2 v23{DV(dP) * b0 + DV(dPx) * b1 + DV(dPy) * b2,
+                       DV(dS00) * b0 + DV(dS00x) * b1 + DV(dS00y) * b2};
+      const float2 v45{DV(dS01) * b0 + DV(dS01x) * b1 + DV(dS01y) * b2,
+                       DV(dS11) * b0 + DV(dS11x) * b1 + DV(dS11y) * b2};
+#undef DV
+      float2* dst = reinterpret_cast<float2*>(&E[e_row + e_col]);  // e_col is even: 8-byte aligned
+      dst[0] = v01;
+      dst[1] = v23;
+      dst[2] = v45;
+    } else if (e_eu) {
+      E[e_row + e_col] = sD[g][dEU][kq];
     }
     __syncthreads();
     const float a = (li < 9 && lk < 3) ? sBas[g][lk][li] : 0.0f;  // Phi_g^T: node li, basis kind lk
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-      const int tile = wave + 4 * t;
-      if (tile < NTILE) {
-        const float b = lk < 3 ? sM[lk * ECOL + tile * 16 + li] : 0.0f;
-        accj[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accj[t], 0, 0, 0);
-      }
+    for (int t = 0; t < kTilesPerWave; t++) {
+      const int tile = wave + 3 * t;
+      const float b = lk < 3 ? E[lk * kECols + tile * 16 + li] : 0.0f;
+      accj[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accj[t], 0, 0, 0);
     }
   }
+  // stores: lane (li, lk) holds rows n = 4 lk + r of column (wave + 3 t) * 16 + li
+  //  of tile t; the destination (values or cbank) is selected, not branched on
+  int64_t rb[4];
+  int32_t rl[4];
 #pragma unroll
-  for (int t = 0; t < 6; t++) {
-    const int tile = wave + 4 * t;
-    if (tile < NTILE) {
-      const int col = tile * 16 + li;
+  for (int r = 0; r < 4; r++) {
+    const int n = min(4 * lk + r, 8);
+    rb[r] = sRb[n];
+    rl[r] = sRl[n];
+  }
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int n = 4 * lk + r;  // test node
-        if (n < 9 && col < 330) {
-          const float v = accj[t][r];
-          if (col < 324) {
-            const int q = col / 54, m = (col % 54) / 6, p = col % 6;  // row (n,q), column (m,p)
-            const int32_t cd = sCode[n * 9 + m];
-            if ((cd & 1) == 0)
-              fo.values[sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 1) * 6 + p] = 0.0 + (double)v;
-            else
-              fo.cbank[(int64_t)(cd >> 1) * 36 + q * 6 + p] = (double)v;
-          } else {
-            fo.acbank[e * 54 + n * 6 + (col - 324)] = (double)v;
-          }
+  for (int t = 0; t < kTilesPerWave; t++) {
+    const int col = (wave + 3 * t) * 16 + li;
+    const int q = col / 54, m = (col % 54) / 6, p = col % 6;  // row (n, q), column (m, p)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int n = 4 * lk + r;  // test node
+      if (n < 9 && col < 330) {
+        double* dst;
+        if (col < 324) {
+          const int32_t cd = sCode[n * 9 + m];
+          double* pv = fo.values + (rb[r] + (int64_t)q * rl[r] + (int64_t)(cd >> 1) * 6 + p);
+          double* pc = fo.cbank + ((int64_t)(cd >> 1) * 36 + q * 6 + p);
+          dst = (cd & 1) ? pc : pv;
+        } else {
+          dst = fo.acbank + (e * 54 + n * 6 + (col - 324));
         }
+        *dst = 0.0 + (double)accj[t][r];  // +0 for -0, as ADD_VALUES into zeroed storage
       }
     }
   }
@@ -2200,7 +2216,7 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
     FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank, h->d_geob};
     vefad::Params<float> prm{(float)h->params.ReN, (float)h->params.BvN, (float)h->params.VeN, (float)h->params.DfN,
                              (float)h->params.WiN0, (float)(1.0 - h->params.BvN), h->params.model};
-    hipLaunchKernelGGL(vef_jac32_kernel, dim3((unsigned)h->n_el), dim3(256), 0, st, h->d_coords, h->d_conn, x, eu,
+    hipLaunchKernelGGL(vef_jac32_kernel, dim3((unsigned)h->n_el), dim3(jac32::kThreads), 0, st, h->d_conn, x, eu,
                        h->n_el, prm, fo);
     VEF_HIP(h, hipGetLastError());
   }
