@@ -155,3 +155,28 @@ def test_dirichlet_rows_take_the_inflow_solution():
         assert np.abs(got - field).max() < 1e-12  # a second solve at the same Wi: equal to rounding
     assert all((6 * n + 2) not in last for n in inf.nodes)  # P is not prescribed at the inlet
     assert np.all(np.diff(mesh.coords[inf.nodes, 1]) > 0)
+
+
+@pytest.mark.parametrize("model,VeN", [("L-PTT", 0.25), ("GIESEKUS", 0.1)])
+def test_grouped_jacobian_equals_one_column_differences(model, VeN):
+    """The grouped forward-difference Jacobian (columns of nodes five apart
+    perturbed together) equals perturbing one column at a time bit for bit on
+    every node row, and on the flow-rate row within the difference quotient's
+    error of the exact (linear) row it takes instead."""
+    y = np.sort(np.concatenate([np.linspace(-H, H, 23), [0.1, -0.37]]))
+    inf = FullyDevelopedInflow(y, H, Params(BvN=0.59, VeN=VeN, model=model, DfN=1e-3))
+    inf.solve(0.7)
+    rng = np.random.default_rng(5)
+    z = inf.z + 1e-3 * rng.standard_normal(len(inf.z))
+    r = inf.residual(z, 0.7)
+    J = inf.jacobian(z, 0.7, r)
+    n = len(z)
+    Jd = np.empty((n, n))
+    for j in range(n):
+        h = 1.0e-7 * max(1.0, abs(z[j]))
+        zp = z.copy()
+        zp[j] += h
+        Jd[:, j] = (inf.residual(zp, 0.7) - r) / h
+    assert np.array_equal(J[:-1], Jd[:-1])
+    assert np.abs(J[-1] - Jd[-1]).max() <= 1e-6 * np.abs(Jd[-1]).max()
+    assert np.count_nonzero(J[-1]) == inf.nn  # dPL and the non-Ux unknowns do not enter the flow rate

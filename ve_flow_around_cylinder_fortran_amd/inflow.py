@@ -160,15 +160,42 @@ class FullyDevelopedInflow:
         """LU of the forward-difference Jacobian at z (r = residual(z))."""
         from scipy.linalg import lu_factor
 
-        n = len(z)
-        J = np.empty((n, n))
-        for j in range(n):
-            h = 1.0e-7 * max(1.0, abs(z[j]))
-            zp = z.copy()
-            zp[j] += h
-            J[:, j] = (self.residual(zp, WiN) - r) / h
         self.jacobians += 1
-        return lu_factor(J)
+        return lu_factor(self.jacobian(z, WiN, r))
+
+    def jacobian(self, z: np.ndarray, WiN: float, r: np.ndarray) -> np.ndarray:
+        """The forward-difference Jacobian at z (r = residual(z)), formed
+        with grouped columns: the rows of node k depend only on the nodes of
+        the elements that hold k (k-2 .. k+2), so the columns of nodes five
+        apart perturb disjoint rows and one residual evaluation serves a whole
+        group (25 groups of node unknowns, and dPL alone); each node row keeps
+        exactly the quotient a one-column perturbation gives it.  The
+        flow-rate row is linear in Ux and taken exactly (sum of phi*WET per
+        node)."""
+        n = len(z)
+        nn = self.nn
+        J = np.zeros((n, n))
+        h = 1.0e-7 * np.maximum(1.0, np.abs(z))
+        node = np.arange(nn)
+        rows_of = np.clip(node[:, None] + np.arange(-2, 3)[None, :], 0, nn - 1)  # nodes whose rows column i touches
+        for c in range(5):
+            members = node[node % 5 == c]
+            for q in range(5):
+                cols = members * 5 + q
+                zp = z.copy()
+                zp[cols] += h[cols]
+                d = self.residual(zp, WiN) - r
+                for i, col in zip(members, cols):
+                    rr = (rows_of[i][:, None] * 5 + np.arange(5)[None, :]).ravel()
+                    J[rr, col] = d[rr] / h[col]
+        zp = z.copy()
+        zp[-1] += h[-1]
+        J[:, -1] = (self.residual(zp, WiN) - r) / h[-1]
+        flow = np.zeros(nn)
+        np.add.at(flow, self.el, np.einsum("gn,ge->en", self.phi, self.WET))
+        J[-1, :-1] = 0.0
+        J[-1, 0:-1:5] = flow
+        return J
 
     def solve(self, WiN: float, max_iter: int = 100):
         """Newton on the inflow problem at WiN from the last solution (or the
