@@ -105,6 +105,9 @@ def parse():
     ap.add_argument("--check-every", type=int, default=10,
                     help="--continuation: every this many steps, an FP64 Newton solve at the step's Wi checks the "
                          "drag of the non-reference modes")
+    ap.add_argument("--inlet-coupling", action="store_true",
+                    help="--continuation: the inlet rows' dependence on EU in the Jacobian (NewtonSolver "
+                         "inlet_coupling; not the reference's Newton)")
     ap.add_argument("--learned-epochs", type=int, default=50)
     ap.add_argument("--learned-nodes", type=int, default=0,
                     help="nodes sampled per training step of the learned predictor (0: all)")
@@ -311,7 +314,8 @@ def continuation_bench(args):
         fp32 = mode.startswith("fp32")
         t_setup = time.perf_counter()
         solver = NewtonSolver(mesh, Params(**pset), factorization=args.factorization, reuse_factors=fp32,
-                              linear_precision="f32" if fp32 else "f64", jacobian="ad32" if fp32 else "fd64")
+                              linear_precision="f32" if fp32 else "f64", jacobian="ad32" if fp32 else "fd64",
+                              inlet_coupling=args.inlet_coupling)
         predictor = None
         if learned:
             from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
@@ -322,6 +326,8 @@ def continuation_bench(args):
             if n > 12 and n % 5 == 0:
                 print(f"[continuation] {mode}   iteration {n}: {rec.flag} ||b|| {rec.res_norm:.2e} "
                       f"||s|| {rec.cor_norm:.2e} xF {xF:g}", flush=True)
+            if time.perf_counter() - t_mode > args.mode_seconds:
+                raise TimeoutError(f"time budget of {args.mode_seconds:g} s spent inside a step")
 
         solver.on_iteration = progress
         cont = ArclengthContinuation(solver, LN0=LN0, dArN=dArN, predictor=predictor, dS_max=args.ds_max)
@@ -365,7 +371,8 @@ def continuation_bench(args):
             if mode != "reference" and (k + 1) % args.check_every == 0:
                 # FP64 reference-policy Newton at this step's Wi from its state
                 if check_solver[0] is None:
-                    check_solver[0] = NewtonSolver(mesh, Params(**pset), factorization=args.factorization)
+                    check_solver[0] = NewtonSolver(mesh, Params(**pset), factorization=args.factorization,
+                                                   inlet_coupling=args.inlet_coupling)
                 xr, hr = check_solver[0].solve(x, float(x[-1]))
                 Fr, Pr = drag_and_wall(check_solver[0], xr)
                 checks.append({"step": k + 1, "Wi": Wi, "Fd": Fd, "Fd_fp64": Fr,
@@ -423,6 +430,7 @@ def continuation_bench(args):
                         f"{LN0 + dArN}; dS0 = 1, dS_max = {args.ds_max:g}, eps_pr = 1e-3), Newton tolerance 1e-8 "
                         f"on ||b|| and ||s||, {'multifrontal' if args.factorization == 'mf' else 'banded'} LU",
             "elements": mesh.n_elements, "unknowns": mesh.n_unknowns,
+            "inlet_coupling": bool(args.inlet_coupling),
             "modes": modes,
         },
         "cpu_baseline": None,

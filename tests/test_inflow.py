@@ -180,3 +180,30 @@ def test_grouped_jacobian_equals_one_column_differences(model, VeN):
     assert np.array_equal(J[:-1], Jd[:-1])
     assert np.abs(J[-1] - Jd[-1]).max() <= 1e-6 * np.abs(Jd[-1]).max()
     assert np.count_nonzero(J[-1]) == inf.nn  # dPL and the non-Ux unknowns do not enter the flow rate
+
+
+@pytest.mark.parametrize("model,VeN", [("L-PTT", 0.0), ("L-PTT", 0.25), ("GIESEKUS", 0.1)])
+def test_inlet_sensitivity_matches_central_differences(model, VeN):
+    """d(inlet values)/dWiN (newton.inlet_sensitivity, the arclength-column
+    entries of the inlet rows under inlet_coupling) against central
+    differences of the inlet's Dirichlet values themselves (the fully
+    developed inflow problem solved at WiN +- h; for Oldroyd-B also the closed
+    form)."""
+    from ve_flow_around_cylinder_fortran_amd import cylinder_channel
+    from ve_flow_around_cylinder_fortran_amd.newton import cylinder_dirichlet, inlet_sensitivity
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    prm = Params(BvN=0.59, VeN=VeN, model=model, DfN=1e-3)
+    WiN, h = 0.45, 1e-5
+    cases = [FullyDevelopedInflow.from_mesh(mesh, prm)]
+    if VeN == 0.0:
+        cases.append(None)
+    for inflow in cases:
+        rows, sens = inlet_sensitivity(mesh, WiN, inflow)
+        rp, vp = cylinder_dirichlet(mesh, WiN + h, inflow)
+        rm, vm = cylinder_dirichlet(mesh, WiN - h, inflow)
+        n = len(rows)
+        assert np.array_equal(rp[:n], rows) and np.array_equal(rm[:n], rows)
+        fd = (vp[:n] - vm[:n]) / (2 * h)
+        assert np.abs(sens - fd).max() <= 1e-6 * max(1.0, np.abs(fd).max())
+        assert np.abs(fd).max() > 1e-2  # the inlet stresses do move with Wi

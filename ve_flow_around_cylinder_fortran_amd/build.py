@@ -7,7 +7,10 @@
   dense blocks); hipcc for gfx950, in-tree.  FP64 results must
   follow the source's operation order, so contraction into FMA is off
   (-ffp-contract=off) and no fast-math is used; packed FP32 instructions are
-  disabled for all HIP code.
+  disabled for all HIP code.  libvef.so's only FP32 arithmetic is the analytic
+  FP32 Jacobian (compared to FP32 accuracy), so its FP32 divisions and square
+  roots may use the hardware's approximate reciprocal and square root
+  (-fno-hip-fp32-correctly-rounded-divide-sqrt; FP64 is not affected).
 * the Fortran iso_c_binding module and its example driver (flang), which show
   the C ABI called the way the reference's Fortran would call it.
 """
@@ -53,7 +56,7 @@ def hipcc() -> str:
 
 
 def build_hip(verbose: bool = False) -> str:
-    for src, lib, extra in ((HIP_SRC, LIB, []), (MLP_SRC, MLP_LIB, []), (BAND_SRC, BAND_LIB, ["-ffp-contract=fast", "-lrocblas"]),
+    for src, lib, extra in ((HIP_SRC, LIB, ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]), (MLP_SRC, MLP_LIB, []), (BAND_SRC, BAND_LIB, ["-ffp-contract=fast", "-lrocblas"]),
                              (MF_SRC, MF_LIB, ["-lrocsolver", "-lrocblas"])):
         tmp = lib + ".tmp"
         cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, src, *extra]

@@ -1030,35 +1030,41 @@ vef_jac32_kernel(const int32_t* __restrict__ conn, const double* __restrict__ x,
       accj[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accj[t], 0, 0, 0);
     }
   }
-  // stores: lane (li, lk) holds rows n = 4 lk + r of column (wave + 3 t) * 16 + li
-  //  of tile t; the destination (values or cbank) is selected, not branched on
-  int64_t rb[4];
-  int32_t rl[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int n = min(4 * lk + r, 8);
-    rb[r] = sRb[n];
-    rl[r] = sRl[n];
-  }
+  // the element matrix to LDS (over D, whose last reads preceded the last
+  // barrier): lane (li, lk) holds rows n = 4 lk + r of column (wave + 3 t) * 16 + li
+  float* sJ = &sD[0][0][0];  // [n][330]
+  static_assert(sizeof(sD) >= 9 * 330 * sizeof(float), "the element matrix fits in D's space");
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; t++) {
     const int col = (wave + 3 * t) * 16 + li;
-    const int q = col / 54, m = (col % 54) / 6, p = col % 6;  // row (n, q), column (m, p)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int n = 4 * lk + r;  // test node
-      if (n < 9 && col < 330) {
-        double* dst;
-        if (col < 324) {
-          const int32_t cd = sCode[n * 9 + m];
-          double* pv = fo.values + (rb[r] + (int64_t)q * rl[r] + (int64_t)(cd >> 1) * 6 + p);
-          double* pc = fo.cbank + ((int64_t)(cd >> 1) * 36 + q * 6 + p);
-          dst = (cd & 1) ? pc : pv;
-        } else {
-          dst = fo.acbank + (e * 54 + n * 6 + (col - 324));
-        }
-        *dst = 0.0 + (double)accj[t][r];  // +0 for -0, as ADD_VALUES into zeroed storage
-      }
+      const int n = 4 * lk + r;
+      if (n < 9 && col < 330) sJ[n * 330 + col] = accj[t][r];
+    }
+  }
+  __syncthreads();
+  // stores: one thread per (test node n, row q, trial node m): the 6-entry run
+  // of row (n, q) in node m's block, to values (single contributor) or cbank;
+  // then the 54 entries of dR/dEU to acbank
+  for (int it = tid; it < 9 * 6 * 9 + 54; it += kThreads) {
+    if (it < 9 * 6 * 9) {
+      const int n = it / 54, q = (it / 9) % 6, m = it % 9;
+      const float2* src = reinterpret_cast<const float2*>(&sJ[n * 330 + q * 54 + m * 6]);  // even index
+      const float2 v01 = src[0], v23 = src[1], v45 = src[2];
+      const int32_t cd = sCode[n * 9 + m];
+      double* dst = (cd & 1) ? fo.cbank + ((int64_t)(cd >> 1) * 36 + q * 6)
+                             : fo.values + (sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 1) * 6);
+      // +0 for -0, as ADD_VALUES into zeroed storage
+      dst[0] = 0.0 + (double)v01.x;
+      dst[1] = 0.0 + (double)v01.y;
+      dst[2] = 0.0 + (double)v23.x;
+      dst[3] = 0.0 + (double)v23.y;
+      dst[4] = 0.0 + (double)v45.x;
+      dst[5] = 0.0 + (double)v45.y;
+    } else {
+      const int i = it - 9 * 6 * 9, n = i / 6;
+      fo.acbank[e * 54 + i] = 0.0 + (double)sJ[n * 330 + 324 + i % 6];
     }
   }
 }

@@ -224,6 +224,23 @@ class FullyDevelopedInflow:
         else:
             raise InflowError(f"inflow Newton: no convergence in {max_iter} iterations at WiN = {WiN}")
         self.z = z
+        self.WiN = WiN
         self.newton_iterations.append(it)
         nod = z[:-1].reshape(self.nn, 5)
         return tuple(nod[:, q].copy() for q in range(5)), float(z[-1])
+
+    def sensitivity(self, WiN: float):
+        """d(Ux, Uy, Sxx, Sxy, Syy)/dWiN at the nodes for the solution at WiN
+        (solved first if it is not the last one): J dz/dWiN = -dR/dWiN with J
+        the Jacobian at the solution and dR/dWiN a forward difference in WiN."""
+        from scipy.linalg import lu_factor, lu_solve
+
+        if self.z is None or getattr(self, "WiN", None) != WiN:
+            self.solve(WiN)
+        z = self.z
+        r = self.residual(z, WiN)
+        h = 1.0e-7 * max(1.0, abs(WiN))
+        dR = (self.residual(z, WiN + h) - r) / h
+        dz = -lu_solve(lu_factor(self.jacobian(z, WiN, r)), dR)
+        nod = dz[:-1].reshape(self.nn, 5)
+        return tuple(nod[:, q].copy() for q in range(5))

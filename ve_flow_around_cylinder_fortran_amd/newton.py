@@ -86,6 +86,25 @@ def cylinder_dirichlet(mesh: Mesh, WiN: float, inflow=None):
     return np.concatenate(rows).astype(np.int64), np.concatenate(vals)
 
 
+def inlet_sensitivity(mesh: Mesh, WiN: float, inflow=None):
+    """Rows of the inlet's Dirichlet conditions (Ux, Uy, Sxx, Sxy, Syy, the
+    rows cylinder_dirichlet lists first) and d(value)/dWiN of each: from the
+    fully developed inflow problem's sensitivity, or for the Oldroyd-B closed
+    form a central difference of it."""
+    g = mesh.geometry
+    if inflow is None:
+        inlet = np.nonzero(np.abs(mesh.coords[:, 0] + g["L_up"]) < MESH_TOLERANCE)[0]
+        h = 1.0e-6 * max(1.0, abs(WiN))
+        yp = oldroyd_b_inflow(mesh.coords[inlet, 1], g["H"], WiN + h)
+        ym = oldroyd_b_inflow(mesh.coords[inlet, 1], g["H"], WiN - h)
+        sens = [(a - b) / (2.0 * h) for a, b in zip(yp, ym)]
+    else:
+        inlet = inflow.nodes
+        sens = inflow.sensitivity(WiN)
+    rows = np.concatenate([6 * inlet + q for q in (0, 1, 3, 4, 5)]).astype(np.int64)
+    return rows, np.concatenate(sens)
+
+
 def initial_state(mesh: Mesh, EU: float) -> np.ndarray:
     """The rest state U = 0, P = 0, S = I with the arclength unknown EU: the
     first Newton iteration from it is the Stokes-like solve with the boundary
@@ -127,11 +146,21 @@ class NewtonSolver:
     unchanged (FP64, ||s|| and ||b|| below 1e-8), so the converged states
     agree with the reference policy's within that tolerance; the first solve
     after an FP64 factorisation is then checked against 1e-6 instead of 1e-10
-    (the Newton iteration absorbs a less accurate solve)."""
+    (the Newton iteration absorbs a less accurate solve).
+
+    inlet_coupling=True (not the reference's): in an arclength step the
+    inlet's Dirichlet values follow EU (the fully developed inflow at WiN =
+    WiN0*EU), but the reference's Jacobian keeps those rows as unit rows with
+    no EU column, so its Newton iteration (and the tangent J t = e_N) treat
+    the inlet values as fixed and converge only linearly in that coupling.
+    With the option each inlet row carries -d(value)/dEU in the arclength
+    column of every factorised Jacobian (the inflow problem's own
+    sensitivity), making the Jacobian of the same equations exact; the
+    residual, update and convergence test are unchanged."""
 
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
                  reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = None,
-                 jacobian: str = "fd64", inflow: str = None):
+                 jacobian: str = "fd64", inflow: str = None, inlet_coupling: bool = False):
         import torch
 
         from .linsolve import BandLU, BlockLU, HybridLU, RfLU
@@ -177,6 +206,8 @@ class NewtonSolver:
         self.jac_flag = "NRP" if jacobian == "fd64" else "NRP32"
         self.reuse_factors = reuse_factors
         self.reuse_max_iters = reuse_max_iters
+        self.inlet_coupling = inlet_coupling
+        self._rowptr = torch.from_numpy(np.asarray(rowptr, dtype=np.int64)).to(device)
         self.factored = False
         self.emergencies = []
         self.on_iteration = None  # optional callback(record, records in this solve, xF) per iteration
@@ -243,6 +274,8 @@ class NewtonSolver:
                 self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
                                    x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
                 self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
+                if jac and self.inlet_coupling and increment >= 3:
+                    self._couple_inlet(EU)
                 res_norm = float(torch.linalg.vector_norm(self.b))
                 if not math.isfinite(res_norm):
                     raise NewtonError("non-finite residual")  # the reference's general stop on NaN
@@ -309,6 +342,15 @@ class NewtonSolver:
             hist[-1].seconds = time.perf_counter() - t0
         self.dS_used = dS
         return x, hist
+
+    def _couple_inlet(self, EU: float):
+        """Arclength-column entries -d(value)/dEU of the inlet's Dirichlet rows
+        (the last stored entry of each row: the column N-1)."""
+        torch = self.torch
+        rows, sens = inlet_sensitivity(self.mesh, self.params.WiN0 * EU, self.inflow)
+        r = torch.from_numpy(rows).to(self.device)
+        pos = self._rowptr[r + 1] - 1
+        self.values[pos] = torch.from_numpy(-self.params.WiN0 * sens).to(self.device)
 
     def residual_norm(self, x, LN0: float, increment: int, dArN: float = 0.0, x_old=None, tangent=None,
                       dS: float = 0.0) -> float:
