@@ -315,7 +315,7 @@ def continuation_bench(args):
         t_setup = time.perf_counter()
         solver = NewtonSolver(mesh, Params(**pset), factorization=args.factorization, reuse_factors=fp32,
                               linear_precision="f32" if fp32 else "f64", jacobian="ad32" if fp32 else "fd64",
-                              inlet_coupling=args.inlet_coupling)
+                              inlet_coupling=args.inlet_coupling, reuse_max_iters="auto")
         predictor = None
         if learned:
             from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor

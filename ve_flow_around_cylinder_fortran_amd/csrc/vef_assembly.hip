@@ -830,18 +830,17 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
           if (!FD) {
             bank[(e0 + eel) * 54 + n * 6 + c] = acc;
           } else if (kk > 0) {
-            // (dRes - Res)/eps_jac for column kk-1 (or EU for kk = 55)
-            const double qv = fd_quotient<EXACT>(acc, sR0[n * 6 + c], dc);
-            if (kk == NEVAL - 1) {
-              fo.acbank[e0 * 54 + n * 6 + c] = qv;
-            } else {
-              const int in = (kk - 1) / 6, iq = (kk - 1) % 6;
-              const int32_t code = sCode[n * 9 + in];
-              if ((code & 1) == 0)
-                fo.values[sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 1) * 6 + iq] = 0.0 + qv;
-              else
-                fo.cbank[(int64_t)(code >> 1) * 36 + c * 6 + iq] = qv;
-            }
+            // (dRes - Res)/eps_jac for column kk-1 (or EU for kk = 55), as
+            // ADD_VALUES into zeroed storage (+0 for -0).  The ordered sums of
+            // cbank and acbank start from +0 as well, so every destination can
+            // take 0 + q: the destination is selected, not branched on.
+            const double qv = 0.0 + fd_quotient<EXACT>(acc, sR0[n * 6 + c], dc);
+            const int in = (kk - 1) / 6, iq = (kk - 1) % 6;
+            const int32_t code = sCode[n * 9 + min(in, 8)];
+            double* pv = fo.values + (sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 1) * 6 + iq);
+            double* pc = fo.cbank + ((int64_t)(code >> 1) * 36 + c * 6 + iq);
+            double* pa = fo.acbank + (e0 * 54 + n * 6 + c);
+            *(kk == NEVAL - 1 ? pa : ((code & 1) ? pc : pv)) = qv;
           }
         }
       }

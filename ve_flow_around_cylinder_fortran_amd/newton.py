@@ -142,7 +142,11 @@ class NewtonSolver:
     the last two corrections, more than reuse_max_iters further iterations
     would be needed to bring ||s|| below the tolerance (a factorisation costs
     about as much as that many modified-Newton iterations), and every 10th
-    iteration.  The residual, the update and the convergence test are
+    iteration.  reuse_max_iters="auto" takes that count from the solver's
+    own measured times: the extra wall time of a factorising iteration over
+    a modified-Newton one, in modified-Newton iterations (running means; at
+    least 1) — about 8 on the 1,600-element mesh's banded LU, about 5 for the
+    multifrontal LU at 40,000 elements.  The residual, the update and the convergence test are
     unchanged (FP64, ||s|| and ||b|| below 1e-8), so the converged states
     agree with the reference policy's within that tolerance; the first solve
     after an FP64 factorisation is then checked against 1e-6 instead of 1e-10
@@ -165,6 +169,8 @@ class NewtonSolver:
 
         from .linsolve import BandLU, BlockLU, HybridLU, RfLU
 
+        if not (reuse_max_iters == "auto" or (isinstance(reuse_max_iters, (int, float)) and reuse_max_iters > 0)):
+            raise NewtonError("reuse_max_iters must be a positive number or 'auto'")
         oldroyd_b = params.VeN == 0.0 and params.model == "L-PTT"
         if inflow is None:
             inflow = "closed_form" if oldroyd_b else "fe1d"
@@ -206,6 +212,7 @@ class NewtonSolver:
         self.jac_flag = "NRP" if jacobian == "fd64" else "NRP32"
         self.reuse_factors = reuse_factors
         self.reuse_max_iters = reuse_max_iters
+        self._t_nrp = self._t_mnr = None  # running mean wall times of factorising / modified-Newton iterations
         self.inlet_coupling = inlet_coupling
         self._rowptr = torch.from_numpy(np.asarray(rowptr, dtype=np.int64)).to(device)
         self.factored = False
@@ -308,7 +315,7 @@ class NewtonSolver:
                         reject = math.isfinite(cor_new)
                     elif cor_new > COR_NORM_EPS and cor_old > 0.0:
                         r = cor_new / cor_old
-                        if r > 0.0 and math.log(COR_NORM_EPS / cor_new) / math.log(r) > self.reuse_max_iters:
+                        if 0.0 < r < 1.0 and math.log(COR_NORM_EPS / cor_new) / math.log(r) > self._reuse_limit():
                             flag = "NRP"
                 else:
                     flag = "NRP" if cor_new > cor_old else "MNR"
@@ -340,8 +347,24 @@ class NewtonSolver:
             x = x - xF * s
             torch.cuda.synchronize()
             hist[-1].seconds = time.perf_counter() - t0
+            self._time_iteration(hist[-1])
         self.dS_used = dS
         return x, hist
+
+    def _time_iteration(self, rec: NewtonRecord):
+        a = 0.3  # weight of the newest iteration in the running means
+        if rec.flag == "MNR":
+            self._t_mnr = rec.seconds if self._t_mnr is None else (1 - a) * self._t_mnr + a * rec.seconds
+        else:
+            self._t_nrp = rec.seconds if self._t_nrp is None else (1 - a) * self._t_nrp + a * rec.seconds
+
+    def _reuse_limit(self) -> float:
+        """Modified-Newton iterations a factorisation is worth (reuse_max_iters)."""
+        if self.reuse_max_iters != "auto":
+            return float(self.reuse_max_iters)
+        if self._t_nrp is None or not self._t_mnr:
+            return 8.0
+        return max(1.0, (self._t_nrp - self._t_mnr) / self._t_mnr)
 
     def _couple_inlet(self, EU: float):
         """Arclength-column entries -d(value)/dEU of the inlet's Dirichlet rows
