@@ -114,6 +114,11 @@ def parse():
     ap.add_argument("--stability", action="store_true",
                     help="linear stability assembly (A and B) of the 40,000-element mesh instead of the Newton "
                          "assembly")
+    ap.add_argument("--base", default="converged", choices=["converged", "synthetic"],
+                    help="--stability: the base state — a converged steady state at --base-wi (FP32 inexact "
+                         "continuation, then an FP64 Newton solve pinned at that Wi) with the 10 leading eigenvalues "
+                         "of the GPU and of the oracle-assembled matrices, or the smooth synthetic state")
+    ap.add_argument("--base-wi", type=float, default=0.6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference launch for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -458,11 +463,18 @@ def stability_bench(args):
     from ve_flow_around_cylinder_fortran_amd.stability import StabilityAssembler
 
     n_el = 40000 if args.elements == N_ELEMENTS else args.elements
-    pset = dict(ReN=0.0, BvN=0.59, VeN=0.25, DfN=1.0e-3, WiN0=1.0, model="L-PTT")
+    ven = args.newton_ven if args.newton_ven > 0.0 else 0.25
+    pset = dict(ReN=0.0, BvN=0.59, VeN=ven, DfN=1.0e-3, WiN0=1.0, model=args.newton_model)
+    model_name = (f"linear PTT epsilon={ven}" if args.newton_model == "L-PTT" else f"Giesekus alpha={ven}")
     mesh = workload_mesh(n_el)
-    x = smooth_state(mesh, EU=0.6)
+    base_info = None
+    if args.base == "converged":
+        xd, base_info = converged_base(mesh, pset, args.base_wi)
+        x = xd.cpu().numpy()
+    else:
+        x = smooth_state(mesh, EU=args.base_wi)
+        xd = torch.from_numpy(x).to("cuda:0")
     sa = StabilityAssembler(mesh, Params(**pset))
-    xd = torch.from_numpy(x).to("cuda:0")
     nnz = len(sa.col)
     A = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
     B = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
@@ -495,11 +507,12 @@ def stability_bench(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        "data": "synthetic mesh; converged base state" if base_info else "synthetic",
         "config": {
             "workload": f"cylinder-in-channel Q2 mesh, {ne} elements, linear stability A and B "
-                        "(Cylinder_Stability_2D) around a smooth base state at Wi=0.6, L-PTT epsilon=0.25 beta=0.59 "
-                        "Re=0 DfN=1e-3, Dirichlet rows applied",
+                        f"(Cylinder_Stability_2D) around a {'converged' if base_info else 'smooth synthetic'} base "
+                        f"state at Wi={args.base_wi}, {model_name} beta=0.59 Re=0 DfN=1e-3, Dirichlet rows applied",
+            "base_state": base_info,
             "elements": ne, "unknowns": 6 * mesh.n_nodes, "nnz": nnz,
             "wavenumbers": "A and B do not depend on the spanwise wavenumber in the reference (spanwise terms "
                            "commented out); one assembly serves a batch of 16",
@@ -522,11 +535,107 @@ def stability_bench(args):
             "flops_per_launch": flops,
         },
     }
+    if base_info is not None:
+        out["config"]["eigenvalues"] = stability_eigenvalues(mesh, pset, x, sa, A, B)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = stability_cpu_baseline(mesh, x, pset, sa.rowptr, sa.col, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def converged_base(mesh, pset, Wi):
+    """A converged steady state at WiN = Wi: arclength continuation from Wi =
+    0.1 with the FP32 inexact Newton (exact inlet coupling) until Wi is
+    passed, then an FP64 Newton solve with the reference policy pinned at Wi
+    from the nearest converged state."""
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd import Params
+    from ve_flow_around_cylinder_fortran_amd.newton import ArclengthContinuation, NewtonSolver, initial_state
+
+    t0 = time.perf_counter()
+    solver = NewtonSolver(mesh, Params(**pset), factorization="mf", reuse_factors=True, linear_precision="f32",
+                          jacobian="ad32", inlet_coupling=True, reuse_max_iters="auto")
+    cont = ArclengthContinuation(solver, LN0=0.1, dArN=0.02)
+    cont.start(initial_state(mesh, 0.1))
+    near, steps = None, 0
+    while True:
+        x, _ = cont.step()
+        steps += 1
+        w = pset["WiN0"] * float(x[-1])
+        if near is None or abs(w - Wi) < abs(pset["WiN0"] * float(near[-1]) - Wi):
+            near = x
+        if w >= Wi or steps >= 200:
+            break
+    print(f"[stability] continuation to Wi {w:.4f} in {steps} steps, {time.perf_counter() - t0:.1f} s", flush=True)
+    del solver, cont
+    ref = NewtonSolver(mesh, Params(**pset), factorization="mf")
+    xb, hist = ref.solve(near, Wi / pset["WiN0"])
+    torch.cuda.synchronize()
+    info = {"continuation_steps": steps, "continuation_Wi": w, "fp64_newton_iterations": len(hist),
+            "final_residual_norm": hist[-1].res_norm, "final_correction_norm": hist[-1].cor_norm,
+            "seconds": time.perf_counter() - t0}
+    print(f"[stability] FP64 Newton at Wi {Wi}: {len(hist)} iterations ({info['seconds']:.1f} s)", flush=True)
+    del ref
+    torch.cuda.empty_cache()
+    return xb, info
+
+
+def mirror_map(mesh):
+    """Node of each node's mirror image in y = 0 (the channel's axis)."""
+    import numpy as np
+
+    key = np.round(mesh.coords * 1e9).astype(np.int64)
+    order = np.lexsort((key[:, 1], key[:, 0]))
+    mk = key.copy()
+    mk[:, 1] = -mk[:, 1]
+    morder = np.lexsort((mk[:, 1], mk[:, 0]))
+    m = np.empty(len(key), dtype=np.int64)
+    m[morder] = order
+    if not np.array_equal(key[m][:, 0], key[:, 0]) or not np.array_equal(key[m][:, 1], -key[:, 1]):
+        return None
+    return m
+
+
+def stability_eigenvalues(mesh, pset, x, sa, A, B, k=10):
+    """The k eigenvalues nearest 0 of A q = lambda B q (shift-invert Arnoldi
+    on the GPU multifrontal LU) for the GPU matrices and for the oracle's,
+    their largest relative difference, and the mirror symmetry of the leading
+    mode (Ux, P, Sxx, Syy even and Uy, Sxy odd in y: symmetric)."""
+    import numpy as np
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd.stability import leading_eigenvalues_gpu, stability_dirichlet_rows
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    t0 = time.perf_counter()
+    lam, V, lu, napply = leading_eigenvalues_gpu(A, B, sa.rowptr, sa.col, mesh, k=k, vectors=True, tol=1e-10)
+    t_gpu = time.perf_counter() - t0
+    oA, oB = oracle.stab_assemble(oracle.params(**pset), mesh, x, sa.rowptr, sa.col, nthreads=16)
+    oracle.stab_dirichlet(sa.rowptr, sa.col, oA, oB, stability_dirichlet_rows(mesh))
+    bitwise = bool(np.array_equal(oA, A.cpu().numpy()) and np.array_equal(oB, B.cpu().numpy()))
+    lam_o, _, _, _ = leading_eigenvalues_gpu(torch.from_numpy(oA).cuda(), torch.from_numpy(oB).cuda(), sa.rowptr,
+                                             sa.col, mesh, k=k, lu=lu, tol=1e-10)
+    rel = max(float(np.abs(lam - lo).min() / abs(lo)) for lo in lam_o)
+    sym = None
+    m = mirror_map(mesh)
+    if m is not None:
+        q = V[:, 0].reshape(-1, 6)
+        D = np.array([1, -1, 1, 1, -1, 1], dtype=float)
+        qm = q[m] * D
+        nq = np.linalg.norm(q)
+        sym = {"symmetric_part": float(np.linalg.norm(q + qm) / (2 * nq)),
+               "antisymmetric_part": float(np.linalg.norm(q - qm) / (2 * nq))}
+    out = {"k": k, "gpu": [[float(v.real), float(v.imag)] for v in lam],
+           "oracle_matrices": [[float(v.real), float(v.imag)] for v in lam_o],
+           "max_rel_diff": rel, "matrices_bitwise_equal_oracle": bitwise, "operator_applications": napply,
+           "seconds_gpu_eigensolve": t_gpu, "leading_mode_mirror": sym}
+    print(f"[stability] eigenvalues {lam[:4]} ... max relative difference to the oracle matrices' {rel:.1e}; "
+          f"leading mode {sym}", flush=True)
+    return out
 
 
 def stability_cpu_baseline(mesh, x, pset, rowptr, col, target_s):
