@@ -478,6 +478,7 @@ def stability_bench(args):
     nnz = len(sa.col)
     A = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
     B = torch.empty(nnz, dtype=torch.float64, device="cuda:0")
+    print("[stability] assembling A and B", flush=True)
     for _ in range(args.warmup):
         sa.assemble(xd, A, B)
     torch.cuda.synchronize()
@@ -612,13 +613,17 @@ def stability_eigenvalues(mesh, pset, x, sa, A, B, k=10):
     import oracle
 
     t0 = time.perf_counter()
-    lam, V, lu, napply = leading_eigenvalues_gpu(A, B, sa.rowptr, sa.col, mesh, k=k, vectors=True, tol=1e-10)
+    say = lambda n: print(f"[stability] eigen solve: {n} operator applications, {time.perf_counter() - t0:.0f} s",  # noqa: E731
+                          flush=True)
+    lam, V, lu, napply = leading_eigenvalues_gpu(A, B, sa.rowptr, sa.col, mesh, k=k, vectors=True, tol=1e-10,
+                                                 progress=say, maxiter=2000)
     t_gpu = time.perf_counter() - t0
+    print(f"[stability] GPU matrices: {napply} applications, {t_gpu:.1f} s; the oracle's matrices", flush=True)
     oA, oB = oracle.stab_assemble(oracle.params(**pset), mesh, x, sa.rowptr, sa.col, nthreads=16)
     oracle.stab_dirichlet(sa.rowptr, sa.col, oA, oB, stability_dirichlet_rows(mesh))
     bitwise = bool(np.array_equal(oA, A.cpu().numpy()) and np.array_equal(oB, B.cpu().numpy()))
     lam_o, _, _, _ = leading_eigenvalues_gpu(torch.from_numpy(oA).cuda(), torch.from_numpy(oB).cuda(), sa.rowptr,
-                                             sa.col, mesh, k=k, lu=lu, tol=1e-10)
+                                             sa.col, mesh, k=k, lu=lu, tol=1e-10, progress=say, maxiter=2000)
     rel = max(float(np.abs(lam - lo).min() / abs(lo)) for lo in lam_o)
     sym = None
     m = mirror_map(mesh)

@@ -78,12 +78,15 @@ def leading_eigenvalues(A, B, k: int = 10, sigma: float = 0.0, seed: int = 0):
 
 
 def leading_eigenvalues_gpu(A, B, rowptr, col, mesh, k: int = 10, sigma: float = 0.0, seed: int = 0,
-                            lu=None, vectors: bool = False, tol: float = 1e-12, ncv: int = 64):
+                            lu=None, vectors: bool = False, tol: float = 1e-12, ncv: int = 64, maxiter: int = None,
+                            progress=None):
     """The k eigenvalues of A q = lambda B q nearest sigma, sorted by
     decreasing real part: ARPACK's shift-invert Arnoldi (scipy) on
     (A - sigma B)^-1 B, the factorisation and solves of A - sigma B on the GPU
     (multifrontal LU in nested-dissection order) and B q as a GPU sparse
     product.  A, B: device value tensors in the pattern (rowptr, col).
+    progress: optional callable(applications) called every 25 operator
+    applications; maxiter: ARPACK's restart limit.
     Returns (eigenvalues, eigenvectors or None, the LU, operator applications)."""
     import scipy.sparse.linalg as sla
     import torch
@@ -101,12 +104,14 @@ def leading_eigenvalues_gpu(A, B, rowptr, col, mesh, k: int = 10, sigma: float =
 
     def matvec(v):
         count[0] += 1
+        if progress is not None and count[0] % 25 == 0:
+            progress(count[0])
         vd = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
         return lu.solve(torch.mv(Bt, vd)).cpu().numpy()
 
     op = sla.LinearOperator((n, n), matvec=matvec, dtype=np.float64)
     v0 = np.random.default_rng(seed).standard_normal(n)
-    out = sla.eigs(op, k=k, which="LM", v0=v0, return_eigenvectors=vectors, tol=tol, ncv=ncv)
+    out = sla.eigs(op, k=k, which="LM", v0=v0, return_eigenvectors=vectors, tol=tol, ncv=ncv, maxiter=maxiter)
     nu, V = (out if vectors else (out, None))
     lam = sigma + 1.0 / nu
     o = np.lexsort((lam.imag, -lam.real))  # decreasing real part, conjugates by imaginary part
