@@ -405,6 +405,9 @@ def continuation_bench(args):
                "max_drag_rel_diff_fp64_checks": max([c["drag_rel_diff"] for c in checks], default=None)}
         if learned:
             out["choices"] = cont.choices
+            # per proposal: ||b|| at the tangent prediction and at the learned proposal
+            out["proposal_residuals"] = [[float(a), float(b)] for a, b in cont.residuals]
+            out["training_losses"] = [float(v) for v in predictor.losses]
         del solver, ref, cont
         torch.cuda.empty_cache()
         return out
