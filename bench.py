@@ -561,6 +561,12 @@ def converged_base(mesh, pset, Wi):
     t0 = time.perf_counter()
     solver = NewtonSolver(mesh, Params(**pset), factorization="mf", reuse_factors=True, linear_precision="f32",
                           jacobian="ad32", inlet_coupling=True, reuse_max_iters="auto")
+    def progress(rec, n, xF):  # long solves (restarts) report as they go
+        if n % 20 == 0:
+            print(f"[stability]   iteration {n}: {rec.flag} ||b|| {rec.res_norm:.2e} ||s|| {rec.cor_norm:.2e} "
+                  f"xF {xF:g}", flush=True)
+
+    solver.on_iteration = progress
     cont = ArclengthContinuation(solver, LN0=0.1, dArN=0.02)
     cont.start(initial_state(mesh, 0.1))
     near, steps = None, 0
@@ -568,6 +574,7 @@ def converged_base(mesh, pset, Wi):
         x, _ = cont.step()
         steps += 1
         w = pset["WiN0"] * float(x[-1])
+        print(f"[stability] continuation step {steps}: Wi {w:.5f} ({time.perf_counter() - t0:.1f} s)", flush=True)
         if near is None or abs(w - Wi) < abs(pset["WiN0"] * float(near[-1]) - Wi):
             near = x
         if w >= Wi or steps >= 200:
