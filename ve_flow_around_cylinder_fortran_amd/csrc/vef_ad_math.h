@@ -185,6 +185,63 @@ struct Params {
   int model;  // 0 L-PTT, 1 Giesekus
 };
 
+// The six test-node terms of basis kind K (0: phi, 1: dphi/dx, 2: dphi/dy) at
+// a Gauss point (momentumMainSQRT, continuityBulkMain, stressesBulkMainSQRT
+// with LSME, LSIC, LSCE and DfN, src/Residuals_mod.F90:90-110,161-234,
+// 295-475) for phi = (b, fx, fy) = the K-th unit vector: X(ax, ay) stands for
+// ax*fx + ay*fy, and every term multiplied by a zero selector is left out.
+template <int K, class R, class S>
+__host__ __device__ inline void test_terms(R Ux, R Uy, R S00, R S01, R S11, const R* F, R WiN, R f_T, R P00, R P01,
+                                           R P11, R CE00, R CE01, R CE10, R CE11, R RC0, R RC1, R Mom0, R Mom1,
+                                           R Cont, R tlsme, R tlsic, R tlsce, R kap, R rW, const Params<S>& prm,
+                                           R T[6]) {
+  static_assert(K >= 0 && K <= 2, "basis kind");
+  if constexpr (K == 0) {
+    // b = 1: the gradient-weighted parts (LSME, LSIC, the LSCE momentum
+    // terms, DfN, the velocity part of the LSCE stress weight) vanish
+    T[0] = RC0;
+    T[1] = RC1;
+    T[2] = Cont;
+    const R lsce_s = tlsce * (f_T * S(0.5));
+    T[3] = CE00 + lsce_s * CE00;
+    T[4] = CE01 + lsce_s * CE01;
+    T[5] = CE11 + lsce_s * CE11;
+  } else {
+    constexpr bool X = K == 1;  // fx = 1 (else fy = 1)
+    const R Ud = X ? Ux : Uy;
+    const R lsme_m = tlsme * prm.ReN * Ud;
+    // momentum x: GUT_lsce row 1 = grad(phi)
+    R w = (X ? S01 : S11) / rW;
+    R u00 = -(X ? S00 : S01) - w * S01, u01 = -(w * S11);
+    R u10 = -(X ? S01 : S11) + w * S00, u11 = w * S01;
+    const R l0 = kap * (u00 * CE00 + u01 * CE10 + u10 * CE01 + u11 * CE11);
+    // momentum y: GUT_lsce row 2 = grad(phi)
+    w = -(X ? S00 : S01) / rW;
+    u00 = -(w * S01);
+    u01 = -(X ? S00 : S01) - w * S11;
+    u10 = w * S00;
+    u11 = -(X ? S01 : S11) + w * S01;
+    const R l1 = kap * (u00 * CE00 + u01 * CE10 + u10 * CE01 + u11 * CE11);
+    const R lsce_s = tlsce * (WiN * Ud);
+    const R tw = tlsme * WiN;
+    if constexpr (X) {
+      T[0] = P00 + lsme_m * Mom0 + tlsic * Cont + l0;
+      T[1] = P01 + lsme_m * Mom1 + l1;
+      T[2] = tlsme * Mom0;
+      T[3] = prm.DfN * F[kS00x] - tw * Mom0 + lsce_s * CE00;
+      T[4] = prm.DfN * F[kS01x] - tw * Mom1 + lsce_s * CE01;
+      T[5] = prm.DfN * F[kS11x] + lsce_s * CE11;
+    } else {
+      T[0] = P01 + lsme_m * Mom0 + l0;
+      T[1] = P11 + lsme_m * Mom1 + tlsic * Cont + l1;
+      T[2] = tlsme * Mom1;
+      T[3] = prm.DfN * F[kS00y] + lsce_s * CE00;
+      T[4] = prm.DfN * F[kS01y] - tw * Mom0 + lsce_s * CE01;
+      T[5] = prm.DfN * F[kS11y] - tw * Mom1 + lsce_s * CE11;
+    }
+  }
+}
+
 // T[k][q]: k = 0 coefficient of phi, 1 of dphi/dx, 2 of dphi/dy; q the row.
 // dFx, dFy: the 9 basis gradients at the point (for tsugn), H2 = Helem^2.
 template <class R, class S>
@@ -259,31 +316,16 @@ __host__ __device__ inline void gauss_terms(const R* F, const S* dFx, const S* d
   const R tlsce = S(1) / dsqrt(WiN * WiN * tsugn * tsugn + WiN * WiN * (S(0.5) * ddG) + f1 * f1 +
                                f2 * f2 * (S(0.5) * ddC) + t7 * t7);
   const R kap = tlsce * prm.omb / WiN;  // LSCEMomentumSQRT: tlsce*UCD_lsce*(1-BvN)/WiN
-  // test-node terms for phi = (1, 0, 0), (0, 1, 0), (0, 0, 1)
-  for (int k = 0; k < 3; k++) {
-    const S b = k == 0 ? S(1) : S(0), fx = k == 1 ? S(1) : S(0), fy = k == 2 ? S(1) : S(0);
-    const R lsme_m = tlsme * prm.ReN * (Ux * fx + Uy * fy);
-    // momentum x: GUT_lsce row 1 = grad(phi)
-    R w = (S01 * fx + S11 * fy) / rW;
-    R u00 = -(S00 * fx + S01 * fy) - w * S01, u01 = -(w * S11);
-    R u10 = -(S01 * fx + S11 * fy) + w * S00, u11 = w * S01;
-    const R l0 = kap * (u00 * CE00 + u01 * CE10 + u10 * CE01 + u11 * CE11);
-    T[k][0] = RC0 * b + (P00 * fx + P01 * fy) + lsme_m * Mom0 + tlsic * fx * Cont + l0;
-    // momentum y: GUT_lsce row 2 = grad(phi)
-    w = (-(S00 * fx) - S01 * fy) / rW;
-    u00 = -(w * S01);
-    u01 = -(S00 * fx + S01 * fy) - w * S11;
-    u10 = w * S00;
-    u11 = -(S01 * fx + S11 * fy) + w * S01;
-    const R l1 = kap * (u00 * CE00 + u01 * CE10 + u10 * CE01 + u11 * CE11);
-    T[k][1] = RC1 * b + (P01 * fx + P11 * fy) + lsme_m * Mom1 + tlsic * fy * Cont + l1;
-    T[k][2] = Cont * b + tlsme * (Mom0 * fx + Mom1 * fy);
-    const R lsce_s = tlsce * (WiN * (Ux * fx + Uy * fy) + f_T * b * S(0.5));
-    T[k][3] = CE00 * b + prm.DfN * (F[kS00x] * fx + F[kS00y] * fy) - tlsme * WiN * (Mom0 * fx) + lsce_s * CE00;
-    T[k][4] = CE01 * b + prm.DfN * (F[kS01x] * fx + F[kS01y] * fy) - tlsme * WiN * (Mom0 * fy + Mom1 * fx) +
-              lsce_s * CE01;
-    T[k][5] = CE11 * b + prm.DfN * (F[kS11x] * fx + F[kS11y] * fy) - tlsme * WiN * (Mom1 * fy) + lsce_s * CE11;
-  }
+  // test-node terms for phi = (1, 0, 0), (0, 1, 0), (0, 0, 1): the reference's
+  // terms with the basis selectors b = phi, fx = dphi/dx, fy = dphi/dy set to
+  // their 0/1 values at compile time, so the products by zero (which IEEE
+  // arithmetic may not drop by itself) are not formed
+  test_terms<0>(Ux, Uy, S00, S01, S11, F, WiN, f_T, P00, P01, P11, CE00, CE01, CE10, CE11, RC0, RC1, Mom0, Mom1,
+                Cont, tlsme, tlsic, tlsce, kap, rW, prm, T[0]);
+  test_terms<1>(Ux, Uy, S00, S01, S11, F, WiN, f_T, P00, P01, P11, CE00, CE01, CE10, CE11, RC0, RC1, Mom0, Mom1,
+                Cont, tlsme, tlsic, tlsce, kap, rW, prm, T[1]);
+  test_terms<2>(Ux, Uy, S00, S01, S11, F, WiN, f_T, P00, P01, P11, CE00, CE01, CE10, CE11, RC0, RC1, Mom0, Mom1,
+                Cont, tlsme, tlsic, tlsce, kap, rW, prm, T[2]);
 }
 
 }  // namespace vefad

@@ -951,11 +951,23 @@ vef_jac32_kernel(const int32_t* __restrict__ conn, const double* __restrict__ x,
     sM[b][(r / (kECols - 330)) * kECols + 330 + r % (kECols - 330)] = 0.0f;
   }
   __syncthreads();
-  // (1) the fields at the Gauss points
+  // (1) the fields at the Gauss points (unknown and basis kind of each field
+  //     from a table: selecting them per lane compiled into branch chains)
+  constexpr unsigned char kFieldQ[vefad::kNumInterp] = {
+#define Q(f) (unsigned char)vefad::field_unknown(f)
+      Q(0), Q(1), Q(2), Q(3), Q(4), Q(5), Q(6), Q(7), Q(8), Q(9), Q(10),
+      Q(11), Q(12), Q(13), Q(14), Q(15), Q(16), Q(17), Q(18), Q(19), Q(20), Q(21)};
+#undef Q
+  constexpr unsigned char kFieldK[vefad::kNumInterp] = {
+#define K(f) (unsigned char)vefad::field_basis(f)
+      K(0), K(1), K(2), K(3), K(4), K(5), K(6), K(7), K(8), K(9), K(10),
+      K(11), K(12), K(13), K(14), K(15), K(16), K(17), K(18), K(19), K(20), K(21)};
+#undef K
+  static_assert(vefad::kNumInterp == 22, "the tables list 22 fields");
   for (int t = tid; t < 9 * vefad::kNumInterp + 9; t += kThreads) {
     if (t < 9 * vefad::kNumInterp) {
       const int g = t / vefad::kNumInterp, f = t % vefad::kNumInterp;
-      const int q = vefad::field_unknown(f), k = vefad::field_basis(f);
+      const int q = kFieldQ[f], k = kFieldK[f];
       float acc = 0.0f;
 #pragma unroll
       for (int n = 0; n < 9; n++) acc += sU[n * 6 + q] * sBas[g][k][n];
@@ -1021,13 +1033,17 @@ vef_jac32_kernel(const int32_t* __restrict__ conn, const double* __restrict__ x,
       E[e_row + e_col] = sD[g][dEU][kq];
     }
     __syncthreads();
-    const float a = (li < 9 && lk < 3) ? sBas[g][lk][li] : 0.0f;  // Phi_g^T: node li, basis kind lk
+    // Phi_g^T: node li, basis kind lk (zero for the padding rows and the
+    // fourth k); loads are unconditional from clamped addresses, so no lane
+    // mask is switched around them.  B's k = 3 row meets A's zero column: it
+    // is read from row 2 (finite), which adds exact zeros.
+    const float av = sBas[g][min(lk, 2)][min(li, 8)];
+    const float a = (li < 9 && lk < 3) ? av : 0.0f;
+    float bv[kTilesPerWave];
 #pragma unroll
-    for (int t = 0; t < kTilesPerWave; t++) {
-      const int tile = wave + 3 * t;
-      const float b = lk < 3 ? E[lk * kECols + tile * 16 + li] : 0.0f;
-      accj[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accj[t], 0, 0, 0);
-    }
+    for (int t = 0; t < kTilesPerWave; t++) bv[t] = E[min(lk, 2) * kECols + (wave + 3 * t) * 16 + li];
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; t++) accj[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[t], accj[t], 0, 0, 0);
   }
   // the element matrix to LDS (over D, whose last reads preceded the last
   // barrier): lane (li, lk) holds rows n = 4 lk + r of column (wave + 3 t) * 16 + li
