@@ -11,7 +11,8 @@ analytic Jacobian and FP32 multifrontal factorisation carried across steps.
   converged states' differences within the Newton tolerance), and an FP64
   Newton solve with the reference policy at the FP32 mode's final Wi, from
   its state, moves the drag on the cylinder by at most 1e-8 relative (the
-  |dCd| bound);
+  |dCd| bound), and some pseudo-arclength step of the FP32 mode runs on the
+  factorisation carried from the step before;
 * the final FP64 state solves the reference's discrete equations as the
   oracle assembles them on the CPU (residual rows of every node, Dirichlet
   rows applied): ||b|| below the Newton tolerance."""
@@ -50,6 +51,7 @@ def _run(mesh, pset, fp32):
         print(f"[gpu continuation 40k] {pset['model']} {'fp32' if fp32 else 'fp64'} step {k + 1}: Wi {Wi[-1]:.6f} "
               f"Fd {Fd[-1]:.12e} flags {flags[-1]}", flush=True)
     solver.last_x = x
+    solver.flags = flags
     return np.array(Wi), np.array(Fd), x.cpu().numpy(), solver
 
 
@@ -65,7 +67,9 @@ def test_continuation_40k_fp32_inexact_matches_fp64(oracle, mesh40k, model, VeN)
     Wi32, Fd32, _, s32 = _run(mesh, pset, True)
     assert np.all(np.diff(Wi64) > 0)  # dS0 = 1 is small against the state's norm (~1e3): Wi moves little
     assert np.abs(Wi32 - Wi64).max() <= 1e-6
-    assert s32.lu.refactorizations < s64.lu.refactorizations  # the factorisation is carried across steps
+    # the factorisation is carried across steps: some pseudo-arclength step
+    # of the FP32 mode runs on the carried factorisation alone
+    assert any("N" not in f for f in s32.flags[2:]), s32.flags
     x32 = s32.last_x
     xr, hr = s64.solve(x32, float(x32[-1]))
     Fr = float(s64.asm.drag(xr).item())
