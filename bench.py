@@ -109,9 +109,6 @@ def parse():
                     help="--continuation: the inlet rows' dependence on EU in the Jacobian (NewtonSolver "
                          "inlet_coupling; not the reference's Newton)")
     ap.add_argument("--learned-epochs", type=int, default=50)
-    ap.add_argument("--learned-scaling", default="step", choices=["step", "first"],
-                    help="learned predictor targets: the tangent's error per dS^2 (step) or scaled by the first "
-                         "values seen (first)")
     ap.add_argument("--learned-nodes", type=int, default=0,
                     help="nodes sampled per training step of the learned predictor (0: all)")
     ap.add_argument("--stability", action="store_true",
@@ -329,7 +326,7 @@ def continuation_bench(args):
             from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
 
             predictor = LearnedPredictor(mesh, seed=0, epochs=args.learned_epochs, overlap=True,
-                                         sample_nodes=args.learned_nodes, scaling=args.learned_scaling)
+                                         sample_nodes=args.learned_nodes)
         def progress(rec, n, xF):  # long solves (restarts) report as they go
             if n > 12 and n % 5 == 0:
                 print(f"[continuation] {mode}   iteration {n}: {rec.flag} ||b|| {rec.res_norm:.2e} "

@@ -154,14 +154,6 @@ class LearnedPredictor:
     proposal is x_t plus the predicted error with its component along t
     removed, so the pseudo-arclength row t.(x - x_k) = dS still holds.
 
-    scaling="step" (default) divides the increment features by the step's
-    dS (the tangent per unit arclength) and the targets by dS^2: the tangent
-    predictor's error is x''(s) dS^2/2 + O(dS^3), so the network learns half
-    the branch's curvature per node, which changes slowly along the branch,
-    while dS grows from dS0 to dS_max; the proposal multiplies the output by
-    the next step's dS^2.  scaling="first" learns the error itself, with
-    every quantity scaled by the first values seen.
-
     overlap=True trains on a second stream (default priority) while the next
     step's Newton solve runs on the main one: the training round for an
     observation is launched after the proposal that follows it, so each
@@ -170,12 +162,8 @@ class LearnedPredictor:
     proposal, the next proposal waits for the training)."""
 
     def __init__(self, mesh: Mesh, seed: int = 0, epochs: int = 200, lr: float = 2e-3, buffer: int = 4,
-                 device: str = "cuda:0", overlap: bool = False, sample_nodes: int = 0, scaling: str = "step"):
+                 device: str = "cuda:0", overlap: bool = False, sample_nodes: int = 0):
         import torch
-
-        if scaling not in ("step", "first"):
-            raise VefError("scaling must be 'step' (targets per dS^2) or 'first' (the first values seen)")
-        self.scaling = scaling
 
         self.torch = torch
         # sample_nodes > 0: each Adam step takes that many rows of the buffer,
@@ -207,34 +195,22 @@ class LearnedPredictor:
 
     def features(self, x_cur, x_t):
         torch = self.torch
-        dS = float(torch.linalg.vector_norm(x_t - x_cur))
-        per = 1.0 / dS if (self.scaling == "step" and dS > 0) else 1.0
         u = self._nodal(x_cur).float()
-        du = ((self._nodal(x_t) - self._nodal(x_cur)) * per).float()
-        dEU = float(x_t[-1] - x_cur[-1]) * per
+        du = (self._nodal(x_t) - self._nodal(x_cur)).float()
+        dEU = float(x_t[-1] - x_cur[-1])
+        dS = float(torch.linalg.vector_norm(x_t - x_cur))
         if self.in_scales is None:
             self.in_scales = (u.std(dim=0).clamp_min(1e-12), du.abs().max(dim=0).values.clamp_min(1e-12),
                               abs(dEU) if dEU != 0 else 1.0, dS if dS > 0 else 1.0)
         s_u, s_du, s_eu, s_ds = self.in_scales
         n = u.shape[0]
         col = lambda v: torch.full((n, 1), float(v), dtype=torch.float32, device=self.device)  # noqa: E731
-        X = torch.cat([self.xy, u / s_u, du / s_du, col(dEU / s_eu), col(dS / s_ds)], dim=1).contiguous()
-        X.step_length = dS  # the step the features describe (read back by observe and propose)
-        return X
-
-    def _target_factor(self, X) -> float:
-        """dS^2 of the step the features X describe (1 with scaling="first")."""
-        if self.scaling != "step":
-            return 1.0
-        dS = getattr(X, "step_length", None)
-        if dS is None:  # features made elsewhere: dS from the last column
-            dS = float(X[0, -1]) * self.in_scales[3]
-        return dS * dS if dS > 0 else 1.0
+        return torch.cat([self.xy, u / s_u, du / s_du, col(dEU / s_eu), col(dS / s_ds)], dim=1).contiguous()
 
     def observe(self, X, err):
         """Adds (features of a tangent prediction, its error x_{k+1} - x_t) and trains."""
         torch = self.torch
-        e = self._nodal(err).float() / self._target_factor(X)
+        e = self._nodal(err).float()
         if self.err_scale is None:
             self.err_scale = e.abs().max(dim=0).values.clamp_min(1e-12)
         self.data.append((X, (e / self.err_scale).contiguous()))
@@ -294,6 +270,6 @@ class LearnedPredictor:
         torch = self.torch
         self._wait_training()
         c = torch.zeros_like(x_t)
-        c[:-1] = (self.net.forward(X).double() * self.err_scale.double() * self._target_factor(X)).reshape(-1)
+        c[:-1] = (self.net.forward(X).double() * self.err_scale.double()).reshape(-1)
         c = c - torch.dot(c, tangent) * tangent
         return x_t + c
