@@ -82,6 +82,16 @@ int vef_mf_set(vef_mf* h, int64_t count, const int64_t* dst, const double* value
  * F11^-1 F12 and gemm for F22 -= F21 F11^-1 F12. */
 int vef_mf_factor(vef_mf* h, void* hip_stream);
 
+/* Timing of the factorisation by height of the elimination tree (no
+ * counterpart in the reference: the library's own profiling).  enable != 0
+ * records HIP events on the caller's stream around each height of every
+ * later vef_mf_factor; vef_mf_profile_read waits for the last one and writes
+ * ms[2 h] = the extend-add of height h and ms[2 h + 1] = its buckets (getrf,
+ * the solves for F11^-1 F12, the Schur update), for h < vef_mf_heights. */
+int vef_mf_profile(vef_mf* h, int32_t enable);
+int32_t vef_mf_heights(const vef_mf* h);
+int vef_mf_profile_read(vef_mf* h, float* ms);
+
 /* Fronts whose last factorisation met a zero pivot (synchronises). */
 int vef_mf_zero_pivots(vef_mf* h, int32_t* count);
 

@@ -179,6 +179,10 @@ struct vef_mf {
   hipEvent_t ev_main = nullptr;
   hipEvent_t ev_side[kSide] = {};
   std::vector<int> b_stream;  // side stream of each bucket (balanced by flops within its height)
+  // vef_mf_profile: events on the caller's stream at each height's start,
+  // after its extend-add and after its buckets (2H + 1 per factorisation)
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev;
   int64_t bytes = 0;
   std::string err;
 };
@@ -207,6 +211,8 @@ static void release(vef_mf* h) {
     if (p) (void)hipFree(p);
   if (h->rb) rocblas_destroy_handle(h->rb);
   h->rb = nullptr;
+  for (hipEvent_t e : h->prof_ev) (void)hipEventDestroy(e);
+  h->prof_ev.clear();
   for (int k = 0; k < vef_mf::kSide; k++) {
     if (h->rbs[k]) rocblas_destroy_handle(h->rbs[k]);
     if (h->ev_side[k]) (void)hipEventDestroy(h->ev_side[k]);
@@ -507,6 +513,8 @@ static int factor_t(vef_mf* h, hipStream_t st) {
   T* buf = (T*)h->buf;
   const T one = T(1), mone = T(-1);
   int b = 0;
+  const bool prof = h->profiling && (int)h->prof_ev.size() == 2 * h->H + 1;
+  if (prof && hipEventRecord(h->prof_ev[0], st) != hipSuccess) return fail(h, "hipEventRecord failed (profile)");
   for (int hh = 0; hh < h->H; hh++) {
     for (int sl = 0; sl < 2; sl++) {
       int k = 2 * hh + sl;
@@ -517,6 +525,8 @@ static int factor_t(vef_mf* h, hipStream_t st) {
       }
     }
     if (hipGetLastError() != hipSuccess) return fail(h, "kernel launch failed (extend-add)");
+    if (prof && hipEventRecord(h->prof_ev[2 * hh + 1], st) != hipSuccess)
+      return fail(h, "hipEventRecord failed (profile)");
     if (fork_streams(h, st)) return -1;
     for (; b < h->nb && h->b_h[b] == hh; b++) {
       int s = h->b_s[b], u = h->b_u[b], c = h->b_c[b];
@@ -536,7 +546,32 @@ static int factor_t(vef_mf* h, hipStream_t st) {
         return fail(h, "gemm failed");
     }
     if (join_streams(h, st)) return -1;
+    if (prof && hipEventRecord(h->prof_ev[2 * hh + 2], st) != hipSuccess)
+      return fail(h, "hipEventRecord failed (profile)");
   }
+  return 0;
+}
+
+int vef_mf_profile(vef_mf* h, int32_t enable) {
+  if (!h) return -1;
+  if (enable && h->prof_ev.empty()) {
+    h->prof_ev.assign(2 * h->H + 1, nullptr);
+    for (auto& e : h->prof_ev)
+      if (hipEventCreate(&e) != hipSuccess) return fail(h, "hipEventCreate failed (profile)");
+  }
+  h->profiling = enable != 0;
+  return 0;
+}
+
+int32_t vef_mf_heights(const vef_mf* h) { return h ? h->H : -1; }
+
+int vef_mf_profile_read(vef_mf* h, float* ms) {
+  if (!h || !ms) return -1;
+  if (h->prof_ev.size() != (size_t)(2 * h->H + 1)) return fail(h, "vef_mf_profile_read: profiling was never enabled");
+  if (hipEventSynchronize(h->prof_ev.back()) != hipSuccess) return fail(h, "hipEventSynchronize failed (profile)");
+  for (int k = 0; k < 2 * h->H; k++)
+    if (hipEventElapsedTime(&ms[k], h->prof_ev[k], h->prof_ev[k + 1]) != hipSuccess)
+      return fail(h, "hipEventElapsedTime failed (profile)");
   return 0;
 }
 

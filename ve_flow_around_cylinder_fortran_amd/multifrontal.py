@@ -447,6 +447,10 @@ def _mf_lib():
         L.vef_mf_device_bytes.restype = i64
         L.vef_mf_set.argtypes = [vp, i64, vp, vp, vp]
         L.vef_mf_factor.argtypes = [vp, vp]
+        L.vef_mf_profile.argtypes = [vp, i32]
+        L.vef_mf_heights.argtypes = [vp]
+        L.vef_mf_heights.restype = i32
+        L.vef_mf_profile_read.argtypes = [vp, vp]
         L.vef_mf_zero_pivots.argtypes = [vp, ctypes.POINTER(i32)]
         L.vef_mf_solve.argtypes = [vp, vp, vp, vp]
         L.vef_mf_last_error.argtypes = [vp]
@@ -539,6 +543,17 @@ class MultifrontalLU(_border_base()):
         c = ctypes.c_int32()
         self._err(_mf_lib().vef_mf_zero_pivots(self.h, ctypes.byref(c)), "vef_mf_zero_pivots")
         return int(c.value)
+
+    def profile(self, enable: bool = True):
+        """Per-height timing of the following factorisations (HIP events)."""
+        self._err(_mf_lib().vef_mf_profile(self.h, 1 if enable else 0), "vef_mf_profile")
+
+    def profile_read(self) -> np.ndarray:
+        """[height][extend-add ms, buckets ms] of the last factorisation."""
+        L = _mf_lib()
+        ms = np.zeros(2 * int(L.vef_mf_heights(self.h)), dtype=np.float32)
+        self._err(L.vef_mf_profile_read(self.h, ctypes.c_void_p(ms.ctypes.data)), "vef_mf_profile_read")
+        return ms.reshape(-1, 2)
 
 
 class NodeBlockLU:
