@@ -207,3 +207,22 @@ def test_inlet_sensitivity_matches_central_differences(model, VeN):
         fd = (vp[:n] - vm[:n]) / (2 * h)
         assert np.abs(sens - fd).max() <= 1e-6 * max(1.0, np.abs(fd).max())
         assert np.abs(fd).max() > 1e-2  # the inlet stresses do move with Wi
+
+
+def test_non_finite_iterates_raise_inflow_error():
+    """A state whose residual is not finite (a wild Newton iterate of the
+    cylinder problem can hand the inflow problem such a Wi) ends the inflow
+    solve, and the sensitivity, with InflowError, which the arclength Newton
+    treats as divergence; never with the LU's ValueError."""
+    y = np.linspace(-H, H, 21)
+    inf = FullyDevelopedInflow(y, H, Params(BvN=0.59, VeN=0.25, model="L-PTT", DfN=1e-3))
+    inf.solve(0.5)
+    good = inf.z.copy()
+    inf.z = np.full_like(good, np.nan)
+    with pytest.raises(InflowError):
+        inf.solve(0.5)
+    with pytest.raises(InflowError):
+        inf.sensitivity(0.5)  # inf.WiN is 0.5: the stored (non-finite) state is used as it is
+    inf.z = good
+    (U, *_), _ = inf.solve(0.5)  # recovers from a finite state
+    assert np.all(np.isfinite(U))

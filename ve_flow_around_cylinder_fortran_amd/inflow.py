@@ -209,8 +209,15 @@ class FullyDevelopedInflow:
         prev = None
         for it in range(1, max_iter + 1):
             r = self.residual(z, WiN)
+            if not np.all(np.isfinite(r)):
+                # e.g. a negative conformation under a square root at a wild WiN
+                self._lu = None
+                raise InflowError(f"inflow Newton: non-finite residual at WiN = {WiN}")
             if self._lu is None:
-                self._lu = self._factor(z, WiN, r)
+                try:
+                    self._lu = self._factor(z, WiN, r)
+                except ValueError as e:  # non-finite Jacobian entries
+                    raise InflowError(f"inflow Newton: {e} at WiN = {WiN}") from None
             dz = lu_solve(self._lu, r)
             z = z - dz
             nd = float(np.linalg.norm(dz))
@@ -241,6 +248,9 @@ class FullyDevelopedInflow:
         r = self.residual(z, WiN)
         h = 1.0e-7 * max(1.0, abs(WiN))
         dR = (self.residual(z, WiN + h) - r) / h
-        dz = -lu_solve(lu_factor(self.jacobian(z, WiN, r)), dR)
+        try:
+            dz = -lu_solve(lu_factor(self.jacobian(z, WiN, r)), dR)
+        except ValueError as e:  # non-finite entries next to the solution
+            raise InflowError(f"inflow sensitivity: {e} at WiN = {WiN}") from None
         nod = dz[:-1].reshape(self.nn, 5)
         return tuple(nod[:, q].copy() for q in range(5))

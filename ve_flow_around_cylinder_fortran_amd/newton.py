@@ -281,22 +281,28 @@ class NewtonSolver:
                 self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
                                    x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
                 self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
+                coupled = True
                 if jac and self.inlet_coupling and increment >= 3:
-                    self._couple_inlet(EU)
+                    try:
+                        self._couple_inlet(EU)
+                    except InflowError:
+                        coupled = False  # no inflow sensitivity at this iterate's Wi
                 res_norm = float(torch.linalg.vector_norm(self.b))
                 if not math.isfinite(res_norm):
                     raise NewtonError("non-finite residual")  # the reference's general stop on NaN
-                try:
-                    if jac:
-                        self.lu.factor(self.values)
-                        self.factored = True
-                    s = self.lu.solve(self.b)
-                    cor_new = float(torch.linalg.vector_norm(s))
-                except RfLUError:
-                    # a Jacobian this factorisation cannot solve (zero pivots at a
-                    # wandering iterate): under MUMPS's static pivoting the
-                    # reference sees a huge correction, i.e. divergence
-                    s, cor_new = None, math.inf
+                s, cor_new = None, math.inf  # divergence unless a correction is found
+                if coupled:
+                    try:
+                        if jac:
+                            self.lu.factor(self.values)
+                            self.factored = True
+                        s = self.lu.solve(self.b)
+                        cor_new = float(torch.linalg.vector_norm(s))
+                    except RfLUError:
+                        # a Jacobian this factorisation cannot solve (zero pivots at a
+                        # wandering iterate): under MUMPS's static pivoting the
+                        # reference sees a huge correction, i.e. divergence
+                        s, cor_new = None, math.inf
                 if not math.isfinite(cor_new):
                     cor_new = math.inf
                 hist.append(NewtonRecord(it, flag, res_norm, cor_new, 0.0))
