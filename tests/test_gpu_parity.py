@@ -128,6 +128,57 @@ def test_bitwise_some_exact_divisions(oracle):
     check_bitwise(oracle, mesh, PARAM_SETS["giesekus"], x, "indefinite S on 3 nodes, 2^70 S01 on 2")
 
 
+def three_quad_triangle():
+    """A triangle cut into three quadrilaterals at its centroid (Q2 nodes):
+    each pair of elements shares an edge, so the elements cannot be split into
+    two colours with no shared edge inside a colour, and the centroid belongs to
+    all three."""
+    A, B, C = np.array([-3.0, -1.2]), np.array([2.5, -0.9]), np.array([-0.4, 1.6])
+    O = (A + B + C) / 3.0
+    quads = [(A, (A + B) / 2, O, (C + A) / 2), (B, (B + C) / 2, O, (A + B) / 2), (C, (C + A) / 2, O, (B + C) / 2)]
+    nodes, conn = {}, []
+
+    def node(p):
+        return nodes.setdefault((round(p[0], 12), round(p[1], 12)), len(nodes))
+
+    for q in quads:
+        c = [np.asarray(p, float) for p in q]
+        if np.cross(c[1] - c[0], c[3] - c[0]) < 0:  # counter-clockwise corners
+            c = [c[0], c[3], c[2], c[1]]
+        mids = [(c[i] + c[(i + 1) % 4]) / 2 for i in range(4)]
+        conn.append([node(p) for p in c + mids + [sum(c) / 4]])
+    coords = np.array(sorted(nodes, key=nodes.get), float)
+    return finish_mesh(coords, np.array(conn), dict(R=1.0, H=4.0, L_up=15.0, L_down=15.0))
+
+
+def test_bitwise_three_colour_mesh(oracle):
+    """A mesh whose shared-edge graph has an odd cycle: the Jacobian pass runs
+    in three colour launches (greedy colouring), the centroid's pair of three
+    elements goes through the ordered sums, and every value stays bitwise."""
+    mesh = three_quad_triangle()
+    assert mesh.n_elements == 3 and mesh.n_nodes == 19 and len(mesh.outlet_edges) == 0
+    for name in ("giesekus", "inertia"):
+        check_bitwise(oracle, mesh, PARAM_SETS[name], smooth_state(mesh, EU=0.7), f"three colours {name}")
+
+
+def test_bitwise_huge_pressures(oracle):
+    """Pressures of 1e300 and 1e305 on three nodes: P enters neither S nor the
+    tau's, so the Gauss-point range checks pass and the wavefronts stay in the
+    fast pass, while their element sums reach 1e306.  Thousands of
+    forward-difference quotients then lie above 2^900, outside the fast
+    division's window (the CPU oracle gives 5,000 to 7,400 of them and no
+    non-finite value), and each such entry takes the IEEE quotient on its
+    own."""
+    mesh = cylinder_channel(4, 2, 3, 4)
+    x = smooth_state(mesh, EU=0.8)
+    u = x[:-1].reshape(-1, 6)
+    u[mesh.conn[[5, 22], 8], 2] = 1e300
+    u[mesh.conn[[12], 8], 2] = 1e305
+    x[:-1] = u.ravel()
+    for name in ("lptt", "inertia"):
+        check_bitwise(oracle, mesh, PARAM_SETS[name], x, f"huge pressures {name}")
+
+
 @pytest.mark.parametrize("turns", [1, 2, 3])
 def test_bitwise_rotated_local_numbering(oracle, turns):
     """Every element's local numbering turned by a quarter turn or more, so the

@@ -8,16 +8,19 @@
 //                           into r0bank[e][54].
 //   vef_edge_kernel         outlet-edge traction (openBC), 3 edge Gauss points
 //                           x 56 evaluations per outlet edge, into ebank.
-//   vef_bulk_kernel<1,7,56> Jacobian pass: eight single-wavefront workgroups
-//                           per element, 63 lanes = 9 Gauss points x 7 of the
-//                           54 nodal perturbations and the EU perturbation.
-//                           Each lane evaluates the Gauss-point strong form and
-//                           the 9 test-node rows; an ordered LDS reduction over
-//                           the Gauss points gives each evaluation's 54 rows,
-//                           and the quotients (R_k - R_0)/h go straight to the
-//                           CSR values when the node pair belongs to this
-//                           element only, else to cbank, and dR/dEU to acbank.
-//   vef_shared_kernel       values of node pairs shared by several elements:
+//   vef_bulk_kernel<1,7,56> Jacobian pass, one launch per element colour:
+//                           eight single-wavefront workgroups per element, 63
+//                           lanes = 9 Gauss points x 7 of the 54 nodal
+//                           perturbations and the EU perturbation.  Each lane
+//                           evaluates the Gauss-point strong form and the 9
+//                           test-node rows; an ordered LDS reduction over the
+//                           Gauss points gives each evaluation's 54 rows, and
+//                           the quotients (R_k - R_0)/h go straight to the CSR
+//                           values when the node pair belongs to this element
+//                           only, are stored (first colour) or added (second)
+//                           when it belongs to two elements, and go to cbank
+//                           when it belongs to more; dR/dEU goes to acbank.
+//   vef_shared_kernel       values of node pairs of three or more elements:
 //                           their cbank entries summed in ascending element
 //                           order (the order of the reference's ADD_VALUES).
 //   vef_edge_add_kernel     outlet-edge quotients added after the bulk ones,
@@ -185,10 +188,9 @@ __device__ __forceinline__ double cdiv(double a, double c) { return (a * c) / (c
 // residual: d in [2^-60, 2^40] and |q0| in [2^-901, 2^900), or a = +-0 (then
 // q0 = +-0, e = +0 and q1 = q0 keeps the sign of a/d because d > 0).  Instead
 // of branching, each division widens the exponent interval [lo, hi] of its q0
-// in a DivCheck, and a denominator outside its range sets hi to kBadExp; a
-// non-finite q0 turns q1 into NaN, which reaches the element residual.  The
-// caller repeats the whole evaluation in the exact mode when the interval or
-// the residual says so.
+// in a DivCheck, and a denominator outside its range or a non-finite q0 sets
+// hi to kBadExp.  The caller repeats the whole evaluation in the exact mode
+// when the interval says so.
 //
 // Exact (EXACT = true): the IEEE division a/d.
 struct Recip {
@@ -201,11 +203,17 @@ struct DivCheck {
 };
 constexpr int kBadExp = 4096;
 
+// The binade of x (0 for +-0), or kBadExp for an infinity or a NaN, whose
+// frexp exponent (0) would pass every range check.
+__device__ __forceinline__ int binade(double x) {
+  return __builtin_isfinite(x) ? __builtin_amdgcn_frexp_exp(x) : kBadExp;
+}
+
 // Base quantities of the untracked divisions: x = 0 or 2^-64 <= |x| < 2^64.
 template <bool EXACT>
 __device__ __forceinline__ void base_range(double x, DivCheck& dc) {
   if constexpr (!EXACT) {
-    const int ex = __builtin_amdgcn_frexp_exp(x);
+    const int ex = binade(x);
     dc.blo = min(dc.blo, ex);
     dc.bhi = max(dc.bhi, ex);
   }
@@ -230,7 +238,7 @@ __device__ __forceinline__ double qdiv(double a, const Recip& r, DivCheck& dc) {
   } else {
     const double q0 = a * r.y;
     const double e = fma(r.d, q0, -a);
-    const int ex = __builtin_amdgcn_frexp_exp(q0);  // 0 for +-0; the binade of q0 otherwise
+    const int ex = binade(q0);
     dc.lo = min(dc.lo, ex);
     dc.hi = max(dc.hi, ex);
     return fma(-e, r.y, q0);
@@ -256,6 +264,24 @@ __device__ __forceinline__ double fd_quotient(double rk, double r0, DivCheck& dc
   // (dRes - Res)/eps_jac with a complex numerator: ((dR-R)*h)/(h*h)
   constexpr double hh = kEpsJac * kEpsJac;
   return qdiv<EXACT>((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh}, dc);
+}
+// The same quotient decided per entry: the two-FMA division where its q0 lies
+// in the window, the IEEE division otherwise (rare: a divergent branch).
+template <bool EXACT>
+__device__ __forceinline__ double fd_quotient_entry(double rk, double r0) {
+  constexpr double hh = kEpsJac * kEpsJac;
+  const double a = (rk - r0) * kEpsJac;
+  if constexpr (EXACT) {
+    return a / hh;
+  } else {
+    constexpr double y = 1.0 / hh;
+    const double q0 = a * y;
+    const double e = fma(hh, q0, -a);
+    double q = fma(-e, y, q0);
+    const int ex = binade(q0);
+    if (ex < -900 || ex > 900) q = a / hh;
+    return q;
+  }
 }
 __device__ __forceinline__ double fd_quotient_ieee(double rk, double r0) {
   DivCheck dc{0, 0, 0, 0};
@@ -298,8 +324,8 @@ __device__ void geometry(const double* X /*[9][2]*/, const double* dc, const dou
   int lo = 0, hi = 0;
 #pragma unroll
   for (int n = 0; n < 9; n++) {
-    lo = min(lo, min(__builtin_amdgcn_frexp_exp(G.dFx[n]), __builtin_amdgcn_frexp_exp(G.dFy[n])));
-    hi = max(hi, max(__builtin_amdgcn_frexp_exp(G.dFx[n]), __builtin_amdgcn_frexp_exp(G.dFy[n])));
+    lo = min(lo, min(binade(G.dFx[n]), binade(G.dFy[n])));
+    hi = max(hi, max(binade(G.dFx[n]), binade(G.dFy[n])));
   }
   G.in_range = lo >= -63 && hi <= 64;
 }
@@ -614,13 +640,19 @@ struct PlainState {
 // residual and FD Jacobian (8 workgroups per element), NEVAL = 1 the residual
 // only (7 elements per workgroup).
 // ----------------------------------------------------------------------------
-// Where the FD quotients go (NEVAL = 56).  For element e and its node pair
-// (jn, in), code[e*81 + jn*9 + in] is pos<<1 when the element is the only one
-// containing both nodes — the 36 matrix entries of the pair are then written
-// directly, at rowptr[6r] + jq*len(r) + pos*6 + iq (r = node jn, every row of a
-// node has the same length) — and (cidx<<1)|1 otherwise, when they go to
-// cbank[cidx*36 + jq*6 + iq] for the ordered sum of vef_shared_kernel.  The
-// dR/dEU quotients go to acbank[e*54 + row].
+// Where the Jacobian entries go.  For element e and its node pair (jn, in),
+// code[e*81 + jn*9 + in] is
+//   pos<<2 | kCodeStore  when the element is the only one containing both
+//       nodes, or the first (by colour) of the two that do: the 36 entries of
+//       the pair are written directly, at rowptr[6r] + jq*len(r) + pos*6 + iq
+//       (r = node jn; every row of a node has the same length);
+//   pos<<2 | kCodeAdd    for the second of two: added to what the first wrote
+//       (the elements of one colour share no such pair, and the colours are
+//       launched in order);
+//   cidx<<2 | kCodeBank  when three or more elements contain both: to
+//       cbank[cidx*36 + jq*6 + iq] for the ordered sum of vef_shared_kernel.
+// The dR/dEU quotients go to acbank[e*54 + row].
+constexpr int kCodeStore = 0, kCodeBank = 1, kCodeAdd = 2;
 struct FdOut {
   const double* r0;      // [n_el][54] base residual rows (residual pass)
   const int32_t* code;   // [n_el][81]
@@ -630,6 +662,8 @@ struct FdOut {
   double* acbank;
   double* geo;           // [n_el][9] Geo records: written by the residual pass when a
                          // Jacobian pass follows (else NULL), read by the Jacobian pass
+  const int32_t* elist;  // Jacobian passes: the elements of this launch (one colour)
+  int32_t n_list;
 };
 constexpr int kGeoWords = (int)(sizeof(Geo) / sizeof(double));
 static_assert(sizeof(Geo) % sizeof(double) == 0, "Geo is a whole number of 8-byte words");
@@ -667,8 +701,8 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
     egroup = blockIdx.x / BPE;
     kg = blockIdx.x % BPE;
   }
-  const int64_t e0 = egroup * EPB;
-  if (FD && e0 >= n_el) return;  // padding workgroups of the XCD-aware numbering (uniform)
+  if (FD && egroup >= fo.n_list) return;  // padding workgroups of the XCD-aware numbering (uniform)
+  const int64_t e0 = FD ? (int64_t)fo.elist[egroup] : egroup * EPB;
   const double EU = x[eu_index];
   const int64_t elast = (int64_t)n_el - 1;
 
@@ -794,9 +828,15 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
   // One pass over the wavefront's 7 slots x 9 Gauss points: Gauss-point
   // quantities, then per test node the 6 terms, their ordered sum over the
   // Gauss points and its destination.  The fast pass divides with shared
-  // reciprocals; if any lane's DivCheck or any sum says the fast divisions may
-  // differ from IEEE ones, the exact pass rewrites every output of the
-  // workgroup (the branch is uniform: one wavefront per workgroup).
+  // reciprocals.  Whether its divisions equal IEEE ones is settled before the
+  // first store: the Gauss-point divisions are tracked and the test-node
+  // divisions bounded by the checked ranges of their inputs (node_terms), so
+  // if any lane's DivCheck fails the wavefront goes to the exact pass before
+  // writing anything (the branch is uniform: one wavefront per workgroup),
+  // and otherwise its sums equal the exact pass's bit for bit and only the
+  // forward-difference quotients remain, each decided on its own
+  // (fd_quotient_entry).  Every output is therefore written exactly once,
+  // which the in-place sums of the two-contributor entries need.
   auto evaluate = [&](auto exact_tag) -> bool {
     constexpr bool EXACT = decltype(exact_tag)::value;
     DivCheck dc{0, 0, 0, 0};
@@ -806,48 +846,69 @@ vef_bulk_kernel(const double* __restrict__ coords, const int32_t* __restrict__ c
       PlainState st{sS[s]};
       gauss_point<MODEL, DCS, INERTIA, EXACT>(st, c_tab.bf[g], sG[el][g], EUk, prm, q, dc);
     }
+    if constexpr (!EXACT) {
+      if (__ballot(dc.failed()) != 0) return true;
+    }
     const double WET = sG[el][g].WET;
 
     for (int n = 0; n < 9; n++) {
       __syncthreads();  // the previous sums have been read
+      // The destination of this lane's quotient for test node n, and, for the
+      // second of two contributors, the first one's value: loaded here so that
+      // the test-node terms below hide its latency.
+      const int c = tid / NSLOT, ss = tid % NSLOT;
+      const int eel = ss / KPB, kk = kg * KPB + ss % KPB;
+      const bool sum_lane = tid < 6 * NSLOT && e0 + eel < n_el;
+      double* dst = nullptr;
+      double first = 0.0;
+      bool add = false;
+      if constexpr (FD) {
+        const int in = (kk - 1) / 6, iq = (kk - 1) % 6;
+        const int32_t code = sCode[n * 9 + min(max(in, 0), 8)];
+        const int kind = code & 3;
+        double* pv = fo.values + (sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 2) * 6 + iq);
+        double* pc = fo.cbank + ((int64_t)(code >> 2) * 36 + c * 6 + iq);
+        double* pa = fo.acbank + (e0 * 54 + n * 6 + c);
+        dst = kk == NEVAL - 1 ? pa : (kind == kCodeBank ? pc : pv);
+        add = sum_lane && kk > 0 && kk != NEVAL - 1 && kind == kCodeAdd;
+        if (add) first = *pv;
+      }
       if (valid) {
         double r[6];
         node_terms<MODEL, DCS, INERTIA, EXACT>(q, c_tab.bf[g][n], sG[el][g].dFx[n], sG[el][g].dFy[n], prm, WET,
                                                r);
 #pragma unroll
-        for (int c = 0; c < 6; c++) sRed[g][s][c] = r[c];
+        for (int cc = 0; cc < 6; cc++) sRed[g][s][cc] = r[cc];
       }
       __syncthreads();
       // Ordered sum over the Gauss points: Res = Res + TERM_RES*WET, ig = 1..9.
-      if (tid < 6 * NSLOT) {
-        const int c = tid / NSLOT, ss = tid % NSLOT;
-        const int eel = ss / KPB, kk = kg * KPB + ss % KPB;
-        if (e0 + eel < n_el) {
-          double acc = sRed[0][ss][c];
+      if (sum_lane) {
+        double acc = sRed[0][ss][c];
 #pragma unroll
-          for (int gg = 1; gg < 9; gg++) acc = acc + sRed[gg][ss][c];
-          if (!EXACT && !__builtin_isfinite(acc)) dc.hi = kBadExp;
-          if (!FD) {
-            bank[(e0 + eel) * 54 + n * 6 + c] = acc;
-          } else if (kk > 0) {
-            // (dRes - Res)/eps_jac for column kk-1 (or EU for kk = 55), as
-            // ADD_VALUES into zeroed storage (+0 for -0).  The ordered sums of
-            // cbank and acbank start from +0 as well, so every destination can
-            // take 0 + q: the destination is selected, not branched on.
-            const double qv = 0.0 + fd_quotient<EXACT>(acc, sR0[n * 6 + c], dc);
-            const int in = (kk - 1) / 6, iq = (kk - 1) % 6;
-            const int32_t code = sCode[n * 9 + min(in, 8)];
-            double* pv = fo.values + (sRb[n] + (int64_t)c * sRl[n] + (int64_t)(code >> 1) * 6 + iq);
-            double* pc = fo.cbank + ((int64_t)(code >> 1) * 36 + c * 6 + iq);
-            double* pa = fo.acbank + (e0 * 54 + n * 6 + c);
-            *(kk == NEVAL - 1 ? pa : ((code & 1) ? pc : pv)) = qv;
-          }
+        for (int gg = 1; gg < 9; gg++) acc = acc + sRed[gg][ss][c];
+        if (!FD) {
+          bank[(e0 + eel) * 54 + n * 6 + c] = acc;
+        } else if (kk > 0) {
+          // (dRes - Res)/eps_jac for column kk-1 (or EU for kk = 55), as
+          // ADD_VALUES into zeroed storage (+0 for -0).  The ordered sums of
+          // cbank and acbank start from +0 as well, so every destination but
+          // the second of two contributors takes 0 + q (the destination is
+          // selected, not branched on); the second adds its 0 + q to the
+          // first's: (0 + a) + (0 + b) = (0 + a) + b = (0 + b) + a, the
+          // reference's sum in either element order.
+          const double qv = 0.0 + fd_quotient_entry<EXACT>(acc, sR0[n * 6 + c]);
+          *dst = add ? first + qv : qv;
         }
       }
     }
-    return dc.failed();
+    return false;
   };
-  if (__ballot(evaluate(std::false_type{})) != 0) evaluate(std::true_type{});
+  const bool exact = evaluate(std::false_type{});
+  // A fence between the passes: without it the compiler interleaves the exact
+  // pass with the fast one's node loop, and at the 128-VGPR limit of this
+  // occupancy that spills.
+  __syncthreads();
+  if (exact) evaluate(std::true_type{});
 }
 
 // ----------------------------------------------------------------------------
@@ -916,8 +977,8 @@ vef_jac32_kernel(const int32_t* __restrict__ conn, const double* __restrict__ x,
   __shared__ int64_t sRb[9];
   __shared__ int32_t sRl[9];
   const int tid = threadIdx.x;
-  const int64_t e = blockIdx.x;
-  if (e >= n_el) return;  // uniform over the workgroup
+  if ((int64_t)blockIdx.x >= fo.n_list) return;  // uniform over the workgroup
+  const int64_t e = fo.elist[blockIdx.x];
   const float EU = (float)x[eu_index];
   if (tid < 54) sU[tid] = (float)x[(int64_t)conn[e * 9 + tid / 6] * 6 + tid % 6];
   if (tid < 81) sCode[tid] = fo.code[e * 81 + tid];
@@ -1060,23 +1121,26 @@ vef_jac32_kernel(const int32_t* __restrict__ conn, const double* __restrict__ x,
   }
   __syncthreads();
   // stores: one thread per (test node n, row q, trial node m): the 6-entry run
-  // of row (n, q) in node m's block, to values (single contributor) or cbank;
-  // then the 54 entries of dR/dEU to acbank
+  // of row (n, q) in node m's block, stored into values (single or first of
+  // two contributors), added there (second of two) or stored to cbank; then
+  // the 54 entries of dR/dEU to acbank
   for (int it = tid; it < 9 * 6 * 9 + 54; it += kThreads) {
     if (it < 9 * 6 * 9) {
       const int n = it / 54, q = (it / 9) % 6, m = it % 9;
       const float2* src = reinterpret_cast<const float2*>(&sJ[n * 330 + q * 54 + m * 6]);  // even index
       const float2 v01 = src[0], v23 = src[1], v45 = src[2];
       const int32_t cd = sCode[n * 9 + m];
-      double* dst = (cd & 1) ? fo.cbank + ((int64_t)(cd >> 1) * 36 + q * 6)
-                             : fo.values + (sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 1) * 6);
+      double* dst = (cd & 3) == kCodeBank ? fo.cbank + ((int64_t)(cd >> 2) * 36 + q * 6)
+                                          : fo.values + (sRb[n] + (int64_t)q * sRl[n] + (int64_t)(cd >> 2) * 6);
       // +0 for -0, as ADD_VALUES into zeroed storage
-      dst[0] = 0.0 + (double)v01.x;
-      dst[1] = 0.0 + (double)v01.y;
-      dst[2] = 0.0 + (double)v23.x;
-      dst[3] = 0.0 + (double)v23.y;
-      dst[4] = 0.0 + (double)v45.x;
-      dst[5] = 0.0 + (double)v45.y;
+      double v[6] = {0.0 + (double)v01.x, 0.0 + (double)v01.y, 0.0 + (double)v23.x,
+                     0.0 + (double)v23.y, 0.0 + (double)v45.x, 0.0 + (double)v45.y};
+      if ((cd & 3) == kCodeAdd) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) v[i] = dst[i] + v[i];
+      }
+#pragma unroll
+      for (int i = 0; i < 6; i++) dst[i] = v[i];
     } else {
       const int i = it - 9 * 6 * 9, n = i / 6;
       fo.acbank[e * 54 + i] = 0.0 + (double)sJ[n * 330 + 324 + i % 6];
@@ -1809,11 +1873,19 @@ struct vef_handle {
   int32_t* d_col = nullptr;
   int64_t* d_diag_pair = nullptr;
   int64_t n_pairs = 0;
-  // direct scatter map and the ordered sums of shared entries
-  int32_t* d_code = nullptr;     // [n_el][81]
-  int64_t* d_sbase = nullptr;    // shared pairs: first slot
+  // Scatter of the Jacobian passes (FdOut codes): direct stores, in-place
+  // sums of two-contributor entries over the element colours, and the ordered
+  // sums of the entries of three or more elements
+  int32_t* d_code_fd = nullptr;  // [n_el][81]
+  int32_t* d_elist = nullptr;    // elements by colour, ascending within a colour
+  std::vector<int32_t> colour_off;  // colour c: d_elist[colour_off[c] .. colour_off[c+1])
+  int64_t* d_sbase = nullptr;    // pairs of three or more elements: first slot
   int32_t* d_slen = nullptr;     //   row length
-  int32_t* d_ustart = nullptr;   //   first cbank row of each shared pair
+  int32_t* d_ustart_m = nullptr; //   first cbank row
+  int64_t n_shared_m = 0, n_cidx_m = 0;
+  // Scatter of the stability assembly: every shared pair through its banks
+  int32_t* d_code = nullptr;     // [n_el][81]: pos<<1, or (cidx<<1)|1 for a shared pair
+  int32_t* d_ustart = nullptr;   //   first bank row of each shared pair
   int64_t n_shared = 0, n_cidx = 0;
   int64_t* d_ep_pair = nullptr;  // pairs with outlet-edge contributions
   int64_t* d_ebase = nullptr;
@@ -1822,7 +1894,7 @@ struct vef_handle {
   double* d_r0bank = nullptr;    // [n_el][54] base residual rows
   double* d_geob = nullptr;      // [n_el][9] Geo records of the residual pass, for the Jacobian pass
   double* d_acbank = nullptr;    // [n_el][54] dR/dEU quotients
-  double* d_cbank = nullptr;     // [n_cidx][36] quotients of shared entries
+  double* d_cbank = nullptr;     // [n_cidx_m][36] quotients of entries of three or more elements
   int32_t* d_cstart = nullptr;
   int32_t* d_contrib = nullptr;
   int32_t* d_estart = nullptr;
@@ -1922,12 +1994,18 @@ int upload_tables(vef_handle* h) {
 namespace {
 template <int MODEL, bool DCS, bool INERTIA>
 void launch_bulk_t(vef_handle* h, int jac, double* values, const double* x, int64_t eu, hipStream_t st, double* geo) {
-  FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank, geo};
-  if (jac)  // 8 workgroups per element, elements padded to a multiple of 8 (XCD-aware ids)
-    hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>),
-                       dim3((unsigned)((h->n_el + 7) / 8) * 64), dim3(64), 0, st, h->d_coords, h->d_conn, x, eu,
-                       h->n_el, h->dprm, (double*)nullptr, fo);
-  else
+  FdOut fo{h->d_r0bank, h->d_code_fd, h->d_rowptr, values, h->d_cbank, h->d_acbank, geo, nullptr, 0};
+  if (jac) {
+    // one launch per colour, in colour order; 8 workgroups per element, the
+    // colour's elements padded to a multiple of 8 (XCD-aware ids)
+    for (size_t c = 0; c + 1 < h->colour_off.size(); c++) {
+      fo.elist = h->d_elist + h->colour_off[c];
+      fo.n_list = h->colour_off[c + 1] - h->colour_off[c];
+      hipLaunchKernelGGL((vef_bulk_kernel<1, 7, kNeval, MODEL, DCS, INERTIA>),
+                         dim3((unsigned)((fo.n_list + 7) / 8) * 64), dim3(64), 0, st, h->d_coords, h->d_conn, x, eu,
+                         h->n_el, h->dprm, (double*)nullptr, fo);
+    }
+  } else
     hipLaunchKernelGGL((vef_bulk_kernel<7, 1, 1, MODEL, DCS, INERTIA>), dim3((unsigned)(h->n_el + 6) / 7), dim3(64), 0,
                        st, h->d_coords, h->d_conn, x, eu, h->n_el, h->dprm, h->d_r0bank, fo);
 }
@@ -1976,6 +2054,7 @@ void vef_destroy(vef_handle* h) {
   void* ptrs[] = {h->d_coords, h->d_conn,   h->d_edges,  h->d_rowptr,  h->d_col,     h->d_diag_pair,
                   h->d_cstart, h->d_contrib, h->d_estart, h->d_econtrib, h->d_code,    h->d_sbase,
                   h->d_slen,   h->d_ustart,  h->d_ep_pair,  h->d_ebase,   h->d_elen,
+                  h->d_code_fd, h->d_elist, h->d_ustart_m,
                   h->d_r0bank, h->d_acbank,  h->d_cbank,  h->d_ebank,    h->d_arc,     h->d_tmp_x,
                   h->d_drag_edges, h->d_drag_terms, h->d_bc_rows, h->d_bc_values,
                   h->d_tmp_b,  h->d_tmp_v, h->d_rowptr_s, h->d_sbase_s, h->d_slen_s,
@@ -2119,11 +2198,78 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
     for (int64_t c = 0; c < h->N; c++) h->h_col[p++] = (int32_t)c;
   }
 
-  // Direct-scatter codes, shared pairs and their contribution lists.
-  std::vector<int32_t> code((size_t)n_el * 81);
+  // Element colours for the Jacobian passes: the two elements of a
+  // two-contributor pair get different colours (breadth-first two-colouring of
+  // that graph; greedy colouring in element order where it has an odd cycle).
+  std::vector<int32_t> colour(n_el, -1);
+  {
+    std::vector<std::pair<int32_t, int32_t>> ce;
+    for (int64_t pp = 0; pp < npairs; pp++)
+      if (cstart[pp + 1] - cstart[pp] == 2) {
+        const int32_t a = contrib[cstart[pp]] / 81, b = contrib[cstart[pp] + 1] / 81;
+        ce.push_back({a, b});
+        ce.push_back({b, a});
+      }
+    std::sort(ce.begin(), ce.end());
+    ce.erase(std::unique(ce.begin(), ce.end()), ce.end());
+    std::vector<int64_t> adj0(n_el + 1, 0);
+    for (const auto& p : ce) adj0[p.first + 1]++;
+    for (int32_t e = 0; e < n_el; e++) adj0[e + 1] += adj0[e];
+    bool two = true;
+    std::vector<int32_t> queue;
+    for (int32_t s0 = 0; s0 < n_el && two; s0++) {
+      if (colour[s0] >= 0) continue;
+      colour[s0] = 0;
+      queue.assign(1, s0);
+      for (size_t qi = 0; qi < queue.size() && two; qi++) {
+        const int32_t u = queue[qi];
+        for (int64_t t = adj0[u]; t < adj0[u + 1]; t++) {
+          const int32_t v = ce[t].second;
+          if (colour[v] < 0) {
+            colour[v] = 1 - colour[u];
+            queue.push_back(v);
+          } else if (colour[v] == colour[u]) {
+            two = false;
+            break;
+          }
+        }
+      }
+    }
+    if (!two) {
+      std::vector<char> used;
+      for (int32_t e = 0; e < n_el; e++) colour[e] = -1;
+      for (int32_t e = 0; e < n_el; e++) {
+        used.assign(used.size(), 0);
+        for (int64_t t = adj0[e]; t < adj0[e + 1]; t++) {
+          const int32_t c = colour[ce[t].second];
+          if (c >= 0) {
+            if ((size_t)c >= used.size()) used.resize(c + 1, 0);
+            used[c] = 1;
+          }
+        }
+        int32_t c = 0;
+        while ((size_t)c < used.size() && used[c]) c++;
+        colour[e] = c;
+      }
+    }
+    int32_t n_colours = 0;
+    for (int32_t e = 0; e < n_el; e++) n_colours = std::max(n_colours, colour[e] + 1);
+    h->colour_off.assign(n_colours + 1, 0);
+    for (int32_t e = 0; e < n_el; e++) h->colour_off[colour[e] + 1]++;
+    for (int32_t c = 0; c < n_colours; c++) h->colour_off[c + 1] += h->colour_off[c];
+  }
+  std::vector<int32_t> elist(std::max(n_el, 1));
+  {
+    std::vector<int32_t> fill(h->colour_off.begin(), h->colour_off.end());
+    for (int32_t e = 0; e < n_el; e++) elist[fill[colour[e]]++] = e;
+  }
+
+  // Scatter codes (the Jacobian passes' and the stability assembly's), the
+  // pairs summed from banks and their contribution lists.
+  std::vector<int32_t> code((size_t)n_el * 81), code_fd((size_t)n_el * 81);
   std::vector<int64_t> sbase;
-  std::vector<int32_t> slen, ustart(1, 0);
-  int64_t n_cidx = 0;
+  std::vector<int32_t> slen, ustart(1, 0), ustart_m(1, 0);
+  int64_t n_cidx = 0, n_cidx_m = 0;
   std::vector<int64_t> ep_pair, ebase;
   std::vector<int32_t> elen;
   for (int64_t pp = 0; pp < npairs; pp++) {
@@ -2132,20 +2278,32 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
     const int64_t rb = h->h_rowptr[(int64_t)r * 6];
     const int32_t rl = (int32_t)(h->h_rowptr[(int64_t)r * 6 + 1] - rb);
     const int32_t c0 = cstart[pp], c1 = cstart[pp + 1];
+    auto at = [](int32_t w) { return (size_t)(w / 81) * 81 + w % 81; };
     if (c1 - c0 == 1) {
-      const int32_t w = contrib[c0];
-      code[(size_t)(w / 81) * 81 + w % 81] = (int32_t)(pos << 1);
+      code[at(contrib[c0])] = (int32_t)(pos << 1);
+      code_fd[at(contrib[c0])] = (int32_t)(pos << 2) | kCodeStore;
     } else {
-      sbase.push_back(rb + pos * 6);
-      slen.push_back(rl);
       h->h_sbase_s.push_back(rb - (int64_t)r * 6 + pos * 6);  // rows before r lose their arclength entry
       h->h_slen_s.push_back(rl - 1);
       for (int32_t c = c0; c < c1; c++) {
-        const int32_t w = contrib[c];
-        code[(size_t)(w / 81) * 81 + w % 81] = (int32_t)((n_cidx << 1) | 1);
+        code[at(contrib[c])] = (int32_t)((n_cidx << 1) | 1);
         n_cidx++;
       }
       ustart.push_back((int32_t)n_cidx);
+      if (c1 - c0 == 2) {
+        const int32_t wa = contrib[c0], wb = contrib[c0 + 1];
+        const bool a_first = colour[wa / 81] < colour[wb / 81];
+        code_fd[at(wa)] = (int32_t)(pos << 2) | (a_first ? kCodeStore : kCodeAdd);
+        code_fd[at(wb)] = (int32_t)(pos << 2) | (a_first ? kCodeAdd : kCodeStore);
+      } else {
+        sbase.push_back(rb + pos * 6);
+        slen.push_back(rl);
+        for (int32_t c = c0; c < c1; c++) {
+          code_fd[at(contrib[c])] = (int32_t)((n_cidx_m << 2) | kCodeBank);
+          n_cidx_m++;
+        }
+        ustart_m.push_back((int32_t)n_cidx_m);
+      }
     }
     if (estart[pp + 1] > estart[pp]) {
       ep_pair.push_back(pp);
@@ -2153,8 +2311,10 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
       elen.push_back(rl);
     }
   }
-  h->n_shared = (int64_t)sbase.size();
+  h->n_shared = (int64_t)h->h_sbase_s.size();
   h->n_cidx = n_cidx;
+  h->n_shared_m = (int64_t)sbase.size();
+  h->n_cidx_m = n_cidx_m;
   h->n_epairs = (int64_t)ep_pair.size();
 
   // Device copies.
@@ -2178,16 +2338,19 @@ int vef_create(int32_t n_nodes, int32_t n_el, const double* coords, const int32_
   UP(h->d_estart, estart.data(), estart.size());
   UP(h->d_econtrib, econtrib.data(), econtrib.size());
   UP(h->d_code, code.data(), code.size());
+  UP(h->d_ustart, ustart.data(), ustart.size());
+  UP(h->d_code_fd, code_fd.data(), code_fd.size());
+  UP(h->d_elist, elist.data(), elist.size());
   UP(h->d_sbase, sbase.data(), sbase.size());
   UP(h->d_slen, slen.data(), slen.size());
-  UP(h->d_ustart, ustart.data(), ustart.size());
+  UP(h->d_ustart_m, ustart_m.data(), ustart_m.size());
   UP(h->d_ep_pair, ep_pair.data(), ep_pair.size());
   UP(h->d_ebase, ebase.data(), ebase.size());
   UP(h->d_elen, elen.data(), elen.size());
   UP(h->d_r0bank, (const double*)nullptr, (size_t)n_el * 54);
   UP(h->d_geob, (const double*)nullptr, (size_t)n_el * 9 * kGeoWords);
   UP(h->d_acbank, (const double*)nullptr, (size_t)n_el * 54);
-  UP(h->d_cbank, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx, 1) * 36);
+  UP(h->d_cbank, (const double*)nullptr, (size_t)std::max<int64_t>(h->n_cidx_m, 1) * 36);
   UP(h->d_ebank, (const double*)nullptr, (size_t)std::max(n_edges, 1) * 18 * kNeval);
   h->arc_blocks = (int)((n_nodes + kArcChunk - 1) / kArcChunk);
   UP(h->d_arc, (const double*)nullptr, (size_t)6 * h->arc_blocks);
@@ -2234,20 +2397,25 @@ int vef_assemble(vef_handle* h, const double* x, int32_t flag, double* b, double
     launch_bulk(h, 1, values, x, eu, st, h->d_geob);
     VEF_HIP(h, hipGetLastError());
   } else if (flag == VEF_RES_JAC_FP32) {
-    FdOut fo{h->d_r0bank, h->d_code, h->d_rowptr, values, h->d_cbank, h->d_acbank, h->d_geob};
+    FdOut fo{h->d_r0bank, h->d_code_fd, h->d_rowptr, values, h->d_cbank, h->d_acbank, h->d_geob, nullptr, 0};
     vefad::Params<float> prm{(float)h->params.ReN, (float)h->params.BvN, (float)h->params.VeN, (float)h->params.DfN,
                              (float)h->params.WiN0, (float)(1.0 - h->params.BvN), h->params.model};
-    hipLaunchKernelGGL(vef_jac32_kernel, dim3((unsigned)h->n_el), dim3(jac32::kThreads), 0, st, h->d_conn, x, eu,
-                       h->n_el, prm, fo);
+    for (size_t c = 0; c + 1 < h->colour_off.size(); c++) {  // one launch per colour, in colour order
+      fo.elist = h->d_elist + h->colour_off[c];
+      fo.n_list = h->colour_off[c + 1] - h->colour_off[c];
+      hipLaunchKernelGGL(vef_jac32_kernel, dim3((unsigned)fo.n_list), dim3(jac32::kThreads), 0, st, h->d_conn, x, eu,
+                         h->n_el, prm, fo);
+    }
     VEF_HIP(h, hipGetLastError());
   }
   if (h->profiling) VEF_HIP(h, hipEventRecord(ev[2], st));
   const int64_t node_rows = (int64_t)h->n_nodes * 6;
   if (jac) {
-    // 4. shared entries in element order, 5. outlet-edge entries after them
-    if (h->n_shared > 0) {
-      hipLaunchKernelGGL(vef_shared_kernel, dim3((unsigned)((h->n_shared * 36 + 255) / 256)), dim3(256), 0, st,
-                         h->n_shared, h->d_sbase, h->d_slen, h->d_ustart, h->d_cbank, values);
+    // 4. entries of three or more elements in element order, 5. outlet-edge
+    //    entries after them
+    if (h->n_shared_m > 0) {
+      hipLaunchKernelGGL(vef_shared_kernel, dim3((unsigned)((h->n_shared_m * 36 + 255) / 256)), dim3(256), 0, st,
+                         h->n_shared_m, h->d_sbase, h->d_slen, h->d_ustart_m, h->d_cbank, values);
       VEF_HIP(h, hipGetLastError());
     }
     if (h->n_epairs > 0) {
