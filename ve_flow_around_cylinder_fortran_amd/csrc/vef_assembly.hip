@@ -188,9 +188,11 @@ __device__ __forceinline__ double cdiv(double a, double c) { return (a * c) / (c
 // residual: d in [2^-60, 2^40] and |q0| in [2^-901, 2^900), or a = +-0 (then
 // q0 = +-0, e = +0 and q1 = q0 keeps the sign of a/d because d > 0).  Instead
 // of branching, each division widens the exponent interval [lo, hi] of its q0
-// in a DivCheck, and a denominator outside its range or a non-finite q0 sets
-// hi to kBadExp.  The caller repeats the whole evaluation in the exact mode
-// when the interval says so.
+// in a DivCheck, and a denominator outside its range sets hi to kBadExp.  The
+// frexp exponent of an infinity or a NaN is 0, which the interval accepts, so
+// the caller also tests the quotients for finiteness (gauss_point: one test of
+// their sum).  The caller repeats the whole evaluation in the exact mode when
+// the interval or that test says so.
 //
 // Exact (EXACT = true): the IEEE division a/d.
 struct Recip {
@@ -209,11 +211,12 @@ __device__ __forceinline__ int binade(double x) {
   return __builtin_isfinite(x) ? __builtin_amdgcn_frexp_exp(x) : kBadExp;
 }
 
-// Base quantities of the untracked divisions: x = 0 or 2^-64 <= |x| < 2^64.
+// Base quantities of the untracked divisions: x = 0 or 2^-64 <= |x| < 2^64
+// (finiteness tested by the caller).
 template <bool EXACT>
 __device__ __forceinline__ void base_range(double x, DivCheck& dc) {
   if constexpr (!EXACT) {
-    const int ex = binade(x);
+    const int ex = __builtin_amdgcn_frexp_exp(x);
     dc.blo = min(dc.blo, ex);
     dc.bhi = max(dc.bhi, ex);
   }
@@ -238,7 +241,7 @@ __device__ __forceinline__ double qdiv(double a, const Recip& r, DivCheck& dc) {
   } else {
     const double q0 = a * r.y;
     const double e = fma(r.d, q0, -a);
-    const int ex = binade(q0);
+    const int ex = __builtin_amdgcn_frexp_exp(q0);  // 0 for +-0 (and for inf/NaN: tested by the caller)
     dc.lo = min(dc.lo, ex);
     dc.hi = max(dc.hi, ex);
     return fma(-e, r.y, q0);
@@ -266,7 +269,11 @@ __device__ __forceinline__ double fd_quotient(double rk, double r0, DivCheck& dc
   return qdiv<EXACT>((rk - r0) * kEpsJac, Recip{hh, 1.0 / hh}, dc);
 }
 // The same quotient decided per entry: the two-FMA division where its q0 lies
-// in the window, the IEEE division otherwise (rare: a divergent branch).
+// in the window, the IEEE division otherwise.  The IEEE division sits behind
+// a branch on the ballot of the active lanes, uniform among them, with an
+// empty volatile statement in it: without one the compiler computes the
+// division for every entry and selects (about 11 more FP64 instructions per
+// quotient).
 template <bool EXACT>
 __device__ __forceinline__ double fd_quotient_entry(double rk, double r0) {
   constexpr double hh = kEpsJac * kEpsJac;
@@ -279,7 +286,11 @@ __device__ __forceinline__ double fd_quotient_entry(double rk, double r0) {
     const double e = fma(hh, q0, -a);
     double q = fma(-e, y, q0);
     const int ex = binade(q0);
-    if (ex < -900 || ex > 900) q = a / hh;
+    const bool outside = ex < -900 || ex > 900;
+    if (__builtin_amdgcn_ballot_w64(outside) != 0) {
+      asm volatile("" ::: "memory");  // keeps the block a branch: not speculated
+      if (outside) q = a / hh;
+    }
     return q;
   }
 }
@@ -443,6 +454,8 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   const Recip rS = make_recip<EXACT>(S00 * S11 - S01 * S10, dc);
   const double SI00 = qdiv<EXACT>(S11, rS, dc), SI01 = qdiv<EXACT>(-S01, rS, dc);
   const double SI10 = qdiv<EXACT>(-S10, rS, dc), SI11 = qdiv<EXACT>(S00, rS, dc);
+  // the tracked quotients so far, summed for the finiteness test below
+  const double fin_part = EXACT ? 0.0 : (W12 + (SI00 + SI01)) + (SI10 + SI11);
 
   // term_S = f3 S^3 + f2 S - f1 S^-1 ; Con_EQ = WiN*UCD + term_S/2
   double tS00, tS01, tS10, tS11;
@@ -537,6 +550,14 @@ __device__ __forceinline__ void gauss_point(const V& v, const double* bf, const 
   base_range<EXACT>(WiN, dc);
   base_range<EXACT>(prm.omb, dc);
   if (!EXACT && !G.in_range) dc.hi = kBadExp;
+  if constexpr (!EXACT) {
+    // The range checks above take an infinity or a NaN for 0: one finiteness
+    // test of the sum of every checked quantity and tracked quotient catches
+    // them (inf - inf is NaN; the checked ranges keep finite sums far below
+    // overflow).
+    const double all = fin_part + (((S00 + S01) + (S11 + o.tlsce)) + ((WiN + prm.omb) + r4));
+    if (!__builtin_isfinite(all)) dc.hi = kBadExp;
+  }
 
   o.Ux = Ux; o.Uy = Uy;
   o.CE00 = CE00; o.CE01 = CE01; o.CE10 = CE10; o.CE11 = CE11;
