@@ -11,11 +11,16 @@
 //   factor: per height, extend-add of the children's update blocks into the
 //           parents (slot 0 children, then slot 1: a fixed order), then per
 //           bucket getrf(F11), getrs(F11, F12) and gemm(F22 -= F21 X);
-//   solve:  forward by height (gather b_s, getrs, F21 w, ordered subtract
-//           into the update unknowns), backward by height (gather x_u,
-//           w -= X x_u, x_s = w).
+//   pivots: after the factorisation, each front's row interchanges become a
+//           permutation of its pivot unknowns (mf_perm), which the solve's
+//           gather applies;
+//   solve:  forward by height (gather P b_s, the L and U solves of every
+//           front of the height in one launch, F21 w in another, ordered
+//           subtract into the update unknowns), backward by height (gather
+//           x_u, w -= X x_u in one launch, x_s = w).
 // Sums whose order matters (extend-add, the forward subtraction) run in a
-// fixed order, so results are deterministic.
+// fixed order, and every solve kernel sums in a fixed order, so results are
+// deterministic.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -102,6 +107,191 @@ __global__ void mf_put(double* __restrict__ x, const int32_t* __restrict__ dst, 
   if (k < n) x[dst[k]] = (double)w[src[k]];
 }
 
+// A front as the solve kernels see it: storage offset, padded pivot and
+// update counts, offsets of its pieces of W (pivot rows) and V/XU (update rows).
+struct MfFront {
+  int64_t base, w, v;
+  int32_t s, u;
+};
+
+constexpr int kSolveThreads = 256;
+constexpr int kDiag = 64;  // diagonal block of the blocked triangular solves
+
+// Pivot rows of the gather in the order of the last factorisation's row
+// interchanges: for front f (one workgroup each), apply getrf's swaps
+// (ipiv, 1-based within the front) to its pivot list in LDS (one thread:
+// they are sequential), then psrc[w + i] = the unknown now at row i.  Padded
+// rows are never interchanged (their columns are zero above an identity
+// diagonal), so they keep -1.
+__global__ void __launch_bounds__(kSolveThreads)
+mf_perm(const MfFront* __restrict__ fr, const int* __restrict__ ipiv, const int64_t* __restrict__ fipiv,
+        const int32_t* __restrict__ prow, int32_t* __restrict__ psrc) {
+  extern __shared__ int32_t sp[];
+  const MfFront d = fr[blockIdx.x];
+  if (d.s == 0) return;
+  const int* piv = ipiv + fipiv[blockIdx.x];
+  for (int i = threadIdx.x; i < d.s; i += blockDim.x) sp[i] = prow[d.w + i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < d.s; i++) {
+      const int k = piv[i] - 1;
+      if (k != i && k > i && k < d.s) {
+        const int32_t t = sp[i];
+        sp[i] = sp[k];
+        sp[k] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < d.s; i += blockDim.x) psrc[d.w + i] = sp[i];
+}
+
+// W[dst[k]] = v[psrc[dst[k]]] (the permuted pivot rows; padded slots stay 0).
+template <typename T>
+__global__ void mf_gather_p(T* __restrict__ w, const int64_t* __restrict__ dst, const int32_t* __restrict__ psrc,
+                            const double* __restrict__ v, int64_t n) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) {
+    const int64_t d = dst[k];
+    const int32_t src = psrc[d];
+    w[d] = src >= 0 ? (T)v[src] : T(0);
+  }
+}
+
+// acc - sum_k col[k * m] * v[k] for k < n (n <= kDiag), k in order; the loads
+// are issued 16 at a time so that many are in flight per thread.
+template <typename T>
+__device__ __forceinline__ T dot_sub(T acc, const T* __restrict__ col, int64_t m, const T* v, int n) {
+  constexpr int kB = 16;
+  for (int k0 = 0; k0 < n; k0 += kB) {
+    T a[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) a[u] = k0 + u < n ? col[(int64_t)(k0 + u) * m] : T(0);
+#pragma unroll
+    for (int u = 0; u < kB; u++)
+      if (k0 + u < n) acc -= a[u] * v[k0 + u];
+  }
+  return acc;
+}
+
+// w = U^-1 L^-1 w for one front per workgroup of NT threads (F11 = L U from
+// getrf, the permutation already applied by the gather).  Blocked by kDiag
+// columns: the diagonal block is staged in LDS and solved by the first
+// wavefront with shuffles (lane l owns row l of the block); then every thread
+// updates its rows outside the block with the block's columns (coalesced:
+// consecutive rows of one column).  w lives in LDS when it fits (lds_w), else
+// in W.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT)
+mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict__ fr, int32_t f0, int32_t lds_w) {
+  extern __shared__ unsigned char smem[];
+  const MfFront d = fr[f0 + blockIdx.x];
+  const int s = d.s;
+  if (s == 0) return;
+  const int tid = threadIdx.x;
+  const int64_t m = (int64_t)d.s + d.u;
+  const T* A = buf + d.base;
+  T* sD = reinterpret_cast<T*>(smem);  // [kDiag][kDiag + 1], column k at k * (kDiag + 1)
+  T* w = lds_w ? sD + kDiag * (kDiag + 1) : W + d.w;
+  if (lds_w) {
+    for (int i = tid; i < s; i += NT) w[i] = W[d.w + i];
+  }
+  __syncthreads();
+  auto stage = [&](int j0, int jb) {
+    for (int t = tid; t < jb * kDiag; t += NT) {
+      const int k = t / kDiag, l = t % kDiag;
+      if (l < jb) sD[k * (kDiag + 1) + l] = A[(int64_t)(j0 + k) * m + j0 + l];
+    }
+  };
+  // L (unit lower), blocks in order
+  for (int j0 = 0; j0 < s; j0 += kDiag) {
+    const int jb = min(kDiag, s - j0);
+    stage(j0, jb);
+    __syncthreads();
+    if (tid < kDiag) {
+      T val = tid < jb ? w[j0 + tid] : T(0);
+      for (int k = 0; k < jb; k++) {
+        const T wk = __shfl(val, k);
+        if (tid > k && tid < jb) val -= sD[k * (kDiag + 1) + tid] * wk;
+      }
+      if (tid < jb) w[j0 + tid] = val;
+    }
+    __syncthreads();
+    for (int i = j0 + jb + tid; i < s; i += NT) w[i] = dot_sub(w[i], A + (int64_t)j0 * m + i, m, w + j0, jb);
+    __syncthreads();
+  }
+  // U (upper), blocks in reverse
+  for (int j0 = ((s - 1) / kDiag) * kDiag; j0 >= 0; j0 -= kDiag) {
+    const int jb = min(kDiag, s - j0);
+    stage(j0, jb);
+    __syncthreads();
+    if (tid < kDiag) {
+      T val = tid < jb ? w[j0 + tid] : T(0);
+      for (int k = jb - 1; k >= 0; k--) {
+        if (tid == k) val = val / sD[k * (kDiag + 1) + k];
+        const T wk = __shfl(val, k);
+        if (tid < k) val -= sD[k * (kDiag + 1) + tid] * wk;
+      }
+      if (tid < jb) w[j0 + tid] = val;
+    }
+    __syncthreads();
+    for (int i = tid; i < j0; i += NT) w[i] = dot_sub(w[i], A + (int64_t)j0 * m + i, m, w + j0, jb);
+    __syncthreads();
+  }
+  if (lds_w) {
+    for (int i = tid; i < s; i += NT) W[d.w + i] = w[i];
+  }
+}
+
+// A chunk of up to kSolveThreads rows of one front's product.
+struct MfChunk {
+  int32_t front, row0;
+};
+
+constexpr int kVecChunk = 4096;  // vector entries staged in LDS at a time
+
+// Forward (BACK = false): V[v + i] = sum_j F21(i, j) W[w + j], i < u, j < s
+// (F21 = rows s.., columns 0..s-1).  Backward (BACK = true):
+// W[w + i] -= sum_j X(i, j) XU[v + j], i < s, j < u (X = F11^-1 F12 in the
+// F12 block).  Thread per row, columns in order (coalesced over the rows of a
+// column); the vector is staged in LDS kVecChunk entries at a time.
+template <typename T, bool BACK>
+__global__ void __launch_bounds__(kSolveThreads)
+mf_gemv(const T* __restrict__ buf, T* __restrict__ W, T* __restrict__ V, const T* __restrict__ XU,
+        const MfFront* __restrict__ fr, const MfChunk* __restrict__ ch) {
+  __shared__ T sv[kVecChunk];
+  const MfChunk c = ch[blockIdx.x];
+  const MfFront d = fr[c.front];
+  const int64_t m = (int64_t)d.s + d.u;
+  const int rows = BACK ? d.s : d.u, cols = BACK ? d.u : d.s;
+  const T* A = buf + d.base + (BACK ? (int64_t)d.s * m : (int64_t)d.s);
+  const T* x = BACK ? XU + d.v : W + d.w;
+  const int i = c.row0 + threadIdx.x;
+  T acc = T(0);
+  for (int j0 = 0; j0 < cols; j0 += kVecChunk) {
+    const int jb = min(kVecChunk, cols - j0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < jb; t += kSolveThreads) sv[t] = x[j0 + t];
+    __syncthreads();
+    if (i < rows) {
+      const T* col = A + (int64_t)j0 * m + i;
+      constexpr int kB = 16;
+      for (int k0 = 0; k0 < jb; k0 += kB) {
+        T a[kB];
+#pragma unroll
+        for (int u = 0; u < kB; u++) a[u] = k0 + u < jb ? col[(int64_t)(k0 + u) * m] : T(0);
+#pragma unroll
+        for (int u = 0; u < kB; u++)
+          if (k0 + u < jb) acc += a[u] * sv[k0 + u];
+      }
+    }
+  }
+  if (i < rows) {
+    if (BACK) W[d.w + i] -= acc;
+    else V[d.v + i] = acc;
+  }
+}
+
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // rocSOLVER / rocBLAS by precision
@@ -133,17 +323,6 @@ rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const float* alpha
   return rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k, alpha, A, lda, sA,
                                        B, ldb, sB, beta, C, ldc, sC, bc);
 }
-rocblas_status gemv_sb(rocblas_handle h, int m, int n, const double* alpha, const double* A, int lda,
-                       rocblas_stride sA, const double* x, rocblas_stride sx, const double* beta, double* y,
-                       rocblas_stride sy, int bc) {
-  return rocblas_dgemv_strided_batched(h, rocblas_operation_none, m, n, alpha, A, lda, sA, x, 1, sx, beta, y, 1, sy,
-                                       bc);
-}
-rocblas_status gemv_sb(rocblas_handle h, int m, int n, const float* alpha, const float* A, int lda, rocblas_stride sA,
-                       const float* x, rocblas_stride sx, const float* beta, float* y, rocblas_stride sy, int bc) {
-  return rocblas_sgemv_strided_batched(h, rocblas_operation_none, m, n, alpha, A, lda, sA, x, 1, sx, beta, y, 1, sy,
-                                       bc);
-}
 
 }  // namespace
 
@@ -167,8 +346,17 @@ struct vef_mf {
   std::vector<int64_t> ea_off;    // [2H + 1] tasks of (height, slot)
   std::vector<int32_t> ea_maxu;   // [2H]
   int64_t *d_fg_dst = nullptr, *d_bg_dst = nullptr, *d_bs_src = nullptr, *d_fs_ptr = nullptr, *d_fs_v = nullptr;
-  int32_t *d_fg_src = nullptr, *d_bg_src = nullptr, *d_bs_dst = nullptr, *d_fs_tgt = nullptr;
+  int32_t *d_bg_src = nullptr, *d_bs_dst = nullptr, *d_fs_tgt = nullptr;
   std::vector<int64_t> fg_off, bg_off, bs_off, fs_off, fs_ptr_off;  // per height
+  // solve kernels: front table, per-height front ranges and product chunks
+  MfFront* d_fr = nullptr;
+  int64_t* d_fipiv = nullptr;  // ipiv offset of each front
+  int32_t* d_prow = nullptr;   // [n_w] pivot unknown of each W slot (-1: padding)
+  int32_t* d_psrc = nullptr;   // [n_w] the same after the row interchanges (mf_perm)
+  MfChunk *d_fch = nullptr, *d_bch = nullptr;
+  std::vector<int32_t> h_f0, h_nf, h_smax;  // per height: first front, fronts, largest padded s
+  std::vector<int64_t> fch_off, bch_off;    // per height: forward / backward chunks
+  int32_t s_max = 0;
   rocblas_handle rb = nullptr;
   // Buckets of one height are independent: they run on kSide internal
   // streams (default priority), each with its own rocBLAS handle, forked from
@@ -206,7 +394,8 @@ static int upload(vef_mf* h, U** d, const std::vector<U>& v) {
 static void release(vef_mf* h) {
   void* ptrs[] = {h->buf,      h->ipiv,     h->info,     h->W,        h->V,        h->XU,
                   h->d_pad,    h->d_ea,     h->d_map,    h->d_fg_dst, h->d_bg_dst, h->d_bs_src,
-                  h->d_fs_ptr, h->d_fs_v,   h->d_fg_src, h->d_bg_src, h->d_bs_dst, h->d_fs_tgt};
+                  h->d_fs_ptr, h->d_fs_v,   h->d_bg_src, h->d_bs_dst, h->d_fs_tgt,
+                  h->d_fr,     h->d_fipiv,  h->d_prow,   h->d_psrc,   h->d_fch,    h->d_bch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->rb) rocblas_destroy_handle(h->rb);
@@ -270,6 +459,7 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
     if (h->b_c[b] <= 0 || h->b_s[b] < 0 || h->b_u[b] < 0 || h->b_h[b] < 0) return bad("plan: bad bucket");
     if (b > 0 && h->b_h[b] < h->b_h[b - 1]) return bad("plan: buckets not ordered by height");
     if ((int64_t)h->b_s[b] + h->b_u[b] > 46340) return bad("plan: front too large");
+    if (h->b_s[b] > 16384) return bad("plan: more than 16384 pivots in a front");
     nfr += h->b_c[b];
   }
   if (nfr != h->nf) return bad("plan: bucket counts do not add up to n_fronts");
@@ -391,7 +581,7 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
   if (upload(h, &h->d_map, map)) return bad(h->err);
   // solve lists per height
   std::vector<int64_t> fg_dst, bg_dst, bs_src, fs_ptr, fs_v;
-  std::vector<int32_t> fg_src, bg_src, bs_dst, fs_tgt;
+  std::vector<int32_t> bg_src, bs_dst, fs_tgt;
   h->fg_off.assign(h->H + 1, 0);
   h->bg_off.assign(h->H + 1, 0);
   h->bs_off.assign(h->H + 1, 0);
@@ -406,7 +596,6 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
         int32_t s = p->front_s[f], u = p->front_u[f];
         for (int32_t i = 0; i < s; i++) {
           fg_dst.push_back(fw[f] + i);
-          fg_src.push_back(rw[i]);
           bs_dst.push_back(rw[i]);
           bs_src.push_back(fw[f] + i);
         }
@@ -437,10 +626,50 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
     h->fs_off[hh + 1] = (int64_t)fs_tgt.size();
     h->fs_ptr_off[hh + 1] = (int64_t)fs_ptr.size();
   }
-  if (upload(h, &h->d_fg_dst, fg_dst) || upload(h, &h->d_fg_src, fg_src) || upload(h, &h->d_bg_dst, bg_dst) ||
+  if (upload(h, &h->d_fg_dst, fg_dst) || upload(h, &h->d_bg_dst, bg_dst) ||
       upload(h, &h->d_bg_src, bg_src) || upload(h, &h->d_bs_dst, bs_dst) || upload(h, &h->d_bs_src, bs_src) ||
       upload(h, &h->d_fs_tgt, fs_tgt) || upload(h, &h->d_fs_ptr, fs_ptr) || upload(h, &h->d_fs_v, fs_v))
     return bad(h->err);
+  // solve kernels' tables: fronts (in plan order, so a height's fronts are
+  // consecutive), their ipiv offsets and pivot unknowns, and the row chunks
+  // of the two products per height
+  {
+    std::vector<MfFront> fr(h->nf);
+    std::vector<int64_t> fipiv(h->nf);
+    std::vector<int32_t> prow(std::max<int64_t>(h->n_w, 1), -1);
+    std::vector<MfChunk> fch, bch;
+    h->h_f0.assign(h->H, 0);
+    h->h_nf.assign(h->H, 0);
+    h->h_smax.assign(h->H, 0);
+    h->fch_off.assign(h->H + 1, 0);
+    h->bch_off.assign(h->H + 1, 0);
+    for (int64_t f = 0; f < h->nf; f++) {
+      const int b = fb[f];
+      const int64_t k = f - h->b_first[b];
+      fr[f] = MfFront{h->f_base[f], fw[f], fv[f], h->b_s[b], h->b_u[b]};
+      fipiv[f] = h->b_ipiv[b] + k * h->b_s[b];
+      const int32_t* rw = p->rows + p->rows_off[f];
+      for (int32_t i = 0; i < p->front_s[f]; i++) prow[fw[f] + i] = rw[i];
+      h->s_max = std::max(h->s_max, h->b_s[b]);
+    }
+    for (int hh = 0, b = 0; hh < h->H; hh++) {
+      h->h_f0[hh] = (int32_t)h->b_first[b];
+      for (; b < h->nb && h->b_h[b] == hh; b++) {
+        h->h_smax[hh] = std::max(h->h_smax[hh], h->b_s[b]);
+        if (h->b_s[b] == 0 || h->b_u[b] == 0) continue;
+        for (int64_t f = h->b_first[b]; f < h->b_first[b + 1]; f++) {
+          for (int32_t r0 = 0; r0 < h->b_u[b]; r0 += kSolveThreads) fch.push_back(MfChunk{(int32_t)f, r0});
+          for (int32_t r0 = 0; r0 < h->b_s[b]; r0 += kSolveThreads) bch.push_back(MfChunk{(int32_t)f, r0});
+        }
+      }
+      h->h_nf[hh] = (int32_t)(h->b_first[b] - h->h_f0[hh]);
+      h->fch_off[hh + 1] = (int64_t)fch.size();
+      h->bch_off[hh + 1] = (int64_t)bch.size();
+    }
+    if (upload(h, &h->d_fr, fr) || upload(h, &h->d_fipiv, fipiv) || upload(h, &h->d_prow, prow) ||
+        upload(h, &h->d_psrc, prow) || upload(h, &h->d_fch, fch) || upload(h, &h->d_bch, bch))
+      return bad(h->err);
+  }
   if (rocblas_create_handle(&h->rb) != rocblas_status_success) return bad("rocblas_create_handle failed");
   // no atomic reductions inside rocBLAS/rocSOLVER: repeated factorisations and
   // solves of the same values give the same bits
@@ -549,6 +778,10 @@ static int factor_t(vef_mf* h, hipStream_t st) {
     if (prof && hipEventRecord(h->prof_ev[2 * hh + 2], st) != hipSuccess)
       return fail(h, "hipEventRecord failed (profile)");
   }
+  // the row interchanges as permutations of the solve's gather
+  mf_perm<<<(unsigned)h->nf, kSolveThreads, (size_t)std::max(h->s_max, 1) * sizeof(int32_t), st>>>(
+      h->d_fr, h->ipiv, h->d_fipiv, h->d_prow, h->d_psrc);
+  if (hipGetLastError() != hipSuccess) return fail(h, "kernel launch failed (pivot permutation)");
   return 0;
 }
 
@@ -599,56 +832,40 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
   T* W = (T*)h->W;
   T* V = (T*)h->V;
   T* XU = (T*)h->XU;
-  const T one = T(1), zero = T(0), mone = T(-1);
-  // forward
-  int b = 0;
+  const size_t diag_bytes = (size_t)kDiag * (kDiag + 1) * sizeof(T);
+  // forward: w = U^-1 L^-1 P b_s per front, then the update rows' b -= F21 w
   for (int hh = 0; hh < h->H; hh++) {
-    int64_t n = h->fg_off[hh + 1] - h->fg_off[hh];
-    if (n) mf_gather<T><<<blocks_for(n, 256), 256, 0, st>>>(W, h->d_fg_dst + h->fg_off[hh], h->d_fg_src + h->fg_off[hh],
-                                                           r, n);
-    if (fork_streams(h, st)) return -1;
-    for (; b < h->nb && h->b_h[b] == hh; b++) {
-      int s = h->b_s[b], u = h->b_u[b], c = h->b_c[b];
-      if (s == 0) continue;
-      rocblas_handle rb = h->rbs[h->b_stream[b]];
-      int m = s + u;
-      rocblas_stride sA = (rocblas_stride)m * m;
-      T* A = buf + h->b_base[b];
-      if (getrs_sb(rb, s, 1, A, m, sA, h->ipiv + h->b_ipiv[b], s, W + h->b_w[b], s, s, c) != rocblas_status_success)
-        return fail(h, "getrs failed (solve)");
-      if (u == 0) continue;
-      if (gemv_sb(rb, u, s, &one, A + s, m, sA, W + h->b_w[b], s, &zero, V + h->b_v[b], u, c) !=
-          rocblas_status_success)
-        return fail(h, "gemv failed (solve)");
+    const int64_t n = h->fg_off[hh + 1] - h->fg_off[hh];
+    if (n) mf_gather_p<T><<<blocks_for(n, 256), 256, 0, st>>>(W, h->d_fg_dst + h->fg_off[hh], h->d_psrc, r, n);
+    if (h->h_nf[hh] > 0 && h->h_smax[hh] > 0) {
+      const size_t wb = (size_t)h->h_smax[hh] * sizeof(T);
+      const int lds_w = diag_bytes + wb <= 65536;
+      const size_t shm = diag_bytes + (lds_w ? wb : 0);
+      // large fronts: a whole CU's worth of threads per front for the loads
+      // in flight; small ones: more fronts per CU
+      if (h->h_smax[hh] > 4 * kSolveThreads)
+        mf_trsv<T, 1024><<<(unsigned)h->h_nf[hh], 1024, shm, st>>>(buf, W, h->d_fr, h->h_f0[hh], lds_w);
+      else
+        mf_trsv<T, kSolveThreads><<<(unsigned)h->h_nf[hh], kSolveThreads, shm, st>>>(buf, W, h->d_fr, h->h_f0[hh],
+                                                                                   lds_w);
     }
-    if (join_streams(h, st)) return -1;
-    int64_t nt = h->fs_off[hh + 1] - h->fs_off[hh];
+    const int64_t nc = h->fch_off[hh + 1] - h->fch_off[hh];
+    if (nc) mf_gemv<T, false><<<(unsigned)nc, kSolveThreads, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_fch + h->fch_off[hh]);
+    const int64_t nt = h->fs_off[hh + 1] - h->fs_off[hh];
     if (nt)
       mf_subtract<T><<<blocks_for(nt, 256), 256, 0, st>>>(r, h->d_fs_tgt + h->fs_off[hh], h->d_fs_ptr + h->fs_ptr_off[hh],
                                                           h->d_fs_v, V, nt);
     if (hipGetLastError() != hipSuccess) return fail(h, "kernel launch failed (forward solve)");
   }
-  // backward
-  b = h->nb - 1;
+  // backward: x_s = w - X x_u per front
   for (int hh = h->H - 1; hh >= 0; hh--) {
-    int64_t n = h->bg_off[hh + 1] - h->bg_off[hh];
+    const int64_t n = h->bg_off[hh + 1] - h->bg_off[hh];
     if (n)
       mf_gather<T><<<blocks_for(n, 256), 256, 0, st>>>(XU, h->d_bg_dst + h->bg_off[hh], h->d_bg_src + h->bg_off[hh], x,
                                                       n);
-    if (fork_streams(h, st)) return -1;
-    for (; b >= 0 && h->b_h[b] == hh; b--) {
-      int s = h->b_s[b], u = h->b_u[b], c = h->b_c[b];
-      if (s == 0 || u == 0) continue;
-      rocblas_handle rb = h->rbs[h->b_stream[b]];
-      int m = s + u;
-      rocblas_stride sA = (rocblas_stride)m * m;
-      T* A = buf + h->b_base[b];
-      if (gemv_sb(rb, s, u, &mone, A + (int64_t)s * m, m, sA, XU + h->b_v[b], u, &one, W + h->b_w[b], s, c) !=
-          rocblas_status_success)
-        return fail(h, "gemv failed (backward solve)");
-    }
-    if (join_streams(h, st)) return -1;
-    int64_t np = h->bs_off[hh + 1] - h->bs_off[hh];
+    const int64_t nc = h->bch_off[hh + 1] - h->bch_off[hh];
+    if (nc) mf_gemv<T, true><<<(unsigned)nc, kSolveThreads, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_bch + h->bch_off[hh]);
+    const int64_t np = h->bs_off[hh + 1] - h->bs_off[hh];
     if (np)
       mf_put<T><<<blocks_for(np, 256), 256, 0, st>>>(x, h->d_bs_dst + h->bs_off[hh], h->d_bs_src + h->bs_off[hh], W,
                                                     np);
