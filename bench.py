@@ -94,7 +94,9 @@ def parse():
     ap.add_argument("--max-steps", type=int, default=80)
     ap.add_argument("--modes", default="reference,fp32",
                     help="comma-separated: reference (FP64 reference policy), fp32 (inexact Newton, FP32 "
-                         "Jacobian and factorisation), fp32_learned (and the learned predictor)")
+                         "Jacobian and factorisation), fp32_learned (and the learned predictor), fp32_krylov "
+                         "(Newton-Krylov: the current FP32 Jacobian solved by GMRES preconditioned with the "
+                         "carried FP32 factorisation)")
     ap.add_argument("--ds-max", type=float, default=50.0,
                     help="largest arclength step (the reference's dS_max, src/Parameters_mod.F90:27: an absolute "
                          "length in the state's 2-norm, which grows as the square root of the unknowns)")
@@ -283,7 +285,9 @@ def continuation_bench(args):
     iterations on its factorisation), "fp32" = the inexact Newton with the
     FP32 analytic Jacobian and FP32 factorisation carried across steps (FP64
     residual, update and convergence test), "fp32_learned" = the same with the
-    learned predictor.  Linear solves: the multifrontal LU in nested-
+    learned predictor, "fp32_krylov" = the inexact Newton whose iterations on
+    a carried factorisation solve the current FP32 Jacobian by GMRES
+    preconditioned with it.  Linear solves: the multifrontal LU in nested-
     dissection order.  Per step: Wi, Newton flags, seconds, drag Fd and peak
     wall Txx; per mode, an FP64 reference-policy Newton solve at the mode's
     final Wi from its final state gives the drag and wall-stress errors.
@@ -317,10 +321,11 @@ def continuation_bench(args):
     def run(mode):
         learned = mode.endswith("learned")
         fp32 = mode.startswith("fp32")
+        krylov = "krylov" in mode
         t_setup = time.perf_counter()
         solver = NewtonSolver(mesh, Params(**pset), factorization=args.factorization, reuse_factors=fp32,
                               linear_precision="f32" if fp32 else "f64", jacobian="ad32" if fp32 else "fd64",
-                              inlet_coupling=args.inlet_coupling, reuse_max_iters="auto")
+                              inlet_coupling=args.inlet_coupling, reuse_max_iters="auto", krylov=krylov)
         predictor = None
         if learned:
             from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor
@@ -396,8 +401,10 @@ def continuation_bench(args):
         out = {"s_per_step": float(np.mean(arc)) if arc else None, "steps": len(steps), "setup_seconds": setup,
                "Wi_final": steps[-1]["Wi"], "total_seconds": float(sum(s["seconds"] for s in steps)),
                "newton_iterations": int(sum(s["newton_iterations"] for s in steps)),
+               "s_per_step_median": float(np.median(arc)) if arc else None,
                "factorizations": solver.lu.refactorizations, "refined_solves": solver.lu.refinements,
-               "host_fallbacks": solver.lu.host_fallbacks,
+               "gmres_iterations": solver.lu.gmres_iterations, "krylov_solves": solver.krylov_solves,
+               "krylov_failures": solver.krylov_failures, "host_fallbacks": solver.lu.host_fallbacks,
                "Fd_final": steps[-1]["Fd"], "fp64_newton_iterations_at_final_Wi": len(hr),
                "drag_rel_diff_vs_fp64_newton": abs(steps[-1]["Fd"] - Fr) / abs(Fr),
                "peak_wall_Txx_rel_diff_vs_fp64_newton": abs(steps[-1]["peak_wall_Txx"] - Pr) / abs(Pr),

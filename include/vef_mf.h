@@ -92,12 +92,31 @@ int vef_mf_profile(vef_mf* h, int32_t enable);
 int32_t vef_mf_heights(const vef_mf* h);
 int vef_mf_profile_read(vef_mf* h, float* ms);
 
+/* Row interchanges within each front's pivot block (on != 0, the default:
+ * getrf's partial pivoting, as MUMPS pivots) or none (on = 0: getrf without
+ * interchanges, for a factorisation whose solves are refined by the caller;
+ * the nested-dissection order puts each node's velocities before its
+ * pressure, so the pressure pivots are the velocity block's Schur
+ * complement).  Takes effect at the next vef_mf_factor.  No counterpart in
+ * the reference's interface (MUMPS's pivoting is its ICNTL setting). */
+int vef_mf_set_pivoting(vef_mf* h, int32_t on);
+
 /* Fronts whose last factorisation met a zero pivot (synchronises). */
 int vef_mf_zero_pivots(vef_mf* h, int32_t* count);
 
 /* x = A^-1 b with the last factors (device arrays of n doubles in the plan's
  * unknown order; b is used as work space and overwritten). */
 int vef_mf_solve(vef_mf* h, double* b, double* x, void* hip_stream);
+
+/* y = A x for a CSR matrix of n rows (FP64; device arrays, rowptr[n + 1]
+ * and col of 64-bit integers), summed in a fixed order (deterministic).
+ * Every row holding more than 16384 entries (the arclength row) must be
+ * listed in long_rows[n_long] (a device array): those rows are summed by a
+ * workgroup each, the others by a wavefront.  For the residual checks and
+ * the refinement of the factorisation's callers; no counterpart in the
+ * reference (PETSc's MatMult). */
+int vef_mf_csr_mv(int64_t n, const int64_t* rowptr, const int64_t* col, const double* values, const double* x,
+                  double* y, int32_t n_long, const int64_t* long_rows, void* hip_stream);
 
 const char* vef_mf_last_error(const vef_mf* h);
 void vef_mf_destroy(vef_mf* h);

@@ -30,9 +30,10 @@ def newton_system(mesh, x, EU=0.1):
     return asm, b, v
 
 
+@pytest.mark.parametrize("pivoting", [False, True])
 @pytest.mark.parametrize("dims,precision", [((8, 4, 6, 8), "f64"), ((8, 4, 6, 8), "f32"),
                                             ((16, 12, 20, 32), "f64"), ((16, 12, 20, 32), "f32")])
-def test_multifrontal_matches_scipy(dims, precision):
+def test_multifrontal_matches_scipy(dims, precision, pivoting):
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
 
@@ -41,12 +42,13 @@ def test_multifrontal_matches_scipy(dims, precision):
     mesh = cylinder_channel(*dims)
     lu = None
     for k, x in enumerate([initial_state(mesh, 0.1), smooth_state(mesh, EU=0.1)]):
-        print(f"[gpu mf] {dims} {precision} matrix {k}", flush=True)
+        print(f"[gpu mf] {dims} {precision} pivoting={pivoting} matrix {k}", flush=True)
         asm, b, v = newton_system(mesh, x)
         rowptr, col = asm.pattern()
         if lu is None:
             # FP32: refined by FP64 GMRES down to a 1e-10 relative residual
-            lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, precision=precision, refine_tol=1e-10)
+            lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, precision=precision, refine_tol=1e-10,
+                                pivoting=pivoting)
         lu.factor(v)
         s = lu.solve(b)
         torch.cuda.synchronize()
@@ -58,7 +60,42 @@ def test_multifrontal_matches_scipy(dims, precision):
         print(f"[gpu mf] {dims} {precision} matrix {k}: max error {err:.2e} (unrefined {plain:.2e})", flush=True)
         assert err < (1e-9 if precision == "f64" else 1e-7), err  # FP32: the 1e-10 residual times the conditioning
         assert plain < (1e-9 if precision == "f64" else 1e-3), plain
-    assert lu.refactorizations == 2 and lu.host_fallbacks == 0
+    assert lu.refactorizations == 2 and lu.host_fallbacks == 0 and lu.pivot_fallbacks == 0
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_multifrontal_failed_solve_tries_pivoting_and_leaves_no_trace(precision):
+    """A matrix with a NaN entry: the solve on the factorisation without row
+    interchanges is not finite, the matrix is factorised again with them (one
+    fallback), which fails too, and the error reaches the caller.  The next
+    factorisation (the clean matrix, without interchanges again) solves as
+    scipy does: nothing of the failed solves (padded slots of the per-front
+    vectors included) is carried into it."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from ve_flow_around_cylinder_fortran_amd.linsolve import RfLUError
+    from ve_flow_around_cylinder_fortran_amd.multifrontal import MultifrontalLU
+
+    mesh = cylinder_channel(8, 4, 6, 8)
+    asm, b, v = newton_system(mesh, smooth_state(mesh, EU=0.1))
+    rowptr, col = asm.pattern()
+    lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, precision=precision, refine_tol=1e-10)
+    vn = v.clone()
+    vn[int(rowptr[100]) + 3] = float("nan")
+    print(f"[gpu mf] failed solve {precision}", flush=True)
+    lu.factor(vn)
+    with pytest.raises(RfLUError):
+        lu.solve(b)
+    assert lu.pivot_fallbacks == 1
+    lu.factor(v)
+    s = lu.solve(b)
+    torch.cuda.synchronize()
+    assert lu.pivot_fallbacks == 1 and torch.isfinite(s).all()
+    A = sp.csr_matrix((v.cpu().numpy(), col, rowptr), shape=(asm.N, asm.N))
+    ref = sla.spsolve(A.tocsc(), b.cpu().numpy())
+    err = np.abs(s.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < (1e-9 if precision == "f64" else 1e-7), err
 
 
 def test_multifrontal_repeats_bitwise():

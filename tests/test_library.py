@@ -69,7 +69,7 @@ def test_mf_library_exports_every_declared_symbol():
 
     lib = multifrontal._mf_lib()
     names = declared_functions("vef_mf.h")
-    assert len(names) == 12
+    assert len(names) == 14
     for n in names:
         assert hasattr(lib, n), n
 
@@ -83,6 +83,7 @@ def test_mf_profile_calls_reject_null_handles():
     assert lib.vef_mf_heights(None) == -1
     ms = np.zeros(4, dtype=np.float32)
     assert lib.vef_mf_profile_read(None, ms.ctypes.data) != 0
+    assert lib.vef_mf_set_pivoting(None, 0) != 0
 
 
 def test_mf_library_rejects_bad_plans_without_a_gpu():

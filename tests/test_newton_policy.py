@@ -43,3 +43,19 @@ def test_auto_limit_from_measured_times():
 def test_bad_limit_rejected(limit):
     with pytest.raises(NewtonError, match="reuse_max_iters"):  # checked before anything touches the GPU
         NewtonSolver.__init__(object.__new__(NewtonSolver), None, None, reuse_max_iters=limit)
+
+
+def test_krylov_options_checked():
+    with pytest.raises(NewtonError, match="reuse_factors"):
+        NewtonSolver.__init__(object.__new__(NewtonSolver), None, None, krylov=True)
+    for tol, its in ((0.0, 10), (1.0, 10), (1e-3, 0)):
+        with pytest.raises(NewtonError, match="krylov_tol"):
+            NewtonSolver.__init__(object.__new__(NewtonSolver), None, None, reuse_factors=True, krylov=True,
+                                  krylov_tol=tol, krylov_max_its=its)
+
+
+def test_krylov_iterations_time_as_carried_factor_iterations():
+    s = _policy("auto")
+    s._time_iteration(NewtonRecord(1, "NRP", 1.0, 1.0, 0.20))
+    s._time_iteration(NewtonRecord(2, "KRY", 1.0, 1.0, 0.05))
+    assert s._reuse_limit() == pytest.approx((0.20 - 0.05) / 0.05)

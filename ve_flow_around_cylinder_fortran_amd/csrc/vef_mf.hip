@@ -108,10 +108,11 @@ __global__ void mf_put(double* __restrict__ x, const int32_t* __restrict__ dst, 
 }
 
 // A front as the solve kernels see it: storage offset, padded pivot and
-// update counts, offsets of its pieces of W (pivot rows) and V/XU (update rows).
+// update counts, offsets of its pieces of W (pivot rows) and V/XU (update
+// rows), and its real pivot count (the padded rows' slots of W are kept 0).
 struct MfFront {
   int64_t base, w, v;
-  int32_t s, u;
+  int32_t s, u, sr;
 };
 
 constexpr int kSolveThreads = 256;
@@ -193,9 +194,10 @@ mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict_
   const T* A = buf + d.base;
   T* sD = reinterpret_cast<T*>(smem);  // [kDiag][kDiag + 1], column k at k * (kDiag + 1)
   T* w = lds_w ? sD + kDiag * (kDiag + 1) : W + d.w;
-  if (lds_w) {
-    for (int i = tid; i < s; i += NT) w[i] = W[d.w + i];
-  }
+  // padded slots start from 0 whatever an earlier solve left there (a
+  // failed factorisation's NaN would otherwise reach every later solve
+  // through the zero padding columns)
+  for (int i = tid; i < s; i += NT) w[i] = i < d.sr ? W[d.w + i] : T(0);
   __syncthreads();
   auto stage = [&](int j0, int jb) {
     for (int t = tid; t < jb * kDiag; t += NT) {
@@ -294,6 +296,44 @@ mf_gemv(const T* __restrict__ buf, T* __restrict__ W, T* __restrict__ V, const T
 
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// y = A x for a CSR matrix (FP64), one wavefront per row: lanes stride the
+// row, then a fixed shuffle tree sums them (deterministic).  Rows longer than
+// kLongRow (the dense arclength row) go to mf_csr_mv_long instead.
+constexpr int kLongRow = 1 << 14;
+__global__ void __launch_bounds__(256)
+mf_csr_mv(int64_t n, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ col,
+          const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (row >= n) return;  // uniform per wavefront
+  const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
+  if (p1 - p0 > kLongRow) return;
+  double acc = 0.0;
+  for (int64_t p = p0 + lane; p < p1; p += 64) acc += val[p] * x[col[p]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) y[row] = acc;
+}
+// The long rows (listed by the caller): one 1024-thread workgroup each,
+// per-thread sums in a fixed stride, then a fixed tree in LDS.
+__global__ void __launch_bounds__(1024)
+mf_csr_mv_long(const int64_t* __restrict__ long_rows, const int64_t* __restrict__ rowptr,
+               const int64_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ x,
+               double* __restrict__ y) {
+  __shared__ double red[1024];
+  const int64_t row = long_rows[blockIdx.x];
+  const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
+  double acc = 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += 1024) acc += val[p] * x[col[p]];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[row] = red[0];
+}
+
 // rocSOLVER / rocBLAS by precision
 rocblas_status getrf_sb(rocblas_handle h, int n, double* A, int lda, rocblas_stride sA, int* ipiv, rocblas_stride sP,
                         int* info, int bc) {
@@ -302,6 +342,12 @@ rocblas_status getrf_sb(rocblas_handle h, int n, double* A, int lda, rocblas_str
 rocblas_status getrf_sb(rocblas_handle h, int n, float* A, int lda, rocblas_stride sA, int* ipiv, rocblas_stride sP,
                         int* info, int bc) {
   return rocsolver_sgetrf_strided_batched(h, n, n, A, lda, sA, ipiv, sP, info, bc);
+}
+rocblas_status getrf_npvt_sb(rocblas_handle h, int n, double* A, int lda, rocblas_stride sA, int* info, int bc) {
+  return rocsolver_dgetrf_npvt_strided_batched(h, n, n, A, lda, sA, info, bc);
+}
+rocblas_status getrf_npvt_sb(rocblas_handle h, int n, float* A, int lda, rocblas_stride sA, int* info, int bc) {
+  return rocsolver_sgetrf_npvt_strided_batched(h, n, n, A, lda, sA, info, bc);
 }
 rocblas_status getrs_sb(rocblas_handle h, int n, int nrhs, double* A, int lda, rocblas_stride sA, const int* ipiv,
                         rocblas_stride sP, double* B, int ldb, rocblas_stride sB, int bc) {
@@ -328,6 +374,8 @@ rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const float* alpha
 
 struct vef_mf {
   int bits = 64;
+  int pivoting = 1;           // 0: getrf without row interchanges (vef_mf_set_pivoting)
+  int* ipiv_id = nullptr;     // the identity interchanges, for getrs and the gather without pivoting
   int64_t n = 0;
   int32_t nb = 0, nf = 0, H = 0;
   std::vector<int32_t> b_h, b_c, b_s, b_u;
@@ -395,7 +443,8 @@ static void release(vef_mf* h) {
   void* ptrs[] = {h->buf,      h->ipiv,     h->info,     h->W,        h->V,        h->XU,
                   h->d_pad,    h->d_ea,     h->d_map,    h->d_fg_dst, h->d_bg_dst, h->d_bs_src,
                   h->d_fs_ptr, h->d_fs_v,   h->d_bg_src, h->d_bs_dst, h->d_fs_tgt,
-                  h->d_fr,     h->d_fipiv,  h->d_prow,   h->d_psrc,   h->d_fch,    h->d_bch};
+                  h->d_fr,     h->d_fipiv,  h->d_prow,   h->d_psrc,   h->d_fch,    h->d_bch,
+                  h->ipiv_id};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->rb) rocblas_destroy_handle(h->rb);
@@ -646,7 +695,7 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
     for (int64_t f = 0; f < h->nf; f++) {
       const int b = fb[f];
       const int64_t k = f - h->b_first[b];
-      fr[f] = MfFront{h->f_base[f], fw[f], fv[f], h->b_s[b], h->b_u[b]};
+      fr[f] = MfFront{h->f_base[f], fw[f], fv[f], h->b_s[b], h->b_u[b], p->front_s[f]};
       fipiv[f] = h->b_ipiv[b] + k * h->b_s[b];
       const int32_t* rw = p->rows + p->rows_off[f];
       for (int32_t i = 0; i < p->front_s[f]; i++) prow[fw[f] + i] = rw[i];
@@ -669,6 +718,11 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
     if (upload(h, &h->d_fr, fr) || upload(h, &h->d_fipiv, fipiv) || upload(h, &h->d_prow, prow) ||
         upload(h, &h->d_psrc, prow) || upload(h, &h->d_fch, fch) || upload(h, &h->d_bch, bch))
       return bad(h->err);
+    std::vector<int32_t> id(std::max<int64_t>(h->n_ipiv, 1));
+    for (int b = 0; b < h->nb; b++)
+      for (int64_t k = 0; k < h->b_c[b]; k++)
+        for (int32_t i = 0; i < h->b_s[b]; i++) id[h->b_ipiv[b] + k * h->b_s[b] + i] = i + 1;
+    if (upload(h, &h->ipiv_id, id)) return bad(h->err);
   }
   if (rocblas_create_handle(&h->rb) != rocblas_status_success) return bad("rocblas_create_handle failed");
   // no atomic reductions inside rocBLAS/rocSOLVER: repeated factorisations and
@@ -742,6 +796,14 @@ static int factor_t(vef_mf* h, hipStream_t st) {
   T* buf = (T*)h->buf;
   const T one = T(1), mone = T(-1);
   int b = 0;
+  // the solves' per-front vectors start from zero with every factorisation
+  // (their padded slots must be 0; nothing of an earlier, possibly failed,
+  // solve is carried over)
+  if (h->n_w) mf_fill<T><<<blocks_for(h->n_w, 256), 256, 0, st>>>((T*)h->W, h->n_w, T(0));
+  if (h->n_v) {
+    mf_fill<T><<<blocks_for(h->n_v, 256), 256, 0, st>>>((T*)h->V, h->n_v, T(0));
+    mf_fill<T><<<blocks_for(h->n_v, 256), 256, 0, st>>>((T*)h->XU, h->n_v, T(0));
+  }
   const bool prof = h->profiling && (int)h->prof_ev.size() == 2 * h->H + 1;
   if (prof && hipEventRecord(h->prof_ev[0], st) != hipSuccess) return fail(h, "hipEventRecord failed (profile)");
   for (int hh = 0; hh < h->H; hh++) {
@@ -764,10 +826,15 @@ static int factor_t(vef_mf* h, hipStream_t st) {
       int m = s + u;
       rocblas_stride sA = (rocblas_stride)m * m;
       T* A = buf + h->b_base[b];
-      if (getrf_sb(rb, s, A, m, sA, h->ipiv + h->b_ipiv[b], s, h->info + h->b_first[b], c) != rocblas_status_success)
-        return fail(h, "getrf failed");
+      int* piv = (h->pivoting ? h->ipiv : h->ipiv_id) + h->b_ipiv[b];
+      if (h->pivoting) {
+        if (getrf_sb(rb, s, A, m, sA, piv, s, h->info + h->b_first[b], c) != rocblas_status_success)
+          return fail(h, "getrf failed");
+      } else if (getrf_npvt_sb(rb, s, A, m, sA, h->info + h->b_first[b], c) != rocblas_status_success) {
+        return fail(h, "getrf_npvt failed");
+      }
       if (u == 0) continue;
-      if (getrs_sb(rb, s, u, A, m, sA, h->ipiv + h->b_ipiv[b], s, A + (int64_t)s * m, m, sA, c) !=
+      if (getrs_sb(rb, s, u, A, m, sA, piv, s, A + (int64_t)s * m, m, sA, c) !=
           rocblas_status_success)
         return fail(h, "getrs failed");
       if (gemm_sb(rb, u, u, s, &mone, A + s, m, sA, A + (int64_t)s * m, m, sA, &one, A + (int64_t)s * m + s, m, sA, c) !=
@@ -780,7 +847,7 @@ static int factor_t(vef_mf* h, hipStream_t st) {
   }
   // the row interchanges as permutations of the solve's gather
   mf_perm<<<(unsigned)h->nf, kSolveThreads, (size_t)std::max(h->s_max, 1) * sizeof(int32_t), st>>>(
-      h->d_fr, h->ipiv, h->d_fipiv, h->d_prow, h->d_psrc);
+      h->d_fr, h->pivoting ? h->ipiv : h->ipiv_id, h->d_fipiv, h->d_prow, h->d_psrc);
   if (hipGetLastError() != hipSuccess) return fail(h, "kernel launch failed (pivot permutation)");
   return 0;
 }
@@ -797,6 +864,12 @@ int vef_mf_profile(vef_mf* h, int32_t enable) {
 }
 
 int32_t vef_mf_heights(const vef_mf* h) { return h ? h->H : -1; }
+
+int vef_mf_set_pivoting(vef_mf* h, int32_t on) {
+  if (!h) return -1;
+  h->pivoting = on != 0;
+  return 0;
+}
 
 int vef_mf_profile_read(vef_mf* h, float* ms) {
   if (!h || !ms) return -1;
@@ -872,6 +945,17 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
     if (hipGetLastError() != hipSuccess) return fail(h, "kernel launch failed (backward solve)");
   }
   return 0;
+}
+
+int vef_mf_csr_mv(int64_t n, const int64_t* rowptr, const int64_t* col, const double* values, const double* x,
+                  double* y, int32_t n_long, const int64_t* long_rows, void* stream) {
+  if (n < 0 || n_long < 0 || (n > 0 && (!rowptr || !col || !values || !x || !y)) || (n_long > 0 && !long_rows))
+    return -1;
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  mf_csr_mv<<<blocks_for(n, 4), 256, 0, st>>>(n, rowptr, col, values, x, y);
+  if (n_long > 0) mf_csr_mv_long<<<(unsigned)n_long, 1024, 0, st>>>(long_rows, rowptr, col, values, x, y);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int vef_mf_solve(vef_mf* h, double* b, double* x, void* stream) {

@@ -706,6 +706,8 @@ class _BorderedLU:
         self.a21 = torch.zeros(n1, dtype=torch.float64, device=device)
         self.ptrA = torch.from_numpy(self.rowptr_h).to(device)
         self.indA = torch.from_numpy(cols).to(device)
+        # rows the sparse product sums with a workgroup each (the arclength row)
+        self.long_rows = torch.from_numpy(np.nonzero(np.diff(self.rowptr_h) > 16384)[0].astype(np.int64)).to(device)
         self.analyses = 1
         self.refactorizations = 0
         self.host_fallbacks = 0
@@ -740,23 +742,60 @@ class _BorderedLU:
         self.s = values[self.src_d] - torch.dot(self.a21, self.z)
         self.refactorizations += 1
 
+    def refresh_row(self, values):
+        """Takes the border row (the last row: the arclength equation) from
+        `values` and recomputes the border's Schur complement s = a22 - a21.z
+        with this factorisation's A11 and z = A11^-1 a12, which the row does
+        not touch: a new continuation step's arclength row costs one dot
+        product, not a factorisation.  `values` becomes the matrix the checks
+        and the refinement use."""
+        torch = self.torch
+        if self._A is None:
+            raise RfLUError("factor() first")
+        self._A = values
+        self.a21.zero_()
+        self.a21[self.dst_r] = values[self.src_r]
+        self.s = values[self.src_d] - torch.dot(self.a21, self.z)
+
+    def _mv(self, values, x):
+        """A x for the matrix with `values` (this pattern): on the GPU the
+        library's fixed-order CSR product (vef_mf_csr_mv; torch's sparse
+        product has returned NaN once after a product with NaN inputs), on the
+        host torch's."""
+        torch = self.torch
+        if not x.is_cuda:
+            return torch.mv(torch.sparse_csr_tensor(self.ptrA, self.indA, values, size=(self.n, self.n)), x)
+        from .multifrontal import _mf_lib
+
+        y = torch.empty_like(x)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        rc = _mf_lib().vef_mf_csr_mv(self.n, p(self.ptrA), p(self.indA), p(values), p(x), p(y), len(self.long_rows),
+                                     p(self.long_rows) if len(self.long_rows) else None, self._stream())
+        if rc != 0:
+            raise RfLUError("vef_mf_csr_mv failed")
+        return y
+
     def _residual(self, values, x, b):
         torch = self.torch
-        A = torch.sparse_csr_tensor(self.ptrA, self.indA, values, size=(self.n, self.n))
-        r = torch.mv(A, x) - b
+        r = self._mv(values, x) - b
         return float(torch.linalg.vector_norm(r)), float(torch.linalg.vector_norm(b))
 
-    def _gmres(self, b, x, bn, tol, m=30, restarts=10):
+    def _gmres(self, b, x, bn, tol, m=30, restarts=10, values=None):
         """Restarted GMRES(m) in FP64 for A x = b from x, right-preconditioned
-        by the factorisation's solve; stops at ||b - A x|| <= tol ||b||."""
+        by the factorisation's solve; stops at ||b - A x|| <= tol ||b||.  A is
+        the factorised matrix, or the one with `values` (same pattern).  The
+        basis is orthogonalised by classical Gram-Schmidt applied twice (two
+        products with the stored basis per iteration, one transfer to the
+        host)."""
         torch = self.torch
-        A = torch.sparse_csr_tensor(self.ptrA, self.indA, self._A, size=(self.n, self.n))
+        vals = self._A if values is None else values
+        V = torch.empty((m + 1, self.n), dtype=torch.float64, device=self.device)
         for _ in range(restarts):
-            r = b - torch.mv(A, x)
+            r = b - self._mv(vals, x)
             beta = float(torch.linalg.vector_norm(r))
             if beta <= tol * bn or not np.isfinite(beta):
                 return x, beta
-            V = [r / beta]
+            V[0] = r / beta
             Z = []
             H = np.zeros((m + 1, m))
             g = np.zeros(m + 1)
@@ -765,11 +804,14 @@ class _BorderedLU:
             for j in range(m):
                 z = self._precondition(V[j])
                 Z.append(z)
-                w = torch.mv(A, z)
-                for i in range(j + 1):  # modified Gram-Schmidt
-                    H[i, j] = float(torch.dot(w, V[i]))
-                    w = w - H[i, j] * V[i]
-                H[j + 1, j] = float(torch.linalg.vector_norm(w))
+                w = self._mv(vals, z)
+                Vj = V[:j + 1]
+                h1 = torch.mv(Vj, w)
+                w = w - torch.mv(Vj.t(), h1)
+                h2 = torch.mv(Vj, w)
+                w = w - torch.mv(Vj.t(), h2)
+                hw = torch.cat([h1 + h2, torch.linalg.vector_norm(w).reshape(1)]).cpu().numpy()
+                H[:j + 2, j] = hw
                 self.gmres_iterations += 1
                 if not np.all(np.isfinite(H[:j + 2, :j + 1])):
                     return x, beta
@@ -777,16 +819,29 @@ class _BorderedLU:
                 est = float(np.linalg.norm(g[:j + 2] - H[:j + 2, :j + 1] @ y))
                 if est <= tol * bn or H[j + 1, j] == 0.0:
                     break
-                V.append(w / H[j + 1, j])
+                V[j + 1] = w / H[j + 1, j]
             xn = x
             for i, yi in enumerate(y):
                 xn = xn + float(yi) * Z[i]
-            rn = float(torch.linalg.vector_norm(b - torch.mv(A, xn)))
+            rn = float(torch.linalg.vector_norm(b - self._mv(vals, xn)))
             if rn >= beta:  # no progress (the residual is at rounding level): keep the better iterate
                 return x, beta
             x = xn
-        r = b - torch.mv(A, x)
+        r = b - self._mv(vals, x)
         return x, float(torch.linalg.vector_norm(r))
+
+    def solve_krylov(self, values, b, tol: float, max_its: int):
+        """x with A x = b for the matrix with `values` (this factorisation's
+        pattern; typically a later Jacobian than the one factorised) by FP64
+        GMRES(max_its) from 0, right-preconditioned by this factorisation.
+        Returns (x, converged: ||b - A x|| <= tol ||b||, iterations)."""
+        torch = self.torch
+        if self._A is None:
+            raise RfLUError("factor() first")
+        bn = float(torch.linalg.vector_norm(b))
+        it0 = self.gmres_iterations
+        x, rn = self._gmres(b, torch.zeros_like(b), bn, tol, m=max_its, restarts=1, values=values)
+        return x, bool(np.isfinite(rn) and rn <= tol * max(bn, 1e-300)), self.gmres_iterations - it0
 
     def _precondition(self, b):
         """The node-block solve with the border (FP64 vectors)."""
@@ -801,10 +856,29 @@ class _BorderedLU:
         x[n1] = x2
         return x
 
+    # A subclass whose factorisation can run without row interchanges sets
+    # this and implements _refactor_pivoted(): a solve that fails its check
+    # (non-finite, or above the tolerance after refinement) on such a
+    # factorisation refactorises the same matrix with interchanges and
+    # solves again.
+    _unpivoted = False
+
+    def _refactor_pivoted(self):
+        raise NotImplementedError
+
     def solve(self, b, check: bool = False):
         """x with A x = b for the last factorised matrix (b is not modified).
         check=True forms this solve's FP64 residual whatever the policy
         (refined or handed to the host factorisation like a first solve)."""
+        try:
+            return self._solve_checked(b, check)
+        except RfLUError:
+            if not self._unpivoted:
+                raise
+        self._refactor_pivoted()
+        return self._solve_checked(b, check)
+
+    def _solve_checked(self, b, check: bool):
         torch = self.torch
         if self._A is None:
             raise RfLUError("factor() first")

