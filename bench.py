@@ -404,7 +404,8 @@ def continuation_bench(args):
                "s_per_step_median": float(np.median(arc)) if arc else None,
                "factorizations": solver.lu.refactorizations, "refined_solves": solver.lu.refinements,
                "gmres_iterations": solver.lu.gmres_iterations, "krylov_solves": solver.krylov_solves,
-               "krylov_failures": solver.krylov_failures, "host_fallbacks": solver.lu.host_fallbacks,
+               "krylov_failures": solver.krylov_failures, "escalations": solver.escalations,
+               "host_fallbacks": solver.lu.host_fallbacks,
                "Fd_final": steps[-1]["Fd"], "fp64_newton_iterations_at_final_Wi": len(hr),
                "drag_rel_diff_vs_fp64_newton": abs(steps[-1]["Fd"] - Fr) / abs(Fr),
                "peak_wall_Txx_rel_diff_vs_fp64_newton": abs(steps[-1]["peak_wall_Txx"] - Pr) / abs(Pr),

@@ -171,7 +171,16 @@ class NewtonSolver:
     factorisation lasts as long as it preconditions well.  A GMRES run that
     misses the tolerance is not applied; the next iteration refactorises at
     the same iterate.  Everything else (residual, update, convergence test,
-    the growth and contraction rules) is as with reuse_factors."""
+    the growth and contraction rules) is as with reuse_factors.
+
+    Escalation (reuse_factors with an FP32 factorisation or the FP32
+    Jacobian): after an emergency restart, the rest of that solve uses the
+    reference's ingredients, the FP64 forward-difference Jacobian and an FP64
+    factorisation (a second factorisation object, built on first use), as
+    every iteration does in the reference policy; the next solve returns to
+    the configured ones.  Near a singular point of the branch (linear PTT at
+    Wi = 0.324 on the 40,000-element mesh) the FP32 Jacobian and factorisation
+    do not carry the Newton iteration through, the FP64 ones do."""
 
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
                  reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = None,
@@ -214,18 +223,29 @@ class NewtonSolver:
                               "(banded LU in reverse Cuthill-McKee node order) or 'mf' (multifrontal LU in "
                               "nested-dissection order)")
         check_tol = 1e-6 if (reuse_factors and linear_precision == "f64") else None
-        if factorization == "mf":
-            self.lu = MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, device=device, precision=linear_precision,
-                                     check_tol=check_tol)
-        else:
-            self.lu = classes[factorization](rowptr, col, device=device, precision=linear_precision,
-                                             check_tol=check_tol)
+
+        def make_lu(precision, tol):
+            if factorization == "mf":
+                return MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, device=device, precision=precision,
+                                      check_tol=tol, plan=getattr(getattr(self, "lu", None), "plan", None))
+            return classes[factorization](rowptr, col, device=device, precision=precision, check_tol=tol)
+
+        self.lu = make_lu(linear_precision, check_tol)
+        # the inexact Newton's escalation after an emergency restart (see the
+        # class docstring): an FP64 factorisation built on first use
+        self._make_lu64 = lambda: make_lu("f64", 1e-6)
+        self._lu_primary = self.lu
+        self._lu64 = None
+        self._escalated = False
+        self.escalations = 0
+        self.linear_precision = linear_precision
         self.N = self.asm.N
         self.b = torch.empty(self.N, dtype=torch.float64, device=device)
         self.values = torch.empty(self.asm.nnz, dtype=torch.float64, device=device)
         if jacobian not in ("fd64", "ad32"):
             raise NewtonError("jacobian must be 'fd64' (the reference's forward differences) or 'ad32' (analytic, FP32)")
         self.jac_flag = "NRP" if jacobian == "fd64" else "NRP32"
+        self._jac_flag_primary = self.jac_flag
         self.reuse_factors = reuse_factors
         self.reuse_max_iters = reuse_max_iters
         self._t_nrp = self._t_mnr = None  # running mean wall times of factorising / modified-Newton iterations
@@ -265,6 +285,10 @@ class NewtonSolver:
         torch = self.torch
         x0 = x0.clone() if torch.is_tensor(x0) else torch.from_numpy(np.asarray(x0, dtype=np.float64)).to(self.device)
         x = x0.clone()
+        if self._escalated:  # back to the configured Jacobian and factorisation after an escalated solve
+            self.lu, self.jac_flag = self._lu_primary, self._jac_flag_primary
+            self._escalated = False
+            self.factored = False
         flag = "MNR" if (self.reuse_factors and self.factored) else "NRP"
         res_norm = cor_new = cor_old = 1.0
         applied = 0  # corrections applied since the solve's start (or last restart)
@@ -392,6 +416,16 @@ class NewtonSolver:
                                       f"{len(self.emergencies)} restarts (last: {why}, ||s|| = {cor_new:.3e})")
                 cap = 50 * max_iter
                 dS = 0.5 * dS
+                if self.reuse_factors and (self.linear_precision == "f32" or self.jac_flag != "NRP") \
+                        and not self._escalated:
+                    # inexact Newton: the rest of this solve with the reference's
+                    # FP64 forward-difference Jacobian and an FP64 factorisation
+                    if self._lu64 is None:
+                        self._lu64 = self._make_lu64()
+                    self.lu, self.jac_flag = self._lu64, "NRP"
+                    self._escalated = True
+                    self.escalations += 1
+                    self.factored = False
                 x = x0.clone()
                 it = 0
                 res_norm = cor_new = cor_old = 1.0
