@@ -226,3 +226,43 @@ def test_non_finite_iterates_raise_inflow_error():
     inf.z = good
     (U, *_), _ = inf.solve(0.5)  # recovers from a finite state
     assert np.all(np.isfinite(U))
+
+
+def test_bordered_band_lu_solves_as_dense_and_falls_back():
+    """The inflow Jacobian's factorisation: a banded block (14 sub- and
+    superdiagonals) bordered by one dense column and row solves as a dense
+    solver does; a singular banded block, or entries outside the band, give
+    no band factorisation (the caller then takes SuperLU of the whole
+    matrix), and the inflow problem still solves."""
+    from ve_flow_around_cylinder_fortran_amd.inflow import _KL, _BorderedBandLU
+
+    rng = np.random.default_rng(3)
+    n = 61
+    A = np.zeros((n, n))
+    for i in range(n - 1):
+        lo, hi = max(0, i - _KL), min(n - 1, i + _KL + 1)
+        A[i, lo:hi] = rng.standard_normal(hi - lo)
+        A[i, i] += 8.0
+    A[:-1, -1] = rng.standard_normal(n - 1)
+    A[-1, :-1] = rng.standard_normal(n - 1)
+    rows, cols = np.nonzero(A)
+    lu = _BorderedBandLU.factor(rows, cols, A[rows, cols], n, _KL)
+    assert lu is not None
+    b = rng.standard_normal(n)
+    x = lu.solve(b)
+    assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-12 * np.abs(x).max()
+    S = A.copy()
+    S[:-1, :-1][:, 5] = 0.0  # a zero column of the banded block
+    rows, cols = np.nonzero(S)
+    assert _BorderedBandLU.factor(rows, cols, S[rows, cols], n, _KL) is None
+    W = A.copy()
+    W[0, _KL + 1] = 1.0  # outside the band
+    rows, cols = np.nonzero(W)
+    assert _BorderedBandLU.factor(rows, cols, W[rows, cols], n, _KL) is None
+    # the inflow solve takes the band factorisation and converges
+    y = np.linspace(-H, H, 41)
+    inf = FullyDevelopedInflow(y, H, Params(BvN=0.59, VeN=0.25, model="L-PTT", DfN=1e-3))
+    inf.solve(0.5)
+    assert isinstance(inf._lu, _BorderedBandLU)
+    r = inf.residual(inf.z, 0.5)
+    assert np.abs(r).max() < 1e-11

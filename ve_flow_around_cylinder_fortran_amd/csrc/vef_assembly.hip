@@ -1925,6 +1925,7 @@ struct vef_handle {
   int64_t* d_bc_rows = nullptr;     // Dirichlet rows, distinct
   double* d_bc_values = nullptr;    // their target values
   int64_t n_bc = 0;
+  int64_t bc_cap = 0;               // capacity of the two buffers
   // linear stability assembly (allocated on first use)
   std::vector<int64_t> h_sbase_s;   // shared pairs: first slot in the stability pattern
   std::vector<int32_t> h_slen_s;    //   its row length there
@@ -2568,27 +2569,41 @@ int vef_arclength_noalloc(vef_handle* h, int32_t mode, const double* x, const do
 int vef_set_dirichlet(vef_handle* h, int64_t n_rows, const int64_t* rows, const double* values) {
   if (!h) return set_err(nullptr, VEF_ERR_ARG, "handle is NULL");
   if (n_rows < 0 || (n_rows > 0 && (!rows || !values))) return set_err(h, VEF_ERR_ARG, "bad Dirichlet list");
-  // The reference inserts boundary by boundary, so a row listed twice keeps its last value.
-  std::vector<int64_t> last(h->N, -1);
+  // The reference inserts boundary by boundary, so a row listed twice keeps
+  // its last value: the list's positions sorted by (row, position), the last
+  // of each row kept.
+  std::vector<int64_t> order((size_t)n_rows);
   for (int64_t i = 0; i < n_rows; i++) {
     if (rows[i] < 0 || rows[i] >= h->N) return set_err(h, VEF_ERR_ARG, "Dirichlet row out of range");
-    last[rows[i]] = i;
+    order[(size_t)i] = i;
   }
+  std::sort(order.begin(), order.end(),
+            [rows](int64_t a, int64_t b) { return rows[a] != rows[b] ? rows[a] < rows[b] : a < b; });
   std::vector<int64_t> r;
   std::vector<double> v;
-  for (int64_t row = 0; row < h->N; row++)
-    if (last[row] >= 0) {
-      r.push_back(row);
-      v.push_back(values[last[row]]);
+  r.reserve(order.size());
+  v.reserve(order.size());
+  for (size_t k = 0; k < order.size(); k++)
+    if (k + 1 == order.size() || rows[order[k + 1]] != rows[order[k]]) {
+      r.push_back(rows[order[k]]);
+      v.push_back(values[order[k]]);
     }
-  if (h->d_bc_rows) (void)hipFree(h->d_bc_rows);
-  if (h->d_bc_values) (void)hipFree(h->d_bc_values);
-  h->d_bc_rows = nullptr;
-  h->d_bc_values = nullptr;
+  // the buffers are kept while the list fits them; work still queued may read
+  // them, so the device is idle before they are written
+  VEF_HIP(h, hipDeviceSynchronize());
+  if ((int64_t)r.size() > h->bc_cap) {
+    if (h->d_bc_rows) (void)hipFree(h->d_bc_rows);
+    if (h->d_bc_values) (void)hipFree(h->d_bc_values);
+    h->d_bc_rows = nullptr;
+    h->d_bc_values = nullptr;
+    h->bc_cap = 0;
+    h->n_bc = 0;
+    VEF_HIP(h, hipMalloc((void**)&h->d_bc_rows, sizeof(int64_t) * r.size()));
+    VEF_HIP(h, hipMalloc((void**)&h->d_bc_values, sizeof(double) * r.size()));
+    h->bc_cap = (int64_t)r.size();
+  }
   h->n_bc = (int64_t)r.size();
   if (h->n_bc == 0) return VEF_OK;
-  VEF_HIP(h, hipMalloc((void**)&h->d_bc_rows, sizeof(int64_t) * r.size()));
-  VEF_HIP(h, hipMalloc((void**)&h->d_bc_values, sizeof(double) * v.size()));
   VEF_HIP(h, hipMemcpy(h->d_bc_rows, r.data(), sizeof(int64_t) * r.size(), hipMemcpyHostToDevice));
   VEF_HIP(h, hipMemcpy(h->d_bc_values, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
   return VEF_OK;

@@ -54,6 +54,32 @@ def oldroyd_b_inflow(y: np.ndarray, H: float, WiN: float):
     return U, np.zeros_like(y), (c11 + s) / t, c12 / t, (c22 + s) / t
 
 
+def _fixed_dirichlet_rows(mesh: Mesh):
+    """The inlet nodes (closed-form order) and the rows of the conditions that
+    do not depend on Wi (outlet P, walls and cylinder U), in the reference's
+    insertion order; kept on the mesh, keyed by its coordinate array and a
+    sample of its values (a mesh is not changed in place after its first
+    solve)."""
+    c = mesh.coords
+    key = (id(c), c.shape, c.__array_interface__["data"][0], float(np.sum(c[::97])))
+    cached = mesh.__dict__.get("_vef_fixed_dirichlet")
+    if cached is not None and cached[0] == key:
+        return cached[1], cached[2]
+    g = mesh.geometry
+    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+    tol = MESH_TOLERANCE
+    inlet = np.nonzero(np.abs(x + g["L_up"]) < tol)[0]
+    rows = [6 * np.nonzero(np.abs(x - g["L_down"]) < tol)[0] + 2]
+    for on in (np.abs(y + g["H"]) < tol, np.abs(y - g["H"]) < tol,
+               np.abs(x * x + y * y - g["R"] * g["R"]) < tol):
+        nodes = np.nonzero(on)[0]
+        for q in (0, 1):
+            rows.append(6 * nodes + q)
+    fixed = np.concatenate(rows).astype(np.int64)
+    mesh.__dict__["_vef_fixed_dirichlet"] = (key, inlet, fixed)
+    return inlet, fixed
+
+
 def cylinder_dirichlet(mesh: Mesh, WiN: float, inflow=None):
     """Rows and values of the Cylinder_2D Dirichlet conditions, in the order
     the reference inserts them (a row listed twice keeps the later value).
@@ -61,28 +87,15 @@ def cylinder_dirichlet(mesh: Mesh, WiN: float, inflow=None):
     with `inflow` (an inflow.FullyDevelopedInflow of this mesh) the solution
     of the reference's Inflow_1D problem at WiN, for any model."""
     g = mesh.geometry
-    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
-    tol = MESH_TOLERANCE
-    rows: List[np.ndarray] = []
-    vals: List[np.ndarray] = []
+    inlet, fixed = _fixed_dirichlet_rows(mesh)
     if inflow is None:
-        inlet = np.nonzero(np.abs(x + g["L_up"]) < tol)[0]
-        prof = oldroyd_b_inflow(y[inlet], g["H"], WiN)
+        prof = oldroyd_b_inflow(mesh.coords[inlet, 1], g["H"], WiN)
     else:
         inlet = inflow.nodes
         prof, _ = inflow.solve(WiN)
-    for q, v in zip((0, 1, 3, 4, 5), prof):  # Ux, Uy, Sxx, Sxy, Syy (P is skipped)
-        rows.append(6 * inlet + q)
-        vals.append(v)
-    outlet = np.nonzero(np.abs(x - g["L_down"]) < tol)[0]
-    rows.append(6 * outlet + 2)
-    vals.append(np.zeros(len(outlet)))
-    for on in (np.abs(y + g["H"]) < tol, np.abs(y - g["H"]) < tol,
-               np.abs(x * x + y * y - g["R"] * g["R"]) < tol):
-        nodes = np.nonzero(on)[0]
-        for q in (0, 1):
-            rows.append(6 * nodes + q)
-            vals.append(np.zeros(len(nodes)))
+    rows = [6 * inlet + q for q in (0, 1, 3, 4, 5)]  # Ux, Uy, Sxx, Sxy, Syy (P is skipped)
+    rows.append(fixed)
+    vals = list(prof) + [np.zeros(len(fixed))]
     return np.concatenate(rows).astype(np.int64), np.concatenate(vals)
 
 
