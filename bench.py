@@ -104,6 +104,9 @@ def parse():
                     help="linear solver of --continuation: multifrontal (nested dissection) or banded LU")
     ap.add_argument("--mode-seconds", type=float, default=600.0,
                     help="wall-time budget of each --continuation mode (the mode stops at the next step)")
+    ap.add_argument("--refine-tol", type=float, default=None,
+                    help="continuation: relative residual to which the FP32 modes' solves are refined by GMRES "
+                         "(default 1e-3)")
     ap.add_argument("--check-every", type=int, default=10,
                     help="--continuation: every this many steps, an FP64 Newton solve at the step's Wi checks the "
                          "drag of the non-reference modes")
@@ -325,7 +328,8 @@ def continuation_bench(args):
         t_setup = time.perf_counter()
         solver = NewtonSolver(mesh, Params(**pset), factorization=args.factorization, reuse_factors=fp32,
                               linear_precision="f32" if fp32 else "f64", jacobian="ad32" if fp32 else "fd64",
-                              inlet_coupling=args.inlet_coupling, reuse_max_iters="auto", krylov=krylov)
+                              inlet_coupling=args.inlet_coupling, reuse_max_iters="auto", krylov=krylov,
+                              refine_tol=args.refine_tol if fp32 else None)
         predictor = None
         if learned:
             from ve_flow_around_cylinder_fortran_amd.learned import LearnedPredictor

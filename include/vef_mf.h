@@ -111,12 +111,15 @@ int vef_mf_solve(vef_mf* h, double* b, double* x, void* hip_stream);
 /* y = A x for a CSR matrix of n rows (FP64; device arrays, rowptr[n + 1]
  * and col of 64-bit integers), summed in a fixed order (deterministic).
  * Every row holding more than 16384 entries (the arclength row) must be
- * listed in long_rows[n_long] (a device array): those rows are summed by a
- * workgroup each, the others by a wavefront.  For the residual checks and
- * the refinement of the factorisation's callers; no counterpart in the
- * reference (PETSc's MatMult). */
+ * listed in long_rows[n_long] (a device array): with work (a device array
+ * of n_long * 256 doubles) each such row is split into 256 fixed chunks
+ * summed by a workgroup each, then the chunks by one workgroup; with work
+ * NULL one workgroup sums the whole row.  The other rows are summed by a
+ * wavefront each.  For the residual checks and the refinement of the
+ * factorisation's callers; no counterpart in the reference (PETSc's
+ * MatMult). */
 int vef_mf_csr_mv(int64_t n, const int64_t* rowptr, const int64_t* col, const double* values, const double* x,
-                  double* y, int32_t n_long, const int64_t* long_rows, void* hip_stream);
+                  double* y, int32_t n_long, const int64_t* long_rows, double* work, void* hip_stream);
 
 const char* vef_mf_last_error(const vef_mf* h);
 void vef_mf_destroy(vef_mf* h);

@@ -98,6 +98,50 @@ def test_multifrontal_failed_solve_tries_pivoting_and_leaves_no_trace(precision)
     assert err < (1e-9 if precision == "f64" else 1e-7), err
 
 
+def test_csr_product_splits_long_rows_deterministically():
+    """vef_mf_csr_mv: rows above 16384 entries (16385, 30001 and 40000 here,
+    among short and empty ones) split into 256 fixed chunks with a workspace,
+    summed whole without one: both within rounding of scipy's product, the
+    same bits on a repeat."""
+    import ctypes
+
+    import scipy.sparse as sp
+    import torch
+
+    from ve_flow_around_cylinder_fortran_amd.multifrontal import _mf_lib
+
+    rng = np.random.default_rng(11)
+    n = 40000
+    lens = rng.integers(0, 40, n)
+    lens[[5, 777, n - 1]] = [16385, 40000, 30001]
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = rng.integers(0, n, int(rowptr[-1])).astype(np.int64)
+    val = rng.standard_normal(int(rowptr[-1]))
+    x = rng.standard_normal(n)
+    ref = sp.csr_matrix((val, col, rowptr), shape=(n, n)) @ x
+    long_rows = np.nonzero(lens > 16384)[0].astype(np.int64)
+    dev = lambda a: torch.from_numpy(a).to("cuda:0")  # noqa: E731
+    d_ptr, d_col, d_val, d_x, d_long = dev(rowptr), dev(col), dev(val), dev(x), dev(long_rows)
+    work = torch.empty(256 * len(long_rows), dtype=torch.float64, device="cuda:0")
+    p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    lib = _mf_lib()
+
+    def mv(w):
+        y = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda:0")
+        assert lib.vef_mf_csr_mv(n, p(d_ptr), p(d_col), p(d_val), p(d_x), p(y), len(long_rows), p(d_long), p(w),
+                                 None) == 0
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    print("[gpu mf] csr product with long rows", flush=True)
+    y1, y2, y3 = mv(work), mv(work), mv(None)
+    assert np.array_equal(y1, y2)
+    scale = np.abs(val).max() * np.abs(x).max() * np.sqrt(lens.max())
+    assert np.abs(y1 - ref).max() <= 1e-13 * scale
+    assert np.abs(y3 - ref).max() <= 1e-13 * scale
+    assert np.all(y1[lens == 0] == 0.0)
+
+
 def test_multifrontal_repeats_bitwise():
     from ve_flow_around_cylinder_fortran_amd.multifrontal import MultifrontalLU
 

@@ -54,6 +54,14 @@ def test_krylov_options_checked():
                                   krylov_tol=tol, krylov_max_its=its)
 
 
+def test_refine_tol_checked():
+    """refine_tol sets the FP32 factorisation's GMRES target: only for FP32,
+    within (0, 1), checked before any GPU work."""
+    for prec, tol in (("f64", 1e-2), ("f32", 0.0), ("f32", 1.0)):
+        with pytest.raises(NewtonError, match="refine_tol"):
+            NewtonSolver.__init__(object.__new__(NewtonSolver), None, None, linear_precision=prec, refine_tol=tol)
+
+
 def test_krylov_iterations_time_as_carried_factor_iterations():
     s = _policy("auto")
     s._time_iteration(NewtonRecord(1, "NRP", 1.0, 1.0, 0.20))

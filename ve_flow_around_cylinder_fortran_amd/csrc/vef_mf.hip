@@ -174,6 +174,44 @@ __device__ __forceinline__ T dot_sub(T acc, const T* __restrict__ col, int64_t m
   }
   return acc;
 }
+// Two rows at once (their loads in flight together, kB per row per batch);
+// each row's terms are subtracted in the order dot_sub takes, so the results
+// are the same bits.
+template <typename T, int kB>
+__device__ __forceinline__ void dot_sub2(T& acc0, T& acc1, const T* __restrict__ c0, const T* __restrict__ c1,
+                                         int64_t m, const T* v, int n) {
+  for (int k0 = 0; k0 < n; k0 += kB) {
+    T a[kB], b[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      a[u] = k0 + u < n ? c0[(int64_t)(k0 + u) * m] : T(0);
+      b[u] = k0 + u < n ? c1[(int64_t)(k0 + u) * m] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kB; u++)
+      if (k0 + u < n) {
+        const T vk = v[k0 + u];
+        acc0 -= a[u] * vk;
+        acc1 -= b[u] * vk;
+      }
+  }
+}
+// w[i] -= A(i, j0 .. j0 + jb) w[j0 ..] for the rows i in [r0, r1) of thread
+// tid's stride, two rows per pass.
+template <typename T, int NT>
+__device__ __forceinline__ void update_rows(T* w, const T* __restrict__ Aj, int64_t m, int j0, int jb, int r0,
+                                            int r1, int tid) {
+  int i = r0 + tid;
+  for (; i + NT < r1; i += 2 * NT) {
+    T a0 = w[i], a1 = w[i + NT];
+    // 32 loads per row in FP32 where the registers allow it (no spill at
+    // 256 threads; 1024 threads leave 128 registers)
+    dot_sub2<T, (sizeof(T) == 4 && NT <= 256) ? 32 : 16>(a0, a1, Aj + i, Aj + i + NT, m, w + j0, jb);
+    w[i] = a0;
+    w[i + NT] = a1;
+  }
+  if (i < r1) w[i] = dot_sub(w[i], Aj + i, m, w + j0, jb);
+}
 
 // w = U^-1 L^-1 w for one front per workgroup of NT threads (F11 = L U from
 // getrf, the permutation already applied by the gather).  Blocked by kDiag
@@ -219,7 +257,7 @@ mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict_
       if (tid < jb) w[j0 + tid] = val;
     }
     __syncthreads();
-    for (int i = j0 + jb + tid; i < s; i += NT) w[i] = dot_sub(w[i], A + (int64_t)j0 * m + i, m, w + j0, jb);
+    update_rows<T, NT>(w, A + (int64_t)j0 * m, m, j0, jb, j0 + jb, s, tid);
     __syncthreads();
   }
   // U (upper), blocks in reverse
@@ -237,7 +275,7 @@ mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict_
       if (tid < jb) w[j0 + tid] = val;
     }
     __syncthreads();
-    for (int i = tid; i < j0; i += NT) w[i] = dot_sub(w[i], A + (int64_t)j0 * m + i, m, w + j0, jb);
+    update_rows<T, NT>(w, A + (int64_t)j0 * m, m, j0, jb, 0, j0, tid);
     __syncthreads();
   }
   if (lds_w) {
@@ -245,52 +283,70 @@ mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict_
   }
 }
 
-// A chunk of up to kSolveThreads rows of one front's product.
+// A chunk of up to kGemvRows rows of one front's product.
 struct MfChunk {
   int32_t front, row0;
 };
+constexpr int kGemvRows = 64;
 
 constexpr int kVecChunk = 4096;  // vector entries staged in LDS at a time
 
 // Forward (BACK = false): V[v + i] = sum_j F21(i, j) W[w + j], i < u, j < s
 // (F21 = rows s.., columns 0..s-1).  Backward (BACK = true):
 // W[w + i] -= sum_j X(i, j) XU[v + j], i < s, j < u (X = F11^-1 F12 in the
-// F12 block).  Thread per row, columns in order (coalesced over the rows of a
-// column); the vector is staged in LDS kVecChunk entries at a time.
-template <typename T, bool BACK>
-__global__ void __launch_bounds__(kSolveThreads)
+// F12 block).  A workgroup takes kGemvRows rows (a wavefront's 64 lanes,
+// consecutive rows of a column: coalesced) times G column groups: group g
+// sums its contiguous share of each staged piece of the vector, in column
+// order, and the G partial sums of a row are added by a fixed pairwise tree
+// (deterministic).  G = 16 for long rows (the top fronts, whose few chunks
+// would otherwise run long serial loops), 4 for short ones.
+template <typename T, bool BACK, int G>
+__global__ void __launch_bounds__(kGemvRows * G)
 mf_gemv(const T* __restrict__ buf, T* __restrict__ W, T* __restrict__ V, const T* __restrict__ XU,
         const MfFront* __restrict__ fr, const MfChunk* __restrict__ ch) {
   __shared__ T sv[kVecChunk];
+  __shared__ T part[G][kGemvRows];
   const MfChunk c = ch[blockIdx.x];
   const MfFront d = fr[c.front];
   const int64_t m = (int64_t)d.s + d.u;
   const int rows = BACK ? d.s : d.u, cols = BACK ? d.u : d.s;
   const T* A = buf + d.base + (BACK ? (int64_t)d.s * m : (int64_t)d.s);
   const T* x = BACK ? XU + d.v : W + d.w;
-  const int i = c.row0 + threadIdx.x;
+  const int r = threadIdx.x % kGemvRows, g = threadIdx.x / kGemvRows;
+  const int i = c.row0 + r;
   T acc = T(0);
   for (int j0 = 0; j0 < cols; j0 += kVecChunk) {
     const int jb = min(kVecChunk, cols - j0);
     __syncthreads();
-    for (int t = threadIdx.x; t < jb; t += kSolveThreads) sv[t] = x[j0 + t];
+    for (int t = threadIdx.x; t < jb; t += kGemvRows * G) sv[t] = x[j0 + t];
     __syncthreads();
+    const int k_lo = jb * g / G, k_hi = jb * (g + 1) / G;
     if (i < rows) {
       const T* col = A + (int64_t)j0 * m + i;
       constexpr int kB = 16;
-      for (int k0 = 0; k0 < jb; k0 += kB) {
+      for (int k0 = k_lo; k0 < k_hi; k0 += kB) {
         T a[kB];
 #pragma unroll
-        for (int u = 0; u < kB; u++) a[u] = k0 + u < jb ? col[(int64_t)(k0 + u) * m] : T(0);
+        for (int u = 0; u < kB; u++) a[u] = k0 + u < k_hi ? col[(int64_t)(k0 + u) * m] : T(0);
 #pragma unroll
         for (int u = 0; u < kB; u++)
-          if (k0 + u < jb) acc += a[u] * sv[k0 + u];
+          if (k0 + u < k_hi) acc += a[u] * sv[k0 + u];
       }
     }
   }
-  if (i < rows) {
-    if (BACK) W[d.w + i] -= acc;
-    else V[d.v + i] = acc;
+  part[g][r] = acc;
+  __syncthreads();
+  if (g == 0 && i < rows) {
+    T v[G];
+#pragma unroll
+    for (int q = 0; q < G; q++) v[q] = part[q][r];
+#pragma unroll
+    for (int width = G; width > 1; width /= 2) {
+#pragma unroll
+      for (int q = 0; q < width / 2; q++) v[q] = v[2 * q] + v[2 * q + 1];
+    }
+    if (BACK) W[d.w + i] -= v[0];
+    else V[d.v + i] = v[0];
   }
 }
 
@@ -332,6 +388,37 @@ mf_csr_mv_long(const int64_t* __restrict__ long_rows, const int64_t* __restrict_
     __syncthreads();
   }
   if (threadIdx.x == 0) y[row] = red[0];
+}
+
+// The long rows split over kLongChunks workgroups each (fixed chunk bounds):
+// each workgroup sums its chunk by a fixed tree into work[row * kLongChunks +
+// chunk], then one workgroup per row sums the chunks by a fixed tree.
+constexpr int kLongChunks = 256;
+__device__ inline double block_sum_256(double acc) {
+  __shared__ double red[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+__global__ void __launch_bounds__(256)
+mf_csr_mv_long_part(const int64_t* __restrict__ long_rows, const int64_t* __restrict__ rowptr,
+                    const int64_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ x,
+                    double* __restrict__ work) {
+  const int r = blockIdx.x / kLongChunks, c = blockIdx.x % kLongChunks;
+  const int64_t row = long_rows[r];
+  const int64_t p0 = rowptr[row], len = rowptr[row + 1] - p0;
+  const int64_t q0 = p0 + len * c / kLongChunks, q1 = p0 + len * (c + 1) / kLongChunks;
+  double acc = 0.0;
+  for (int64_t p = q0 + threadIdx.x; p < q1; p += 256) acc += val[p] * x[col[p]];
+  acc = block_sum_256(acc);
+  if (threadIdx.x == 0) work[blockIdx.x] = acc;
+}
+__global__ void __launch_bounds__(256)
+mf_csr_mv_long_sum(const int64_t* __restrict__ long_rows, const double* __restrict__ work, double* __restrict__ y) {
+  const double acc = block_sum_256(work[(int64_t)blockIdx.x * kLongChunks + threadIdx.x]);
+  if (threadIdx.x == 0) y[long_rows[blockIdx.x]] = acc;
 }
 
 // rocSOLVER / rocBLAS by precision
@@ -403,6 +490,7 @@ struct vef_mf {
   int32_t* d_psrc = nullptr;   // [n_w] the same after the row interchanges (mf_perm)
   MfChunk *d_fch = nullptr, *d_bch = nullptr;
   std::vector<int32_t> h_f0, h_nf, h_smax;  // per height: first front, fronts, largest padded s
+  std::vector<int32_t> h_umax;              // per height: largest padded u
   std::vector<int64_t> fch_off, bch_off;    // per height: forward / backward chunks
   int32_t s_max = 0;
   rocblas_handle rb = nullptr;
@@ -690,6 +778,7 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
     h->h_f0.assign(h->H, 0);
     h->h_nf.assign(h->H, 0);
     h->h_smax.assign(h->H, 0);
+    h->h_umax.assign(h->H, 0);
     h->fch_off.assign(h->H + 1, 0);
     h->bch_off.assign(h->H + 1, 0);
     for (int64_t f = 0; f < h->nf; f++) {
@@ -706,9 +795,10 @@ int vef_mf_create(const vef_mf_plan* p, int32_t bits, vef_mf** out) {
       for (; b < h->nb && h->b_h[b] == hh; b++) {
         h->h_smax[hh] = std::max(h->h_smax[hh], h->b_s[b]);
         if (h->b_s[b] == 0 || h->b_u[b] == 0) continue;
+        h->h_umax[hh] = std::max(h->h_umax[hh], h->b_u[b]);
         for (int64_t f = h->b_first[b]; f < h->b_first[b + 1]; f++) {
-          for (int32_t r0 = 0; r0 < h->b_u[b]; r0 += kSolveThreads) fch.push_back(MfChunk{(int32_t)f, r0});
-          for (int32_t r0 = 0; r0 < h->b_s[b]; r0 += kSolveThreads) bch.push_back(MfChunk{(int32_t)f, r0});
+          for (int32_t r0 = 0; r0 < h->b_u[b]; r0 += kGemvRows) fch.push_back(MfChunk{(int32_t)f, r0});
+          for (int32_t r0 = 0; r0 < h->b_s[b]; r0 += kGemvRows) bch.push_back(MfChunk{(int32_t)f, r0});
         }
       }
       h->h_nf[hh] = (int32_t)(h->b_first[b] - h->h_f0[hh]);
@@ -923,7 +1013,10 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
                                                                                    lds_w);
     }
     const int64_t nc = h->fch_off[hh + 1] - h->fch_off[hh];
-    if (nc) mf_gemv<T, false><<<(unsigned)nc, kSolveThreads, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_fch + h->fch_off[hh]);
+    if (nc && h->h_smax[hh] > 256)
+      mf_gemv<T, false, 16><<<(unsigned)nc, kGemvRows * 16, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_fch + h->fch_off[hh]);
+    else if (nc)
+      mf_gemv<T, false, 4><<<(unsigned)nc, kGemvRows * 4, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_fch + h->fch_off[hh]);
     const int64_t nt = h->fs_off[hh + 1] - h->fs_off[hh];
     if (nt)
       mf_subtract<T><<<blocks_for(nt, 256), 256, 0, st>>>(r, h->d_fs_tgt + h->fs_off[hh], h->d_fs_ptr + h->fs_ptr_off[hh],
@@ -937,7 +1030,10 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
       mf_gather<T><<<blocks_for(n, 256), 256, 0, st>>>(XU, h->d_bg_dst + h->bg_off[hh], h->d_bg_src + h->bg_off[hh], x,
                                                       n);
     const int64_t nc = h->bch_off[hh + 1] - h->bch_off[hh];
-    if (nc) mf_gemv<T, true><<<(unsigned)nc, kSolveThreads, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_bch + h->bch_off[hh]);
+    if (nc && h->h_umax[hh] > 256)
+      mf_gemv<T, true, 16><<<(unsigned)nc, kGemvRows * 16, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_bch + h->bch_off[hh]);
+    else if (nc)
+      mf_gemv<T, true, 4><<<(unsigned)nc, kGemvRows * 4, 0, st>>>(buf, W, V, XU, h->d_fr, h->d_bch + h->bch_off[hh]);
     const int64_t np = h->bs_off[hh + 1] - h->bs_off[hh];
     if (np)
       mf_put<T><<<blocks_for(np, 256), 256, 0, st>>>(x, h->d_bs_dst + h->bs_off[hh], h->d_bs_src + h->bs_off[hh], W,
@@ -948,13 +1044,19 @@ static int solve_t(vef_mf* h, double* r, double* x, hipStream_t st) {
 }
 
 int vef_mf_csr_mv(int64_t n, const int64_t* rowptr, const int64_t* col, const double* values, const double* x,
-                  double* y, int32_t n_long, const int64_t* long_rows, void* stream) {
+                  double* y, int32_t n_long, const int64_t* long_rows, double* work, void* stream) {
   if (n < 0 || n_long < 0 || (n > 0 && (!rowptr || !col || !values || !x || !y)) || (n_long > 0 && !long_rows))
     return -1;
+  if ((int64_t)n_long * kLongChunks > (int64_t)INT32_MAX) return -1;
   if (n == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   mf_csr_mv<<<blocks_for(n, 4), 256, 0, st>>>(n, rowptr, col, values, x, y);
-  if (n_long > 0) mf_csr_mv_long<<<(unsigned)n_long, 1024, 0, st>>>(long_rows, rowptr, col, values, x, y);
+  if (n_long > 0 && work) {
+    mf_csr_mv_long_part<<<(unsigned)(n_long * kLongChunks), 256, 0, st>>>(long_rows, rowptr, col, values, x, work);
+    mf_csr_mv_long_sum<<<(unsigned)n_long, 256, 0, st>>>(long_rows, work, y);
+  } else if (n_long > 0) {
+    mf_csr_mv_long<<<(unsigned)n_long, 1024, 0, st>>>(long_rows, rowptr, col, values, x, y);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -708,6 +708,8 @@ class _BorderedLU:
         self.indA = torch.from_numpy(cols).to(device)
         # rows the sparse product sums with a workgroup each (the arclength row)
         self.long_rows = torch.from_numpy(np.nonzero(np.diff(self.rowptr_h) > 16384)[0].astype(np.int64)).to(device)
+        # the CSR product's partial sums of the long rows (256 chunks each)
+        self._mv_work = torch.empty(max(1, 256 * len(self.long_rows)), dtype=torch.float64, device=device)
         self.analyses = 1
         self.refactorizations = 0
         self.host_fallbacks = 0
@@ -770,7 +772,8 @@ class _BorderedLU:
         y = torch.empty_like(x)
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
         rc = _mf_lib().vef_mf_csr_mv(self.n, p(self.ptrA), p(self.indA), p(values), p(x), p(y), len(self.long_rows),
-                                     p(self.long_rows) if len(self.long_rows) else None, self._stream())
+                                     p(self.long_rows) if len(self.long_rows) else None, p(self._mv_work),
+                                     self._stream())
         if rc != 0:
             raise RfLUError("vef_mf_csr_mv failed")
         return y

@@ -454,7 +454,7 @@ def _mf_lib():
         L.vef_mf_set_pivoting.argtypes = [vp, i32]
         L.vef_mf_zero_pivots.argtypes = [vp, ctypes.POINTER(i32)]
         L.vef_mf_solve.argtypes = [vp, vp, vp, vp]
-        L.vef_mf_csr_mv.argtypes = [i64, vp, vp, vp, vp, vp, i32, vp, vp]
+        L.vef_mf_csr_mv.argtypes = [i64, vp, vp, vp, vp, vp, i32, vp, vp, vp]
         L.vef_mf_last_error.argtypes = [vp]
         L.vef_mf_last_error.restype = ctypes.c_char_p
         L.vef_mf_destroy.argtypes = [vp]

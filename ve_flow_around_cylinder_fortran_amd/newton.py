@@ -198,7 +198,7 @@ class NewtonSolver:
     def __init__(self, mesh: Mesh, params: Params, device: str = "cuda:0", linear_precision: str = "f64",
                  reuse_factors: bool = False, reuse_max_iters: float = 8.0, factorization: str = None,
                  jacobian: str = "fd64", inflow: str = None, inlet_coupling: bool = False, krylov: bool = False,
-                 krylov_tol: float = 1.0e-3, krylov_max_its: int = 10):
+                 krylov_tol: float = 1.0e-3, krylov_max_its: int = 10, refine_tol: float = None):
         import torch
 
         from .linsolve import BandLU, BlockLU, HybridLU, RfLU
@@ -209,6 +209,8 @@ class NewtonSolver:
             raise NewtonError("krylov=True needs reuse_factors=True (it preconditions with a carried factorisation)")
         if not (0.0 < krylov_tol < 1.0) or int(krylov_max_its) < 1:
             raise NewtonError("krylov_tol must lie in (0, 1) and krylov_max_its be at least 1")
+        if refine_tol is not None and not (linear_precision == "f32" and 0.0 < refine_tol < 1.0):
+            raise NewtonError("refine_tol applies to an FP32 factorisation and must lie in (0, 1)")
         oldroyd_b = params.VeN == 0.0 and params.model == "L-PTT"
         if inflow is None:
             inflow = "closed_form" if oldroyd_b else "fe1d"
@@ -237,13 +239,21 @@ class NewtonSolver:
                               "nested-dissection order)")
         check_tol = 1e-6 if (reuse_factors and linear_precision == "f64") else None
 
-        def make_lu(precision, tol):
+        def make_lu(precision, tol, rtol=None):
+            kw = {} if rtol is None else {"refine_tol": rtol}
             if factorization == "mf":
                 return MultifrontalLU(rowptr, col, mesh.coords, mesh.conn, device=device, precision=precision,
-                                      check_tol=tol, plan=getattr(getattr(self, "lu", None), "plan", None))
-            return classes[factorization](rowptr, col, device=device, precision=precision, check_tol=tol)
+                                      check_tol=tol, plan=getattr(getattr(self, "lu", None), "plan", None), **kw)
+            return classes[factorization](rowptr, col, device=device, precision=precision, check_tol=tol, **kw)
 
-        self.lu = make_lu(linear_precision, check_tol)
+        if refine_tol is not None:
+            if factorization not in ("mf", "band"):
+                raise NewtonError("refine_tol applies to the 'mf' and 'band' factorisations (GMRES refinement)")
+            # an FP32 solve above refine_tol (relative residual) is refined to
+            # it by GMRES, and one below it is accepted
+            check_tol = max(1e-3, refine_tol)
+
+        self.lu = make_lu(linear_precision, check_tol, refine_tol)
         # the inexact Newton's escalation after an emergency restart (see the
         # class docstring): an FP64 factorisation built on first use
         self._make_lu64 = lambda: make_lu("f64", 1e-6)
