@@ -161,6 +161,17 @@ __global__ void mf_gather_p(T* __restrict__ w, const int64_t* __restrict__ dst, 
 
 // acc - sum_k col[k * m] * v[k] for k < n (n <= kDiag), k in order; the loads
 // are issued 16 at a time so that many are in flight per thread.
+// The value lane k (uniform across the wavefront) holds, read from its
+// register (v_readlane) rather than through LDS.
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T dot_sub(T acc, const T* __restrict__ col, int64_t m, const T* v, int n) {
   constexpr int kB = 16;
@@ -250,9 +261,21 @@ mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict_
     __syncthreads();
     if (tid < kDiag) {
       T val = tid < jb ? w[j0 + tid] : T(0);
-      for (int k = 0; k < jb; k++) {
-        const T wk = __shfl(val, k);
-        if (tid > k && tid < jb) val -= sD[k * (kDiag + 1) + tid] * wk;
+      // the block's columns read 8 ahead of the chain (rows past jb clamped
+      // into the staged block, their values unused)
+      for (int k0 = 0; k0 < jb; k0 += 8) {
+        T a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = sD[min(k0 + u, kDiag - 1) * (kDiag + 1) + tid];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int k = k0 + u;
+          if (k < jb) {  // uniform
+            const T wk = lane_value(val, k);
+            const T nv = val - a[u] * wk;
+            val = (tid > k && tid < jb) ? nv : val;
+          }
+        }
       }
       if (tid < jb) w[j0 + tid] = val;
     }
@@ -267,10 +290,25 @@ mf_trsv(const T* __restrict__ buf, T* __restrict__ W, const MfFront* __restrict_
     __syncthreads();
     if (tid < kDiag) {
       T val = tid < jb ? w[j0 + tid] : T(0);
-      for (int k = jb - 1; k >= 0; k--) {
-        if (tid == k) val = val / sD[k * (kDiag + 1) + k];
-        const T wk = __shfl(val, k);
-        if (tid < k) val -= sD[k * (kDiag + 1) + tid] * wk;
+      for (int k0 = jb - 1; k0 >= 0; k0 -= 8) {
+        T a[8], dg[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int kk = max(k0 - u, 0);
+          a[u] = sD[kk * (kDiag + 1) + tid];
+          dg[u] = sD[kk * (kDiag + 1) + kk];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int k = k0 - u;
+          if (k >= 0) {  // uniform
+            // lane k's value over its pivot, formed on every lane from the
+            // same two operands (the same bits as lane k's own division)
+            const T wk = lane_value(val, k) / dg[u];
+            const T nv = val - a[u] * wk;
+            val = tid == k ? wk : (tid < k ? nv : val);
+          }
+        }
       }
       if (tid < jb) w[j0 + tid] = val;
     }
