@@ -266,3 +266,60 @@ def test_bordered_band_lu_solves_as_dense_and_falls_back():
     assert isinstance(inf._lu, _BorderedBandLU)
     r = inf.residual(inf.z, 0.5)
     assert np.abs(r).max() < 1e-11
+
+
+def test_cached_dirichlet_rows_equal_a_direct_build():
+    """cylinder_dirichlet keeps the rows that do not depend on Wi on the mesh:
+    they equal the boundary scan done afresh, for the closed-form and the
+    fully developed inlet, and a mesh given a new coordinate array is scanned
+    again."""
+    from ve_flow_around_cylinder_fortran_amd import cylinder_channel
+    from ve_flow_around_cylinder_fortran_amd.mesh import MESH_TOLERANCE
+    from ve_flow_around_cylinder_fortran_amd.newton import cylinder_dirichlet
+
+    def direct(mesh, prof, inlet):
+        g = mesh.geometry
+        x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+        rows = [6 * inlet + q for q in (0, 1, 3, 4, 5)]
+        vals = list(prof)
+        outlet = np.nonzero(np.abs(x - g["L_down"]) < MESH_TOLERANCE)[0]
+        rows.append(6 * outlet + 2)
+        vals.append(np.zeros(len(outlet)))
+        for on in (np.abs(y + g["H"]) < MESH_TOLERANCE, np.abs(y - g["H"]) < MESH_TOLERANCE,
+                   np.abs(x * x + y * y - g["R"] * g["R"]) < MESH_TOLERANCE):
+            nodes = np.nonzero(on)[0]
+            for q in (0, 1):
+                rows.append(6 * nodes + q)
+                vals.append(np.zeros(len(nodes)))
+        return np.concatenate(rows), np.concatenate(vals)
+
+    from ve_flow_around_cylinder_fortran_amd.newton import oldroyd_b_inflow
+
+    for dims in ((2, 1, 1, 2), (8, 4, 6, 8)):
+        mesh = cylinder_channel(*dims)
+        g = mesh.geometry
+        inlet = np.nonzero(np.abs(mesh.coords[:, 0] + g["L_up"]) < MESH_TOLERANCE)[0]
+        for _ in range(2):  # the second call takes the kept rows
+            r, v = cylinder_dirichlet(mesh, 0.3)
+            rd, vd = direct(mesh, oldroyd_b_inflow(mesh.coords[inlet, 1], g["H"], 0.3), inlet)
+            assert np.array_equal(r, rd) and np.array_equal(v, vd)
+        prm = Params(BvN=0.59, VeN=0.25, model="L-PTT", DfN=1e-3)
+        inf = FullyDevelopedInflow.from_mesh(mesh, prm)
+        r, v = cylinder_dirichlet(mesh, 0.3, inf)
+        fresh = FullyDevelopedInflow.from_mesh(mesh, prm)  # the same iteration from the same start
+        prof, _ = fresh.solve(0.3)
+        rd, vd = direct(mesh, prof, fresh.nodes)
+        assert np.array_equal(r, rd) and np.array_equal(v, vd)
+    # a new coordinate array (the channel stretched downstream): scanned again
+    mesh = cylinder_channel(8, 4, 6, 8)
+    cylinder_dirichlet(mesh, 0.3)
+    c = mesh.coords.copy()
+    far = c[:, 0] > 0
+    c[far, 0] *= 1.5
+    mesh.coords = c
+    mesh.geometry = dict(mesh.geometry, L_down=1.5 * mesh.geometry["L_down"])
+    g = mesh.geometry
+    inlet = np.nonzero(np.abs(mesh.coords[:, 0] + g["L_up"]) < MESH_TOLERANCE)[0]
+    r, v = cylinder_dirichlet(mesh, 0.3)
+    rd, vd = direct(mesh, oldroyd_b_inflow(mesh.coords[inlet, 1], g["H"], 0.3), inlet)
+    assert np.array_equal(r, rd) and np.array_equal(v, vd)

@@ -186,6 +186,11 @@ class NewtonSolver:
     the same iterate.  Everything else (residual, update, convergence test,
     the growth and contraction rules) is as with reuse_factors.
 
+    refine_tol (FP32 factorisations, 'mf' or 'band'): the relative residual to
+    which GMRES refines a solve that misses it (1e-3 when None); on the
+    40,000-element continuation 1e-2 and 1e-1 saved no time, the stale
+    factorisation's corrections then costing Newton iterations instead.
+
     Escalation (reuse_factors with an FP32 factorisation or the FP32
     Jacobian): after an emergency restart, the rest of that solve uses the
     reference's ingredients, the FP64 forward-difference Jacobian and an FP64
